@@ -1,0 +1,348 @@
+"""ibfs_amd: MI355X-native concurrent multi-source BFS (iBFS operator surface).
+
+Python host over the C ABI of libibfs.so (include/ibfs.h).  The classes
+`graph` and `gpu_ibfs` mirror the reference's host interface (graph.h,
+gpu_ibfs.cuh); `HostCsr`, `DeviceGraph` and `Engine` are the general API.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import (IbfsError, OUT_DEVICE, OUT_HOST, POLICY_ALPHA_BETA, POLICY_BU_ONLY,
+                   POLICY_FIXED, POLICY_TD_ONLY, Config, LevelInfo, Stats, check, lib)
+
+__all__ = [
+    "IbfsError", "HostCsr", "DeviceGraph", "Engine", "DeviceBuffer", "graph", "gpu_ibfs",
+    "default_config", "device_count", "POLICY_FIXED", "POLICY_ALPHA_BETA", "POLICY_TD_ONLY",
+    "POLICY_BU_ONLY",
+]
+
+
+def _i32p(a: np.ndarray):
+    return a.ctypes.data_as(_lib.I32P)
+
+
+def _i64p(a: np.ndarray):
+    return a.ctypes.data_as(_lib.I64P)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(lib().ibfs_device_count(C.byref(n)))
+    return n.value
+
+
+def default_config(**overrides) -> Config:
+    cfg = Config()
+    check(lib().ibfs_config_default(C.byref(cfg)))
+    for k, v in overrides.items():
+        if not hasattr(cfg, k):
+            raise KeyError(f"unknown config field {k}")
+        setattr(cfg, k, v)
+    return cfg
+
+
+class HostCsr:
+    """Host CSR: int64 beg_pos[V+1], int32 adjacency[E] (owned by libibfs)."""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+
+    @classmethod
+    def from_arrays(cls, beg_pos, adj) -> "HostCsr":
+        beg = np.ascontiguousarray(beg_pos, dtype=np.int64)
+        a = np.ascontiguousarray(adj, dtype=np.int32)
+        h = C.c_void_p()
+        check(lib().ibfs_hostcsr_create(_i64p(beg), len(beg) - 1, _i32p(a), len(a), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def load(cls, beg_file: str, csr_file: str) -> "HostCsr":
+        h = C.c_void_p()
+        check(lib().ibfs_hostcsr_load(os.fsencode(beg_file), os.fsencode(csr_file), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def rmat(cls, scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19,
+             c: float = 0.19, seed: int = 1, permute: bool = True, threads: int = 0) -> "HostCsr":
+        h = C.c_void_p()
+        check(lib().ibfs_gen_rmat(scale, edge_factor, a, b, c, seed, int(permute), threads,
+                                  C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def chung_lu(cls, vert_count: int, gamma: float = 2.2, mean_degree: float = 76.0,
+                 max_degree: float = 33000.0, seed: int = 1, threads: int = 0) -> "HostCsr":
+        h = C.c_void_p()
+        check(lib().ibfs_gen_chung_lu(vert_count, gamma, mean_degree, max_degree, seed, threads,
+                                      C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_text(cls, path: str, reverse: bool = True, skip_lines: int = 0) -> "HostCsr":
+        h = C.c_void_p()
+        check(lib().ibfs_convert_text(os.fsencode(path), int(reverse), skip_lines, C.byref(h)))
+        return cls(h.value)
+
+    def save(self, beg_file: str, csr_file: str) -> None:
+        check(lib().ibfs_hostcsr_save(self._h, os.fsencode(beg_file), os.fsencode(csr_file)))
+
+    @property
+    def vert_count(self) -> int:
+        v, e = C.c_int64(), C.c_int64()
+        check(lib().ibfs_hostcsr_info(self._h, C.byref(v), C.byref(e)))
+        return v.value
+
+    @property
+    def edge_count(self) -> int:
+        v, e = C.c_int64(), C.c_int64()
+        check(lib().ibfs_hostcsr_info(self._h, C.byref(v), C.byref(e)))
+        return e.value
+
+    def arrays(self) -> tuple[np.ndarray, np.ndarray]:
+        """Copies of (beg_pos int64[V+1], adj int32[E])."""
+        V, E = self.vert_count, self.edge_count
+        bp, ap = _lib.I64P(), _lib.I32P()
+        check(lib().ibfs_hostcsr_data(self._h, C.byref(bp), C.byref(ap)))
+        beg = np.ctypeslib.as_array(bp, shape=(V + 1,)).copy()
+        adj = np.ctypeslib.as_array(ap, shape=(max(E, 1),))[:E].copy() if E else np.zeros(0, np.int32)
+        return beg, adj
+
+    def gen_sources(self, count: int, seed: int = 2) -> np.ndarray:
+        out = np.zeros(count, dtype=np.int32)
+        check(lib().ibfs_gen_sources(self._h, count, seed, _i32p(out)))
+        return out
+
+    def traversed_edges(self, depth: np.ndarray) -> int:
+        d = np.ascontiguousarray(depth, dtype=np.int32)
+        e = C.c_int64()
+        check(lib().ibfs_traversed_edges(self._h, _i32p(d), C.byref(e)))
+        return e.value
+
+    def close(self) -> None:
+        if self._h:
+            lib().ibfs_hostcsr_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    """Raw device allocation (for device-resident depth outputs)."""
+
+    def __init__(self, nbytes: int, device: int = 0):
+        p = C.c_void_p()
+        check(lib().ibfs_device_alloc(device, nbytes, C.byref(p)))
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    def to_host(self, dtype=np.int32, count: Optional[int] = None) -> np.ndarray:
+        n = self.nbytes // np.dtype(dtype).itemsize if count is None else count
+        out = np.empty(n, dtype=dtype)
+        check(lib().ibfs_memcpy_d2h(out.ctypes.data, C.c_void_p(self.ptr), out.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().ibfs_device_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceGraph:
+    """A CSR resident in device memory (read-only, shareable by engines)."""
+
+    def __init__(self, csr: HostCsr, device: int = 0):
+        h = C.c_void_p()
+        check(lib().ibfs_graph_from_host(csr._h, device, C.byref(h)))
+        self._h = h
+        self.device = device
+        self.vert_count = csr.vert_count
+        self.edge_count = csr.edge_count
+
+    @classmethod
+    def load_files(cls, beg_file: str, csr_file: str, device: int = 0) -> "DeviceGraph":
+        return cls(HostCsr.load(beg_file, csr_file), device)
+
+    def groupby(self, src_list) -> np.ndarray:
+        """GPU GroupBy: the permutation of src_list of groupby.cu:36-149."""
+        s = np.ascontiguousarray(src_list, dtype=np.int32)
+        out = np.zeros_like(s)
+        check(lib().ibfs_groupby(self._h, _i32p(s), len(s), _i32p(out)))
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            lib().ibfs_graph_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Engine:
+    """One traversal engine (HIP stream, status ring, queues) over a DeviceGraph."""
+
+    def __init__(self, g: DeviceGraph, cfg: Optional[Config] = None, **overrides):
+        if cfg is None:
+            cfg = default_config(device=g.device, **overrides)
+        elif overrides:
+            for k, v in overrides.items():
+                setattr(cfg, k, v)
+        h = C.c_void_p()
+        check(lib().ibfs_engine_create(g._h, C.byref(cfg), C.byref(h)))
+        self._h = h
+        self.graph = g
+        self.cfg = cfg
+        self.last_stats = Stats()
+
+    @property
+    def lane_bits(self) -> int:
+        return self.cfg.lane_bits
+
+    def traversal(self, sources, out: Optional[DeviceBuffer] = None) -> Optional[np.ndarray]:
+        """Runs one group (<= lane_bits sources).  Returns depth int32[k][V] on
+        the host, or writes into `out` (device) and returns None."""
+        s = np.ascontiguousarray(sources, dtype=np.int32)
+        k, V = len(s), self.graph.vert_count
+        st = Stats()
+        if out is None:
+            depth = np.empty((k, V), dtype=np.int32)
+            check(lib().ibfs_traversal(self._h, _i32p(s), k, depth.ctypes.data, OUT_HOST,
+                                       C.byref(st)))
+            self.last_stats = st
+            return depth
+        if out.nbytes < 4 * k * V:
+            raise ValueError("device buffer too small")
+        check(lib().ibfs_traversal(self._h, _i32p(s), k, C.c_void_p(out.ptr), OUT_DEVICE,
+                                   C.byref(st)))
+        self.last_stats = st
+        return None
+
+    def run_groups(self, sources, out: Optional[DeviceBuffer] = None) -> Optional[np.ndarray]:
+        s = np.ascontiguousarray(sources, dtype=np.int32)
+        n, V = len(s), self.graph.vert_count
+        st = Stats()
+        if out is None:
+            depth = np.empty((n, V), dtype=np.int32)
+            check(lib().ibfs_run_groups(self._h, _i32p(s), n, depth.ctypes.data, OUT_HOST,
+                                        C.byref(st)))
+            self.last_stats = st
+            return depth
+        if out.nbytes < 4 * n * V:
+            raise ValueError("device buffer too small")
+        check(lib().ibfs_run_groups(self._h, _i32p(s), n, C.c_void_p(out.ptr), OUT_DEVICE,
+                                    C.byref(st)))
+        self.last_stats = st
+        return None
+
+    def level_log(self) -> list[dict]:
+        n = C.c_int32()
+        check(lib().ibfs_engine_level_log(self._h, None, 0, C.byref(n)))
+        arr = (LevelInfo * max(n.value, 1))()
+        check(lib().ibfs_engine_level_log(self._h, arr, n.value, C.byref(n)))
+        out = []
+        for i in range(n.value):
+            li = arr[i]
+            out.append({
+                "level": li.level, "direction": li.direction, "n_changed": li.n_changed,
+                "n_frontier": li.n_frontier, "m_frontier": li.m_frontier,
+                "n_unvisited": li.n_unvisited, "m_unvisited": li.m_unvisited,
+                "q_td": list(li.q_td), "q_bu": list(li.q_bu),
+                "bu_work": li.bu_work, "bu_prefix": li.bu_prefix,
+                "expand_ms": li.expand_ms, "inspect_ms": li.inspect_ms,
+            })
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            lib().ibfs_engine_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Mirror of the reference's host classes (graph.h / graph.cu, gpu_ibfs.cuh).
+
+class graph:
+    """graph(beg_file, csr_file, src_count): loads the two-file CSR
+    (graph.cu:63-119).  gen_src() draws src_count seeded distinct sources of
+    degree > 0 (replacing graph.cu:121-148); groupby() reorders src_list with
+    the GPU GroupBy kernel (groupby.cu:36-149)."""
+
+    def __init__(self, beg_file: Optional[str] = None, csr_file: Optional[str] = None,
+                 src_count: int = 0, csr: Optional[HostCsr] = None):
+        self.host = csr if csr is not None else HostCsr.load(beg_file, csr_file)
+        self.src_count = src_count
+        self.vert_count = self.host.vert_count
+        self.edge_count = self.host.edge_count
+        self.beg_pos, self.csr = self.host.arrays()
+        self.src_list: Optional[np.ndarray] = None
+        self._dev: Optional[DeviceGraph] = None
+
+    def device_graph(self, device: int = 0) -> DeviceGraph:
+        if self._dev is None:
+            self._dev = DeviceGraph(self.host, device)
+        return self._dev
+
+    def gen_src(self, seed: int = 2) -> np.ndarray:
+        self.src_list = self.host.gen_sources(self.src_count, seed)
+        return self.src_list
+
+    def groupby(self, device: int = 0) -> np.ndarray:
+        if self.src_list is None:
+            raise IbfsError(_lib.IBFS_ERR_STATE, "groupby() before gen_src()")
+        self.src_list = self.device_graph(device).groupby(self.src_list)
+        return self.src_list
+
+
+class gpu_ibfs:
+    """gpu_ibfs(g, concurr_count, sw_level, gpu_count, sml_shed, lrg_shed)
+    (gpu_ibfs.cu:36-45).  traversal(grp_beg) runs the group of concurr_count
+    sources starting at g.src_list[grp_beg] (traversal.cu:78) and returns
+    their depth arrays int32[concurr_count][V] (-1 unreached); the reference
+    leaves them as device status words.  sw_level >= 0 selects the
+    reference's fixed switch (top-down while level <= sw_level); pass
+    sw_level=None for the alpha/beta policy."""
+
+    def __init__(self, g: graph, concurr_count: int = 128, sw_level: Optional[int] = None,
+                 gpu_count: int = 1, sml_shed: int = 32, lrg_shed: int = 512, device: int = 0,
+                 **overrides):
+        if gpu_count != 1:
+            raise IbfsError(_lib.IBFS_ERR_ARG, "one GPU per engine; shard sources across engines")
+        if concurr_count not in (64, 128, 256):
+            raise IbfsError(_lib.IBFS_ERR_ARG, "concurr_count must be 64, 128 or 256")
+        self.g = g
+        self.concurr_count = concurr_count
+        policy = POLICY_ALPHA_BETA if sw_level is None else POLICY_FIXED
+        cfg = default_config(lane_bits=concurr_count, sml_shed=sml_shed, lrg_shed=lrg_shed,
+                             policy=policy, sw_level=sw_level or 0, device=device, **overrides)
+        self.engine = Engine(g.device_graph(device), cfg)
+
+    def traversal(self, grp_beg: int) -> np.ndarray:
+        if self.g.src_list is None:
+            raise IbfsError(_lib.IBFS_ERR_STATE, "traversal() before gen_src()")
+        src = self.g.src_list[grp_beg:grp_beg + self.concurr_count]
+        return self.engine.traversal(src)

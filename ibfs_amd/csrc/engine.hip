@@ -1,0 +1,609 @@
+// Engine: device graph, per-group traversal driver (level loop, direction
+// switch, snapshot ring, depth emission) and GroupBy, behind the C ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+
+using namespace ibfs;
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess)                                                              \
+      return fail(IBFS_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                  __FILE__, __LINE__);                                                 \
+  } while (0)
+
+struct ibfs_graph_s {
+  int device = 0;
+  int64_t V = 0, E = 0;
+  int64_t *d_beg = nullptr;
+  int32_t *d_adj = nullptr;
+  int32_t *d_deg = nullptr;
+  int64_t max_deg = 0;
+};
+
+struct ibfs_engine_s {
+  ibfs_graph_s *g = nullptr;
+  ibfs_config cfg{};
+  int K = 128, W = 2, R = 32, chunk_shift = 12;
+  hipStream_t st = nullptr;
+  u64 *ring = nullptr;
+  LevelCtl *d_ctl = nullptr;
+  LevelCtl *h_ctl = nullptr;  // pinned
+  Queues q{};
+  u64 *d_masks = nullptr;     // [0,W) lanes_all, [W,2W) sel, [2W, 2W+(R+1)W) level masks
+  u64 *h_masks = nullptr;     // pinned mirror
+  int32_t *d_src = nullptr;
+  int32_t *h_src = nullptr;   // pinned, K entries
+  LaneOut *d_lanes = nullptr;
+  LaneOut *h_lanes = nullptr; // pinned, K entries
+  int32_t *d_out = nullptr;   // staging for host-placed output
+  size_t d_out_cap = 0;
+  u64 *d_count = nullptr;
+  u64 *h_count = nullptr;
+  std::vector<ibfs_level_info> log;
+  std::vector<hipEvent_t> ev;  // event pool
+  size_t ev_used = 0;
+};
+
+namespace {
+
+size_t masks_words(const ibfs_engine_s *e) { return (size_t)(2 + (e->R + 1)) * e->W; }
+
+hipEvent_t next_event(ibfs_engine_s *e) {
+  if (e->ev_used == e->ev.size()) {
+    hipEvent_t x;
+    if (hipEventCreate(&x) != hipSuccess) return nullptr;
+    e->ev.push_back(x);
+  }
+  return e->ev[e->ev_used++];
+}
+
+int check_device(int dev) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return fail(IBFS_ERR_NODEVICE, "no HIP device available");
+  if (dev < 0 || dev >= n) return fail(IBFS_ERR_ARG, "device %d out of range [0,%d)", dev, n);
+  if (hipSetDevice(dev) != hipSuccess) return fail(IBFS_ERR_HIP, "hipSetDevice(%d) failed", dev);
+  return IBFS_OK;
+}
+
+template <class T>
+void dfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+template <class T>
+void hfree(T *&p) {
+  if (p) (void)hipHostFree((void *)p);
+  p = nullptr;
+}
+
+void engine_release(ibfs_engine_s *e) {
+  if (!e) return;
+  if (e->g) (void)hipSetDevice(e->g->device);
+  if (e->st) (void)hipStreamSynchronize(e->st);
+  dfree(e->ring);
+  dfree(e->d_ctl);
+  hfree(e->h_ctl);
+  for (int i = 0; i < 6; ++i) dfree(e->q.q[i]);
+  dfree(e->q.chunks);
+  dfree(e->d_masks);
+  hfree(e->h_masks);
+  dfree(e->d_src);
+  hfree(e->h_src);
+  dfree(e->d_lanes);
+  hfree(e->h_lanes);
+  dfree(e->d_out);
+  dfree(e->d_count);
+  hfree(e->h_count);
+  for (auto x : e->ev) (void)hipEventDestroy(x);
+  e->ev.clear();
+  if (e->st) (void)hipStreamDestroy(e->st);
+  e->st = nullptr;
+}
+
+u64 *slot(ibfs_engine_s *e, int L) {
+  return e->ring + (size_t)(L % e->R) * (size_t)e->g->V * (size_t)e->W;
+}
+
+// direction of level L (0 top-down, 1 bottom-up) from the counters of L-1
+int decide(const ibfs_engine_s *e, int L, const LevelCtl &c, int dir_prev, u64 nf_prev) {
+  switch (e->cfg.policy) {
+    case IBFS_POLICY_TD_ONLY: return 0;
+    case IBFS_POLICY_BU_ONLY: return 1;
+    case IBFS_POLICY_FIXED: return L <= e->cfg.sw_level ? 0 : 1;
+    default: break;
+  }
+  if (dir_prev != 1) return (c.m_front * (u64)e->cfg.alpha > c.m_cand) ? 1 : 0;
+  const bool small = c.n_front * (u64)e->cfg.beta < (u64)e->g->V;
+  return (small && c.n_front < nf_prev) ? 0 : 1;
+}
+
+// Emits the window [b, en] for the lanes in e->h_lanes (already on device).
+int emit_window(ibfs_engine_s *e, int b, int en, int k, ibfs_stats *stats) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (e->cfg.kernel_timing) {
+    e0 = next_event(e);
+    HIP_TRY(hipEventRecord(e0, e->st));
+  }
+  HIP_TRY(launch_emit(e->W, e->ring, e->R, e->g->V, b, en, e->d_lanes, e->d_masks + 2 * e->W,
+                      e->d_masks + e->W, k, e->st));
+  if (e->cfg.kernel_timing) {
+    e1 = next_event(e);
+    HIP_TRY(hipEventRecord(e1, e->st));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    if (stats) stats->emit_ms += ms;
+  }
+  if (stats) stats->n_emit_launches++;
+  return IBFS_OK;
+}
+
+// One group of k sources (lanes 0..k-1), depths into dout (device, k*V).
+int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_stats *stats) {
+  ibfs_graph_s *g = e->g;
+  const int64_t V = g->V;
+  const int W = e->W;
+  e->ev_used = 0;
+  e->log.clear();
+
+  // lanes, masks, sources (pinned staging -> device)
+  std::memset(e->h_masks, 0, sizeof(u64) * masks_words(e));
+  for (int i = 0; i < k; ++i) {
+    e->h_masks[i >> 6] |= 1ull << (i & 63);
+    e->h_src[i] = src[i];
+    e->h_lanes[i].out = dout + (size_t)i * (size_t)V;
+    e->h_lanes[i].admit = 0;
+    e->h_lanes[i].pad = 0;
+  }
+  for (int w = 0; w < W; ++w) e->h_masks[W + w] = e->h_masks[w];
+  for (size_t i = 0; i < (size_t)(e->R + 1) * W; ++i) e->h_masks[2 * W + i] = ~0ull;
+  HIP_TRY(hipMemcpyAsync(e->d_masks, e->h_masks, sizeof(u64) * masks_words(e),
+                         hipMemcpyHostToDevice, e->st));
+  HIP_TRY(hipMemcpyAsync(e->d_src, e->h_src, sizeof(int32_t) * k, hipMemcpyHostToDevice, e->st));
+  HIP_TRY(hipMemcpyAsync(e->d_lanes, e->h_lanes, sizeof(LaneOut) * k, hipMemcpyHostToDevice,
+                         e->st));
+
+  const bool timing = e->cfg.kernel_timing != 0;
+  hipEvent_t t_begin = next_event(e);
+  HIP_TRY(hipEventRecord(t_begin, e->st));
+
+  // level 0: S_0 = source bits; inspection builds the level-1 queues
+  HIP_TRY(launch_fill_u64(slot(e, 0), V * W, 0ull, e->st));
+  HIP_TRY(launch_fill_u64(reinterpret_cast<u64 *>(e->d_ctl), (int64_t)e->R * 16, 0ull, e->st));
+  HIP_TRY(launch_init_sources(W, e->d_src, k, 0, slot(e, 0), e->st));
+  hipEvent_t ti0 = nullptr, ti1 = nullptr;
+  if (timing) { ti0 = next_event(e); HIP_TRY(hipEventRecord(ti0, e->st)); }
+  HIP_TRY(launch_inspect(W, nullptr, slot(e, 0), slot(e, 1), g->d_deg, V, e->d_masks, nullptr,
+                         nullptr, &e->d_ctl[0], &e->d_ctl[1 % e->R], e->q, e->cfg.sml_shed,
+                         e->cfg.lrg_shed, e->chunk_shift, e->st));
+  if (timing) { ti1 = next_event(e); HIP_TRY(hipEventRecord(ti1, e->st)); }
+  HIP_TRY(hipMemcpyAsync(e->h_ctl, &e->d_ctl[0], sizeof(LevelCtl), hipMemcpyDeviceToHost, e->st));
+  HIP_TRY(hipStreamSynchronize(e->st));
+  auto record = [&](int L, int dir, const LevelCtl &c, float exp_ms, float insp_ms) {
+    ibfs_level_info li{};
+    li.level = L;
+    li.direction = dir;
+    li.n_changed = (int64_t)c.n_changed;
+    li.n_frontier = (int64_t)c.n_front;
+    li.m_frontier = (int64_t)c.m_front;
+    li.n_unvisited = (int64_t)c.n_cand;
+    li.m_unvisited = (int64_t)c.m_cand;
+    for (int i = 0; i < 3; ++i) {
+      li.q_td[i] = c.q[i];
+      li.q_bu[i] = c.q[3 + i];
+    }
+    li.expand_ms = exp_ms;
+    li.inspect_ms = insp_ms;
+    e->log.push_back(li);
+  };
+  float insp0 = 0;
+  if (timing) {
+    HIP_TRY(hipEventElapsedTime(&insp0, ti0, ti1));
+    if (stats) stats->inspect_ms += insp0;
+  }
+  if (stats) stats->n_inspect_launches++;
+  record(0, -1, *e->h_ctl, 0.f, insp0);
+
+  int base = 0, last = 0, dir_prev = -1;
+  u64 nf_prev = ~0ull;
+  for (int L = 1;; ++L) {
+    const LevelCtl c = *e->h_ctl;  // counters of level L-1
+    if (c.n_changed == 0) {        // only possible at L == 1 with no sources
+      last = L - 1;
+      break;
+    }
+    if (L > 1 << 20) return fail(IBFS_ERR_STATE, "level limit exceeded");
+    const int dir = decide(e, L, c, dir_prev, nf_prev);
+    // lam of level L-1 is the first member of its control block
+    const u64 *lamp = reinterpret_cast<const u64 *>(e->d_ctl + (L - 1) % e->R);
+    hipEvent_t x0 = nullptr, x1 = nullptr, y1 = nullptr;
+    if (timing) { x0 = next_event(e); HIP_TRY(hipEventRecord(x0, e->st)); }
+    if (dir == 0) {
+      HIP_TRY(launch_expand_td(W, g->d_beg, g->d_adj, slot(e, L - 1), slot(e, L), lamp, e->q,
+                               c.q[Q_TD_S], c.q[Q_TD_M], c.q[Q_TD_L], e->chunk_shift, e->st));
+      if (stats) stats->n_td_launches++;
+    } else {
+      if (e->cfg.work_model) {  // instrumentation: exact early-termination prefixes
+        HIP_TRY(hipMemsetAsync(e->d_count + 1, 0, 2 * sizeof(u64), e->st));
+        HIP_TRY(launch_bu_model(W, g->d_beg, g->d_adj, slot(e, L - 1), lamp, V, e->d_count + 1,
+                                e->st));
+        HIP_TRY(hipMemcpyAsync(e->h_count + 1, e->d_count + 1, 2 * sizeof(u64),
+                               hipMemcpyDeviceToHost, e->st));
+        if (timing) HIP_TRY(hipEventRecord(x0, e->st));
+      }
+      HIP_TRY(launch_expand_bu(W, g->d_beg, g->d_adj, slot(e, L - 1), slot(e, L), lamp, e->q,
+                               c.q[Q_BU_S], c.q[Q_BU_M], c.q[Q_BU_L], e->st));
+      if (stats) stats->n_bu_launches++;
+    }
+    if (timing) { x1 = next_event(e); HIP_TRY(hipEventRecord(x1, e->st)); }
+    if ((L + 1) - base >= e->R) {  // the next copy would overwrite S_base
+      int rc = emit_window(e, base, L, k, stats);
+      if (rc) return rc;
+      if (stats) stats->emit_windows++;
+      base = L;
+    }
+    hipEvent_t y0 = nullptr;
+    if (timing) { y0 = next_event(e); HIP_TRY(hipEventRecord(y0, e->st)); }
+    HIP_TRY(launch_inspect(W, slot(e, L - 1), slot(e, L), slot(e, L + 1), g->d_deg, V, lamp,
+                           nullptr, nullptr, &e->d_ctl[L % e->R], &e->d_ctl[(L + 1) % e->R], e->q,
+                           e->cfg.sml_shed, e->cfg.lrg_shed, e->chunk_shift, e->st));
+    if (timing) { y1 = next_event(e); HIP_TRY(hipEventRecord(y1, e->st)); }
+    HIP_TRY(hipMemcpyAsync(e->h_ctl, &e->d_ctl[L % e->R], sizeof(LevelCtl),
+                           hipMemcpyDeviceToHost, e->st));
+    HIP_TRY(hipStreamSynchronize(e->st));
+    float ems = 0, ims = 0;
+    if (timing) {
+      HIP_TRY(hipEventElapsedTime(&ems, x0, x1));
+      HIP_TRY(hipEventElapsedTime(&ims, y0, y1));
+      if (stats) {
+        (dir == 0 ? stats->expand_td_ms : stats->expand_bu_ms) += ems;
+        stats->inspect_ms += ims;
+      }
+    }
+    if (stats) stats->n_inspect_launches++;
+    record(L, dir, *e->h_ctl, ems, ims);
+    if (e->cfg.work_model && dir == 1) {
+      e->log.back().bu_work = (int64_t)e->h_count[1];
+      e->log.back().bu_prefix = (int64_t)e->h_count[2];
+    }
+    dir_prev = dir;
+    nf_prev = c.n_front;
+    if (e->h_ctl->n_changed == 0) {
+      last = L - 1;
+      break;
+    }
+  }
+  if (last >= base) {
+    int rc = emit_window(e, base, last, k, stats);
+    if (rc) return rc;
+    if (stats) stats->emit_windows++;
+  }
+  hipEvent_t t_end = next_event(e);
+  HIP_TRY(hipEventRecord(t_end, e->st));
+  HIP_TRY(hipEventSynchronize(t_end));
+  if (stats) {
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, t_begin, t_end));
+    stats->total_ms += ms;
+    stats->levels = std::max(stats->levels, last + 1);
+    // TEPS numerator over the final snapshot (outside the timed span)
+    HIP_TRY(hipMemsetAsync(e->d_count, 0, sizeof(u64), e->st));
+    HIP_TRY(launch_count_edges(W, slot(e, last), g->d_deg, V, e->d_masks, e->d_count, e->st));
+    HIP_TRY(hipMemcpyAsync(e->h_count, e->d_count, sizeof(u64), hipMemcpyDeviceToHost, e->st));
+    HIP_TRY(hipStreamSynchronize(e->st));
+    stats->traversed_edges += (int64_t)*e->h_count;
+  }
+  return IBFS_OK;
+}
+
+int ensure_out(ibfs_engine_s *e, size_t elems) {
+  if (e->d_out_cap >= elems) return IBFS_OK;
+  dfree(e->d_out);
+  e->d_out_cap = 0;
+  HIP_TRY(hipMalloc((void **)&e->d_out, elems * sizeof(int32_t)));
+  e->d_out_cap = elems;
+  return IBFS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ibfs_config_default(ibfs_config *cfg) {
+  if (!cfg) return fail(IBFS_ERR_ARG, "null config");
+  std::memset(cfg, 0, sizeof *cfg);
+  cfg->lane_bits = 128;
+  cfg->sml_shed = 32;
+  cfg->lrg_shed = 512;
+  cfg->policy = IBFS_POLICY_ALPHA_BETA;
+  cfg->sw_level = 2;
+  cfg->alpha = 14;
+  cfg->beta = 24;
+  cfg->wg_size = 256;
+  cfg->td_chunk = 4096;
+  cfg->ring_levels = 32;
+  cfg->kernel_timing = 0;
+  cfg->device = 0;
+  return IBFS_OK;
+}
+
+int ibfs_device_count(int *n) {
+  if (!n) return fail(IBFS_ERR_ARG, "null argument");
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  *n = c;
+  return IBFS_OK;
+}
+
+int ibfs_device_alloc(int device, size_t bytes, void **dptr) {
+  if (!dptr) return fail(IBFS_ERR_ARG, "null argument");
+  int rc = check_device(device);
+  if (rc) return rc;
+  if (hipMalloc(dptr, bytes ? bytes : 1) != hipSuccess)
+    return fail(IBFS_ERR_NOMEM, "hipMalloc(%zu) failed", bytes);
+  return IBFS_OK;
+}
+
+int ibfs_device_free(void *dptr) {
+  if (dptr) HIP_TRY(hipFree(dptr));
+  return IBFS_OK;
+}
+
+int ibfs_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return IBFS_OK;
+}
+
+int ibfs_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return IBFS_OK;
+}
+
+int ibfs_graph_create(const int64_t *beg, int64_t V, const int32_t *adj, int64_t E, int device,
+                      ibfs_graph_t *out) {
+  if (!out || !beg || V < 1 || E < 0 || (E > 0 && !adj))
+    return fail(IBFS_ERR_ARG, "ibfs_graph_create: bad arguments (V=%lld, E=%lld)", (long long)V,
+                (long long)E);
+  if (V > 0x7fffffffLL) return fail(IBFS_ERR_ARG, "vertex count exceeds int32 ids");
+  if (beg[0] != 0 || beg[V] != E)
+    return fail(IBFS_ERR_FORMAT, "beg_pos[0]=%lld, beg_pos[V]=%lld, expected 0 and %lld",
+                (long long)beg[0], (long long)beg[V], (long long)E);
+  std::vector<int32_t> deg((size_t)V);
+  int64_t maxd = 0;
+  for (int64_t v = 0; v < V; ++v) {
+    const int64_t d = beg[v + 1] - beg[v];
+    if (d < 0) return fail(IBFS_ERR_FORMAT, "beg_pos decreases at vertex %lld", (long long)v);
+    if (d > 0x7fffffffLL) return fail(IBFS_ERR_FORMAT, "degree exceeds int32");
+    deg[v] = (int32_t)d;
+    maxd = std::max(maxd, d);
+  }
+  for (int64_t i = 0; i < E; ++i)
+    if (adj[i] < 0 || adj[i] >= V)
+      return fail(IBFS_ERR_FORMAT, "adjacency entry %lld = %d out of range", (long long)i, adj[i]);
+  int rc = check_device(device);
+  if (rc) return rc;
+  auto *g = new ibfs_graph_s;
+  g->device = device;
+  g->V = V;
+  g->E = E;
+  g->max_deg = maxd;
+  hipError_t err = hipMalloc((void **)&g->d_beg, sizeof(int64_t) * (size_t)(V + 1));
+  if (err == hipSuccess) err = hipMalloc((void **)&g->d_adj, sizeof(int32_t) * (size_t)std::max<int64_t>(E, 1));
+  if (err == hipSuccess) err = hipMalloc((void **)&g->d_deg, sizeof(int32_t) * (size_t)V);
+  if (err == hipSuccess) err = hipMemcpy(g->d_beg, beg, sizeof(int64_t) * (size_t)(V + 1), hipMemcpyHostToDevice);
+  if (err == hipSuccess && E > 0) err = hipMemcpy(g->d_adj, adj, sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(g->d_deg, deg.data(), sizeof(int32_t) * (size_t)V, hipMemcpyHostToDevice);
+  if (err != hipSuccess) {
+    dfree(g->d_beg);
+    dfree(g->d_adj);
+    dfree(g->d_deg);
+    delete g;
+    return fail(err == hipErrorOutOfMemory ? IBFS_ERR_NOMEM : IBFS_ERR_HIP,
+                "graph upload failed: %s", hipGetErrorString(err));
+  }
+  *out = g;
+  return IBFS_OK;
+}
+
+int ibfs_graph_from_host(ibfs_hostcsr_t h, int device, ibfs_graph_t *out) {
+  if (!h) return fail(IBFS_ERR_ARG, "null host csr");
+  return ibfs_graph_create(h->csr.beg.data(), h->csr.V, h->csr.adj.data(), h->csr.E, device, out);
+}
+
+int ibfs_graph_load_files(const char *beg_path, const char *csr_path, int device,
+                          ibfs_graph_t *out) {
+  ibfs_hostcsr_t h = nullptr;
+  int rc = ibfs_hostcsr_load(beg_path, csr_path, &h);
+  if (rc) return rc;
+  rc = ibfs_graph_from_host(h, device, out);
+  ibfs_hostcsr_free(h);
+  return rc;
+}
+
+int ibfs_graph_info(ibfs_graph_t g, int64_t *V, int64_t *E) {
+  if (!g) return fail(IBFS_ERR_ARG, "null graph");
+  if (V) *V = g->V;
+  if (E) *E = g->E;
+  return IBFS_OK;
+}
+
+int ibfs_graph_destroy(ibfs_graph_t g) {
+  if (!g) return IBFS_OK;
+  (void)hipSetDevice(g->device);
+  dfree(g->d_beg);
+  dfree(g->d_adj);
+  dfree(g->d_deg);
+  delete g;
+  return IBFS_OK;
+}
+
+int ibfs_groupby(ibfs_graph_t g, const int32_t *src, int64_t n, int32_t *out) {
+  if (!g || (n > 0 && (!src || !out)) || n < 0) return fail(IBFS_ERR_ARG, "ibfs_groupby: bad arguments");
+  if (n == 0) return IBFS_OK;
+  for (int64_t i = 0; i < n; ++i)
+    if (src[i] < 0 || src[i] >= g->V)
+      return fail(IBFS_ERR_ARG, "source %lld = %d out of range", (long long)i, src[i]);
+  int rc = check_device(g->device);
+  if (rc) return rc;
+  int32_t *d = nullptr;
+  HIP_TRY(hipMalloc((void **)&d, sizeof(int32_t) * (size_t)n * 3));
+  hipError_t err = hipMemcpy(d, src, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice);
+  if (err == hipSuccess)
+    err = launch_groupby(g->d_beg, g->d_adj, g->d_deg, d, n, d + n, d + 2 * n, nullptr);
+  if (err == hipSuccess)
+    err = hipMemcpy(out, d + 2 * n, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (err != hipSuccess) return fail(IBFS_ERR_HIP, "groupby failed: %s", hipGetErrorString(err));
+  return IBFS_OK;
+}
+
+int ibfs_engine_create(ibfs_graph_t g, const ibfs_config *cfg_in, ibfs_engine_t *out) {
+  if (!g || !out) return fail(IBFS_ERR_ARG, "ibfs_engine_create: null argument");
+  ibfs_config cfg;
+  if (cfg_in) cfg = *cfg_in;
+  else ibfs_config_default(&cfg);
+  if (cfg.lane_bits != 64 && cfg.lane_bits != 128 && cfg.lane_bits != 256)
+    return fail(IBFS_ERR_ARG, "lane_bits must be 64, 128 or 256 (got %d)", cfg.lane_bits);
+  if (cfg.sml_shed < 1 || cfg.lrg_shed < cfg.sml_shed)
+    return fail(IBFS_ERR_ARG, "need 1 <= sml_shed <= lrg_shed (got %d, %d)", cfg.sml_shed, cfg.lrg_shed);
+  if (cfg.policy < IBFS_POLICY_FIXED || cfg.policy > IBFS_POLICY_BU_ONLY)
+    return fail(IBFS_ERR_ARG, "unknown policy %d", cfg.policy);
+  if (cfg.policy == IBFS_POLICY_ALPHA_BETA && (cfg.alpha < 1 || cfg.beta < 1))
+    return fail(IBFS_ERR_ARG, "alpha and beta must be >= 1");
+  if (cfg.td_chunk < 256 || (cfg.td_chunk & (cfg.td_chunk - 1)))
+    return fail(IBFS_ERR_ARG, "td_chunk must be a power of two >= 256 (got %d)", cfg.td_chunk);
+  if (cfg.ring_levels < 4 || cfg.ring_levels > 255)
+    return fail(IBFS_ERR_ARG, "ring_levels must be in [4, 255] (got %d)", cfg.ring_levels);
+  if (cfg.device != g->device)
+    return fail(IBFS_ERR_ARG, "config device %d differs from the graph's device %d", cfg.device, g->device);
+  int rc = check_device(g->device);
+  if (rc) return rc;
+  auto *e = new ibfs_engine_s;
+  e->g = g;
+  e->cfg = cfg;
+  e->K = cfg.lane_bits;
+  e->W = cfg.lane_bits / 64;
+  e->chunk_shift = __builtin_ctz((unsigned)cfg.td_chunk);
+  const int64_t V = g->V;
+  // ring: at most 8 GiB of snapshots, at least 4 levels
+  const size_t slot_bytes = (size_t)V * (size_t)e->W * sizeof(u64);
+  int R = cfg.ring_levels;
+  const size_t budget = (size_t)8 << 30;
+  while (R > 4 && (size_t)R * slot_bytes > budget) --R;
+  e->R = R;
+  hipError_t err = hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking);
+  auto A = [&](void **p, size_t bytes) {
+    if (err == hipSuccess) err = hipMalloc(p, bytes ? bytes : 1);
+  };
+  auto H = [&](void **p, size_t bytes) {
+    if (err == hipSuccess) err = hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocDefault);
+  };
+  A((void **)&e->ring, (size_t)R * slot_bytes);
+  A((void **)&e->d_ctl, sizeof(LevelCtl) * (size_t)R);
+  H((void **)&e->h_ctl, sizeof(LevelCtl));
+  for (int i = 0; i < 6; ++i)
+    if (i != Q_TD_L) A((void **)&e->q.q[i], sizeof(int32_t) * (size_t)V);
+  const size_t nchunks = (size_t)V + (size_t)(g->E >> e->chunk_shift) + 2;
+  A((void **)&e->q.chunks, sizeof(int2) * nchunks);
+  A((void **)&e->d_masks, sizeof(u64) * masks_words(e));
+  H((void **)&e->h_masks, sizeof(u64) * masks_words(e));
+  A((void **)&e->d_src, sizeof(int32_t) * (size_t)e->K);
+  H((void **)&e->h_src, sizeof(int32_t) * (size_t)e->K);
+  A((void **)&e->d_lanes, sizeof(LaneOut) * (size_t)e->K);
+  H((void **)&e->h_lanes, sizeof(LaneOut) * (size_t)e->K);
+  A((void **)&e->d_count, 4 * sizeof(u64));
+  H((void **)&e->h_count, 4 * sizeof(u64));
+  if (err != hipSuccess) {
+    engine_release(e);
+    delete e;
+    return fail(err == hipErrorOutOfMemory ? IBFS_ERR_NOMEM : IBFS_ERR_HIP,
+                "engine allocation failed: %s", hipGetErrorString(err));
+  }
+  *out = e;
+  return IBFS_OK;
+}
+
+int ibfs_engine_destroy(ibfs_engine_t e) {
+  if (!e) return IBFS_OK;
+  engine_release(e);
+  delete e;
+  return IBFS_OK;
+}
+
+int ibfs_traversal(ibfs_engine_t e, const int32_t *src, int32_t k, int32_t *depth_out,
+                   int32_t out_place, ibfs_stats *stats) {
+  if (!e) return fail(IBFS_ERR_ARG, "null engine");
+  if (k < 0 || k > e->K) return fail(IBFS_ERR_ARG, "k=%d outside [0, %d]", k, e->K);
+  if (k > 0 && (!src || !depth_out)) return fail(IBFS_ERR_ARG, "null sources or output");
+  if (out_place != IBFS_OUT_HOST && out_place != IBFS_OUT_DEVICE)
+    return fail(IBFS_ERR_ARG, "bad out_place %d", out_place);
+  const int64_t V = e->g->V;
+  for (int i = 0; i < k; ++i)
+    if (src[i] < 0 || src[i] >= V)
+      return fail(IBFS_ERR_ARG, "source %d = %d out of range [0,%lld)", i, src[i], (long long)V);
+  if (stats) std::memset(stats, 0, sizeof *stats);
+  if (k == 0) return IBFS_OK;
+  int rc = check_device(e->g->device);
+  if (rc) return rc;
+  int32_t *dout = depth_out;
+  if (out_place == IBFS_OUT_HOST) {
+    rc = ensure_out(e, (size_t)k * (size_t)V);
+    if (rc) return rc;
+    dout = e->d_out;
+  }
+  rc = traverse(e, src, k, dout, stats);
+  if (rc) return rc;
+  if (out_place == IBFS_OUT_HOST)
+    HIP_TRY(hipMemcpy(depth_out, dout, sizeof(int32_t) * (size_t)k * (size_t)V,
+                      hipMemcpyDeviceToHost));
+  return IBFS_OK;
+}
+
+int ibfs_run_groups(ibfs_engine_t e, const int32_t *src, int64_t n, int32_t *depth_out,
+                    int32_t out_place, ibfs_stats *stats) {
+  if (!e || n < 0 || (n > 0 && (!src || !depth_out))) return fail(IBFS_ERR_ARG, "ibfs_run_groups: bad arguments");
+  if (stats) std::memset(stats, 0, sizeof *stats);
+  const int64_t V = e->g->V;
+  for (int64_t g0 = 0; g0 < n; g0 += e->K) {
+    const int k = (int)std::min<int64_t>(e->K, n - g0);
+    ibfs_stats s{};
+    int rc = ibfs_traversal(e, src + g0, k, depth_out + (size_t)g0 * (size_t)V, out_place,
+                            stats ? &s : nullptr);
+    if (rc) return rc;
+    if (stats) {
+      stats->levels = std::max(stats->levels, s.levels);
+      stats->emit_windows += s.emit_windows;
+      stats->traversed_edges += s.traversed_edges;
+      stats->total_ms += s.total_ms;
+      stats->init_ms += s.init_ms;
+      stats->expand_td_ms += s.expand_td_ms;
+      stats->expand_bu_ms += s.expand_bu_ms;
+      stats->inspect_ms += s.inspect_ms;
+      stats->emit_ms += s.emit_ms;
+      stats->n_td_launches += s.n_td_launches;
+      stats->n_bu_launches += s.n_bu_launches;
+      stats->n_inspect_launches += s.n_inspect_launches;
+      stats->n_emit_launches += s.n_emit_launches;
+    }
+  }
+  return IBFS_OK;
+}
+
+int ibfs_engine_level_log(ibfs_engine_t e, ibfs_level_info *out, int32_t cap, int32_t *n) {
+  if (!e || !n || cap < 0 || (cap > 0 && !out)) return fail(IBFS_ERR_ARG, "bad arguments");
+  *n = (int32_t)e->log.size();
+  for (int32_t i = 0; i < cap && i < *n; ++i) out[i] = e->log[i];
+  return IBFS_OK;
+}
+
+}  // extern "C"

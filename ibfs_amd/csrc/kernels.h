@@ -1,0 +1,72 @@
+// Device-side data layout and kernel launchers of libibfs (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ibfs {
+
+typedef unsigned long long u64;
+
+// Per-level control block, written by the inspector of level L with device
+// atomics and read by the expanders of level L+1 and by the host.
+struct alignas(128) LevelCtl {
+  u64 lam[4];          // lanes that gained a vertex at level L (OR of new bits)
+  u64 n_changed;       // vertices with new bits at L
+  u64 n_front;         // ... of degree > 0 (top-down frontier of L+1)
+  u64 m_front;         // sum of their degrees
+  u64 n_cand;          // bottom-up candidates for L+1
+  u64 m_cand;          // sum of their degrees
+  unsigned int q[6];   // queue lengths: td sml, td mid, td lrg chunks, bu sml, bu mid, bu lrg
+  unsigned int pad[2];
+};
+static_assert(sizeof(LevelCtl) == 128, "LevelCtl layout");
+
+enum { Q_TD_S = 0, Q_TD_M = 1, Q_TD_L = 2, Q_BU_S = 3, Q_BU_M = 4, Q_BU_L = 5 };
+
+// Output slot of one lane for the depth emission.
+struct LaneOut {
+  int32_t *out;   // depth array of V int32 for this lane (null: lane not emitted)
+  int32_t admit;  // level at which the lane's source was set (depth 0)
+  int32_t pad;
+};
+
+struct Queues {
+  int32_t *q[6];  // q[Q_TD_L] unused: top-down hubs use `chunks`
+  int2 *chunks;   // (vertex, chunk index) of top-down large vertices
+};
+
+// Launchers (all asynchronous on `st`).  W = status words (u64) per vertex.
+hipError_t launch_fill_u64(u64 *p, int64_t n, u64 value, hipStream_t st);
+hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t value, hipStream_t st);
+hipError_t launch_init_sources(int W, const int32_t *src, int k, int lane0, u64 *S0,
+                               hipStream_t st);
+hipError_t launch_inspect(int W, const u64 *prev, const u64 *cur, u64 *next,
+                          const int32_t *deg, int64_t V, const u64 *lam_prev,
+                          const u64 *prev_keep, const u64 *next_and, LevelCtl *ctl_out,
+                          LevelCtl *ctl_zero, Queues q, int sml, int lrg, int chunk_shift,
+                          hipStream_t st);
+hipError_t launch_expand_td(int W, const int64_t *beg, const int32_t *adj, const u64 *prev,
+                            u64 *cur, const u64 *lam_prev, Queues q, int64_t ns, int64_t nm,
+                            int64_t nl, int chunk_shift, hipStream_t st);
+hipError_t launch_expand_bu(int W, const int64_t *beg, const int32_t *adj, const u64 *prev,
+                            u64 *cur, const u64 *lam_prev, Queues q, int64_t ns, int64_t nm,
+                            int64_t nl, hipStream_t st);
+// Depth emission over the snapshot window [b, e] (ring slots L % R, each
+// V*W words).  level_masks[(L-b)*W + w] = lanes admitted at or before L.
+hipError_t launch_emit(int W, const u64 *ring, int R, int64_t V, int b, int e,
+                       const LaneOut *lanes, const u64 *level_masks, const u64 *sel,
+                       int k_lanes, hipStream_t st);
+hipError_t launch_count_edges(int W, const u64 *S, const int32_t *deg, int64_t V,
+                              const u64 *lanes, u64 *out, hipStream_t st);
+// Bottom-up work model of one level: out[0] += candidates with work,
+// out[1] += sum of their sequential early-termination prefixes.
+hipError_t launch_bu_model(int W, const int64_t *beg, const int32_t *adj, const u64 *prev,
+                           const u64 *lam_prev, int64_t V, u64 *out, hipStream_t st);
+// GroupBy (groupby.cu:36-149): key[i] = first neighbour of degree > 128 of a
+// source of degree <= 128, else -1; then the stable placement.
+hipError_t launch_groupby(const int64_t *beg, const int32_t *adj, const int32_t *deg,
+                          const int32_t *src, int64_t n, int32_t *key, int32_t *out,
+                          hipStream_t st);
+
+}  // namespace ibfs

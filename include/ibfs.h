@@ -1,0 +1,203 @@
+/*
+ * ibfs.h - C ABI of the MI355X concurrent multi-source BFS engine.
+ *
+ * Every entry point returns an int status (IBFS_OK == 0) and never exits the
+ * process; the message of the last failure on the calling thread is returned
+ * by ibfs_last_error().  Handles are opaque.  A graph handle is read-only once
+ * built and may be shared by several engines; an engine belongs to one host
+ * thread at a time and owns its HIP stream and device buffers.
+ *
+ * Reference interfaces each entry point replaces are cited as file:line of the
+ * iBFS sources (Gen B unless noted).
+ */
+#ifndef IBFS_H
+#define IBFS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IBFS_ABI_VERSION 1
+
+/* status codes */
+#define IBFS_OK 0
+#define IBFS_ERR_ARG -1      /* invalid argument (null, range, size)            */
+#define IBFS_ERR_IO -2       /* file missing, unreadable, truncated              */
+#define IBFS_ERR_FORMAT -3   /* file contents inconsistent (sizes, id range)     */
+#define IBFS_ERR_HIP -4      /* HIP runtime error                                */
+#define IBFS_ERR_NOMEM -5    /* device or host allocation failed / over budget   */
+#define IBFS_ERR_STATE -6    /* call not valid in the object's current state     */
+#define IBFS_ERR_NODEVICE -7 /* no HIP device available                          */
+
+/* direction policies (reference: fixed switch level, expander.cu:503-515) */
+#define IBFS_POLICY_FIXED 0      /* top-down while level <= sw_level, then bottom-up */
+#define IBFS_POLICY_ALPHA_BETA 1 /* Beamer-style switch on frontier/unvisited edges  */
+#define IBFS_POLICY_TD_ONLY 2
+#define IBFS_POLICY_BU_ONLY 3
+
+/* output placement for depth arrays */
+#define IBFS_OUT_HOST 0
+#define IBFS_OUT_DEVICE 1
+
+typedef struct ibfs_graph_s *ibfs_graph_t;
+typedef struct ibfs_engine_s *ibfs_engine_t;
+typedef struct ibfs_hostcsr_s *ibfs_hostcsr_t;
+typedef struct ibfs_server_s *ibfs_server_t;
+
+/* Engine configuration.  Defaults (ibfs_config_default) follow the reference:
+ * 128 lanes (concurrent-count, readme.md:27), sml_shed=32 and lrg_shed=512
+ * (main.cu:56-57). */
+typedef struct ibfs_config {
+  int32_t lane_bits;   /* 64, 128 or 256 source lanes per status word          */
+  int32_t sml_shed;    /* degree < sml_shed        -> thread class             */
+  int32_t lrg_shed;    /* degree > lrg_shed        -> workgroup class          */
+  int32_t policy;      /* IBFS_POLICY_*                                        */
+  int32_t sw_level;    /* last top-down level for IBFS_POLICY_FIXED (argv[4])   */
+  int32_t alpha;       /* TD->BU when m_f * alpha > m_u   (ALPHA_BETA)          */
+  int32_t beta;        /* BU->TD when n_f * beta < V      (ALPHA_BETA)          */
+  int32_t wg_size;     /* threads per workgroup of the expanders: 256/512/1024 */
+  int32_t td_chunk;    /* edges per workgroup chunk of a top-down hub          */
+  int32_t ring_levels; /* status snapshots kept on the device (>= 4)           */
+  int32_t kernel_timing; /* 1: time every kernel with HIP events             */
+  int32_t device;      /* HIP device ordinal                                   */
+  int32_t work_model;  /* 1: measure each bottom-up level's early-termination
+                          prefix into the level log (instrumentation)        */
+  int32_t reserved[3];
+} ibfs_config;
+
+/* Per-level record of the last traversal (the build's monitoring log, in place
+ * of ENABLE_MONITORING prints at traversal.cu:127-128). */
+typedef struct ibfs_level_info {
+  int32_t level;       /* level L: the pass that produced status_L             */
+  int32_t direction;   /* 0 top-down, 1 bottom-up, -1 init                     */
+  int64_t n_changed;   /* vertices whose status gained bits at L               */
+  int64_t n_frontier;  /* changed vertices with degree > 0                     */
+  int64_t m_frontier;  /* sum of degrees of the frontier                       */
+  int64_t n_unvisited; /* bottom-up candidates after L                         */
+  int64_t m_unvisited; /* sum of degrees of the candidates                     */
+  int64_t q_td[3];     /* top-down queue sizes (sml, mid, lrg chunks)          */
+  int64_t q_bu[3];     /* bottom-up queue sizes (sml, mid, lrg)                */
+  int64_t bu_work;     /* bottom-up level: candidates missing an active lane   */
+  int64_t bu_prefix;   /* ... sum of their early-termination prefixes (edges)  */
+  float expand_ms;     /* kernel time of the expansion producing status_L      */
+  float inspect_ms;    /* kernel time of the inspection of status_L            */
+} ibfs_level_info;
+
+typedef struct ibfs_stats {
+  int32_t levels;          /* depth of the deepest source + 1                 */
+  int32_t emit_windows;    /* depth-emission passes (1 unless the ring wrapped) */
+  int64_t traversed_edges; /* sum over lanes of deg(v) for reached v            */
+  double total_ms;         /* device time from first to last kernel (events)   */
+  double init_ms, expand_td_ms, expand_bu_ms, inspect_ms, emit_ms; /* timing=1 */
+  int32_t n_td_launches, n_bu_launches, n_inspect_launches, n_emit_launches;
+} ibfs_stats;
+
+/* ---- errors / version ---------------------------------------------------- */
+int ibfs_abi_version(void);
+/* Copies the last error message of the calling thread into buf.
+ * Replaces the print-and-exit of H_ERR (util.h:43-54, comm.h:5-16). */
+int ibfs_last_error(char *buf, size_t n);
+int ibfs_config_default(ibfs_config *cfg);
+int ibfs_device_count(int *n);
+
+/* ---- device memory helpers (callers without a GPU framework) ------------- */
+int ibfs_device_alloc(int device, size_t bytes, void **dptr);
+int ibfs_device_free(void *dptr);
+int ibfs_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int ibfs_memcpy_h2d(void *dst, const void *src, size_t bytes);
+
+/* ---- host CSR: reference file format, generators, converter -------------- */
+/* Reads <beg_pos.bin> (V+1 little-endian 8-byte integers) and <csr.bin>
+ * (E 8-byte integers), V and E derived from the file sizes and narrowed to
+ * int32 ids with a range check.  Replaces graph::graph (graph.cu:63-119). */
+int ibfs_hostcsr_load(const char *beg_path, const char *csr_path,
+                      ibfs_hostcsr_t *out);
+/* Writes the same two-file format (tuple_text_to_bin.cpp:149-178). */
+int ibfs_hostcsr_save(ibfs_hostcsr_t h, const char *beg_path,
+                      const char *csr_path);
+int ibfs_hostcsr_create(const int64_t *beg_pos, int64_t vert_count,
+                        const int32_t *adj, int64_t edge_count,
+                        ibfs_hostcsr_t *out);
+int ibfs_hostcsr_info(ibfs_hostcsr_t h, int64_t *vert_count,
+                      int64_t *edge_count);
+/* Borrowed views, valid until ibfs_hostcsr_free. */
+int ibfs_hostcsr_data(ibfs_hostcsr_t h, const int64_t **beg_pos,
+                      const int32_t **adj);
+int ibfs_hostcsr_free(ibfs_hostcsr_t h);
+/* Seeded undirected R-MAT (Graph500 a,b,c; d = 1-a-b-c), scale/edge factor,
+ * self-loops and duplicates removed, symmetrised, optional seeded vertex
+ * permutation, adjacency sorted ascending. No reference counterpart (the
+ * reference reads SNAP files, readme.md:36-42). */
+int ibfs_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b,
+                  double c, uint64_t seed, int32_t permute, int32_t nthreads,
+                  ibfs_hostcsr_t *out);
+/* Seeded Chung-Lu graph: expected degrees ~ power law with exponent gamma,
+ * rescaled to mean_degree and capped at max_degree; endpoints drawn with
+ * probability proportional to weight, cleaned like ibfs_gen_rmat. */
+int ibfs_gen_chung_lu(int64_t vert_count, double gamma, double mean_degree,
+                      double max_degree, uint64_t seed, int32_t nthreads,
+                      ibfs_hostcsr_t *out);
+/* Seeded distinct sources drawn uniformly from vertices with degree > 0
+ * (replaces the unseeded rand() rejection loop of graph::gen_src,
+ * graph.cu:121-148). */
+int ibfs_gen_sources(ibfs_hostcsr_t h, int64_t count, uint64_t seed,
+                     int32_t *out);
+/* Text edge list -> CSR with the converter's rules (tuple_text_to_bin.cpp:
+ * 56-272): skip `skip_lines` header lines, ids shifted by the minimum id,
+ * reverse=1 adds the reversed edge, no dedup. */
+int ibfs_convert_text(const char *text_path, int32_t reverse,
+                      int64_t skip_lines, ibfs_hostcsr_t *out);
+
+/* ---- device graph -------------------------------------------------------- */
+/* Uploads a CSR to the device (int64 beg_pos, int32 adjacency, int32 degree).
+ * Replaces the graph upload in the gpu_ibfs constructor (gpu_ibfs.cu:54-116). */
+int ibfs_graph_create(const int64_t *beg_pos, int64_t vert_count,
+                      const int32_t *adj, int64_t edge_count, int device,
+                      ibfs_graph_t *out);
+int ibfs_graph_from_host(ibfs_hostcsr_t h, int device, ibfs_graph_t *out);
+int ibfs_graph_load_files(const char *beg_path, const char *csr_path,
+                          int device, ibfs_graph_t *out);
+int ibfs_graph_info(ibfs_graph_t g, int64_t *vert_count, int64_t *edge_count);
+int ibfs_graph_destroy(ibfs_graph_t g);
+
+/* GroupBy on the device: a permutation of src_list in which sources of degree
+ * <= 128 are ordered by their first neighbour of degree > 128 (stable), the
+ * rest follow in input order.  Bit-identical to graph::groupby
+ * (groupby.cu:36-149). */
+int ibfs_groupby(ibfs_graph_t g, const int32_t *src_list, int64_t src_count,
+                 int32_t *out_list);
+
+/* ---- engine: one 128-lane (or 64/256) group per call --------------------- */
+/* Replaces gpu_ibfs::gpu_ibfs (gpu_ibfs.cu:36-117). */
+int ibfs_engine_create(ibfs_graph_t g, const ibfs_config *cfg,
+                       ibfs_engine_t *out);
+int ibfs_engine_destroy(ibfs_engine_t e);
+/* Runs one group of k <= lane_bits sources (lane i = src[i]; duplicates and
+ * zero-degree sources allowed) to completion and writes depth arrays
+ * depth_out[i*V + v] (int32, -1 = unreached).  out_place selects a host or a
+ * device pointer.  Replaces the per-group status reset (main.cu:79-84) plus
+ * gpu_ibfs::traversal (traversal.cu:78-136). */
+int ibfs_traversal(ibfs_engine_t e, const int32_t *src, int32_t k,
+                   int32_t *depth_out, int32_t out_place, ibfs_stats *stats);
+/* Level log of the last traversal. */
+int ibfs_engine_level_log(ibfs_engine_t e, ibfs_level_info *out, int32_t cap,
+                          int32_t *n_levels);
+
+/* Runs src_count sources as consecutive groups of lane_bits (the reference's
+ * main loop, main.cu:76-90); depth_out is [src_count][V]. */
+int ibfs_run_groups(ibfs_engine_t e, const int32_t *src, int64_t src_count,
+                    int32_t *depth_out, int32_t out_place, ibfs_stats *stats);
+
+/* TEPS numerator of one depth array: sum of deg(v) over reached v
+ * (validate.h:59-76 report()). depth is a host array. */
+int ibfs_traversed_edges(ibfs_hostcsr_t h, const int32_t *depth,
+                         int64_t *edges);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IBFS_H */

@@ -1,0 +1,188 @@
+"""Bit-exact parity of the HIP traversal (through the C ABI) against the CPU
+oracle and the golden fixtures.  Needs an MI355X."""
+import sys
+
+import numpy as np
+import pytest
+
+import ibfs_amd
+import oracle
+from conftest import GOLDEN, golden_bfs_names, load_golden, symmetric_csr_from_edges
+
+pytestmark = pytest.mark.gpu
+
+POLICIES = {
+    "ab": dict(policy=ibfs_amd.POLICY_ALPHA_BETA),
+    "td": dict(policy=ibfs_amd.POLICY_TD_ONLY),
+    "bu": dict(policy=ibfs_amd.POLICY_BU_ONLY),
+    "sw1": dict(policy=ibfs_amd.POLICY_FIXED, sw_level=1),
+    "sw2": dict(policy=ibfs_amd.POLICY_FIXED, sw_level=2),
+}
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def s16():
+    h = ibfs_amd.HostCsr.rmat(16, 16, seed=1)
+    beg, adj = h.arrays()
+    src = h.gen_sources(128, seed=2)
+    g = ibfs_amd.DeviceGraph(h)
+    want = oracle.bfs_many(beg, adj, src, threads=8)
+    return dict(h=h, beg=beg, adj=adj, src=src, g=g, want=want)
+
+
+@pytest.mark.parametrize("name", golden_bfs_names())
+@pytest.mark.parametrize("pol", list(POLICIES))
+def test_golden_all_sources(name, pol):
+    say(f"[gpu] golden {name} policy {pol}")
+    z = load_golden(name)
+    h = ibfs_amd.HostCsr.from_arrays(z["beg_pos"], z["adj"])
+    g = ibfs_amd.DeviceGraph(h)
+    eng = ibfs_amd.Engine(g, **POLICIES[pol])
+    V = len(z["beg_pos"]) - 1
+    src = np.arange(V, dtype=np.int32)
+    got = eng.run_groups(src)
+    np.testing.assert_array_equal(got, z["depth"])
+
+
+@pytest.mark.parametrize("lane_bits", [64, 128, 256])
+@pytest.mark.parametrize("pol", ["ab", "bu", "td"])
+def test_s16_128_sources(s16, lane_bits, pol):
+    say(f"[gpu] s16 lane_bits={lane_bits} policy={pol}")
+    eng = ibfs_amd.Engine(s16["g"], lane_bits=lane_bits, **POLICIES[pol])
+    got = eng.run_groups(s16["src"])
+    np.testing.assert_array_equal(got, s16["want"])
+    st = eng.last_stats
+    want_edges = sum(oracle.traversed_edges(s16["beg"], d) for d in s16["want"])
+    assert st.traversed_edges == want_edges
+
+
+def test_s16_fixed_switch_every_level(s16):
+    src = s16["src"][:64]
+    for sw in range(0, 8):
+        say(f"[gpu] s16 fixed sw={sw}")
+        eng = ibfs_amd.Engine(s16["g"], policy=ibfs_amd.POLICY_FIXED, sw_level=sw)
+        np.testing.assert_array_equal(eng.traversal(src), s16["want"][:64])
+
+
+def test_partial_group_duplicates_and_isolated_sources():
+    say("[gpu] duplicates / isolated")
+    z = load_golden("bfs_twocomp.npz")
+    h = ibfs_amd.HostCsr.from_arrays(z["beg_pos"], z["adj"])
+    g = ibfs_amd.DeviceGraph(h)
+    src = np.array([7, 0, 0, 3, 7, 6, 3], dtype=np.int32)  # 7 is isolated
+    for pol in POLICIES:
+        eng = ibfs_amd.Engine(g, **POLICIES[pol])
+        got = eng.traversal(src)
+        np.testing.assert_array_equal(got, z["depth"][src])
+
+
+def test_lane_position_invariance():
+    say("[gpu] lane position")
+    z = load_golden("bfs_grid6x7.npz")
+    g = ibfs_amd.DeviceGraph(ibfs_amd.HostCsr.from_arrays(z["beg_pos"], z["adj"]))
+    eng = ibfs_amd.Engine(g, lane_bits=256)
+    for lane in (0, 63, 64, 127, 128, 255):
+        src = np.full(lane + 1, 5, dtype=np.int32)
+        src[lane] = 17
+        got = eng.traversal(src)
+        np.testing.assert_array_equal(got[lane], z["depth"][17])
+
+
+@pytest.mark.parametrize("ring", [4, 5, 8, 255])
+def test_ring_wrap_long_path(ring):
+    say(f"[gpu] ring {ring}")
+    z = load_golden("bfs_classes1500.npz")
+    g = ibfs_amd.DeviceGraph(ibfs_amd.HostCsr.from_arrays(z["beg_pos"], z["adj"]))
+    src = np.array([0, 700, 801, 1499, 1200, 5, 5], dtype=np.int32)
+    eng = ibfs_amd.Engine(g, ring_levels=ring)
+    got = eng.traversal(src)
+    np.testing.assert_array_equal(got, z["depth"][src])
+    if ring < 255:
+        assert eng.last_stats.emit_windows > 1
+
+
+def test_device_output_and_determinism(s16):
+    say("[gpu] device output")
+    eng = ibfs_amd.Engine(s16["g"])
+    V = s16["h"].vert_count
+    buf = ibfs_amd.DeviceBuffer(4 * 128 * V)
+    eng.traversal(s16["src"], out=buf)
+    got = buf.to_host().reshape(128, V)
+    np.testing.assert_array_equal(got, s16["want"])
+    again = eng.traversal(s16["src"])
+    np.testing.assert_array_equal(again, got)
+
+
+def test_level_log_matches_oracle_model(s16):
+    say("[gpu] level log")
+    src = s16["src"][:100]
+    eng = ibfs_amd.Engine(s16["g"], policy=ibfs_amd.POLICY_BU_ONLY)
+    eng.traversal(src)
+    log = eng.level_log()
+    m = oracle.level_model(s16["beg"], s16["adj"], src, dirs=[1] * 64)
+    assert len(log) == m["levels"] + 1
+    for L, li in enumerate(log):
+        assert li["n_changed"] == m["n_changed"][L], L
+        assert li["n_frontier"] == m["n_front"][L], L
+        assert li["m_frontier"] == m["m_front"][L], L
+        assert li["n_unvisited"] == m["n_cand"][L], L
+        assert li["m_unvisited"] == m["m_cand"][L], L
+        assert sum(li["q_bu"]) == m["n_cand"][L]
+
+
+def test_groupby_known_answer_gpu():
+    say("[gpu] groupby known")
+    z = load_golden("groupby_known.npz")
+    g = ibfs_amd.DeviceGraph(ibfs_amd.HostCsr.from_arrays(z["beg_pos"], z["adj"]))
+    np.testing.assert_array_equal(g.groupby(z["src"]), z["expected"])
+
+
+def test_groupby_matches_oracle_s16(s16):
+    say("[gpu] groupby s16")
+    src = s16["h"].gen_sources(3000, seed=9)
+    got = s16["g"].groupby(src)
+    np.testing.assert_array_equal(got, oracle.groupby(s16["beg"], s16["adj"], src))
+
+
+def test_reference_mirror_classes(tmp_path):
+    say("[gpu] mirror classes")
+    h = ibfs_amd.HostCsr.rmat(12, 16, seed=4)
+    h.save(str(tmp_path / "g_beg_pos.bin"), str(tmp_path / "g_csr.bin"))
+    gr = ibfs_amd.graph(str(tmp_path / "g_beg_pos.bin"), str(tmp_path / "g_csr.bin"), 256)
+    gr.gen_src(seed=3)
+    before = gr.src_list.copy()
+    gr.groupby()
+    assert sorted(before.tolist()) == sorted(gr.src_list.tolist())
+    np.testing.assert_array_equal(gr.src_list, oracle.groupby(gr.beg_pos, gr.csr, before))
+    inst = ibfs_amd.gpu_ibfs(gr, 128, sw_level=2)
+    for grp in range(2):
+        d = inst.traversal(grp * 128)
+        want = oracle.bfs_many(gr.beg_pos, gr.csr, gr.src_list[grp * 128:(grp + 1) * 128], 8)
+        np.testing.assert_array_equal(d, want)
+
+
+def test_chung_lu_small():
+    say("[gpu] chung-lu")
+    h = ibfs_amd.HostCsr.chung_lu(30000, 2.2, 30, 3000, seed=5)
+    beg, adj = h.arrays()
+    src = h.gen_sources(256, seed=6)
+    g = ibfs_amd.DeviceGraph(h)
+    for lb in (128, 256):
+        eng = ibfs_amd.Engine(g, lane_bits=lb)
+        np.testing.assert_array_equal(eng.run_groups(src), oracle.bfs_many(beg, adj, src, 8))
+
+
+def test_engine_argument_errors(s16):
+    with pytest.raises(ibfs_amd.IbfsError):
+        ibfs_amd.Engine(s16["g"], lane_bits=96)
+    eng = ibfs_amd.Engine(s16["g"])
+    with pytest.raises(ibfs_amd.IbfsError):
+        eng.traversal(np.arange(129, dtype=np.int32))
+    with pytest.raises(ibfs_amd.IbfsError):
+        eng.traversal(np.array([70000], dtype=np.int32))
+    # the engine still works after rejected calls
+    np.testing.assert_array_equal(eng.traversal(s16["src"][:3]), s16["want"][:3])
