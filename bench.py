@@ -128,11 +128,12 @@ def main() -> int:
 
     # algorithmic bytes per kernel from one untimed pass with the work model
     model = ibfs_amd.Engine(g, lane_bits=K, work_model=1)
+    scratch = ibfs_amd.DeviceBuffer(4 * K * V)  # keeps the timed outputs intact
     alg = {"expand_td": 0, "expand_bu": 0, "inspect": 0, "emit": 0}
     levels = []
     for g0 in range(0, n, K):
         grp = src[g0:g0 + K]
-        model.traversal(grp, out=out)
+        model.traversal(grp, out=scratch)
         lv = model.level_log()
         levels.append([(x["level"], x["direction"]) for x in lv])
         for kname, b in alg_bytes_per_kernel(lv, V, K // 8, len(grp)).items():
