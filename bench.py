@@ -73,7 +73,16 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--emit-mode", type=int, default=0)
+    ap.add_argument("--hubs", type=int, default=None)
+    ap.add_argument("--alpha", type=int, default=None)
+    ap.add_argument("--beta", type=int, default=None)
     args = ap.parse_args()
+    tune = {}
+    if args.alpha is not None:
+        tune["alpha"] = args.alpha
+    if args.beta is not None:
+        tune["beta"] = args.beta
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
               f"--gpus {args.gpus}); source groups would shard across GPUs with no exchange.",
@@ -91,7 +100,7 @@ def main() -> int:
     V, E = h.vert_count, h.edge_count
     log(f"graph V={V} E={E} in {time.perf_counter() - t0:.1f}s")
     src0 = h.gen_sources(wl["sources"], seed=2)
-    g = ibfs_amd.DeviceGraph(h, device=0)
+    g = ibfs_amd.DeviceGraph(h, device=0, hub_count=args.hubs)
     # GroupBy on the device (timed separately, outside the step)
     g.groupby(src0)
     t0 = time.perf_counter()
@@ -99,7 +108,7 @@ def main() -> int:
     groupby_ms = (time.perf_counter() - t0) * 1e3
     K = args.lane_bits
     n = len(src)
-    eng = ibfs_amd.Engine(g, lane_bits=K, kernel_timing=1)
+    eng = ibfs_amd.Engine(g, lane_bits=K, kernel_timing=1, emit_mode=args.emit_mode, **tune)
     out = ibfs_amd.DeviceBuffer(4 * n * V)
     log(f"engine ready: lanes={K} groups={(n + K - 1) // K} depth buffer {4 * n * V / 2**30:.1f} GiB")
     for i in range(args.warmup):
@@ -127,7 +136,7 @@ def main() -> int:
     log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, {gteps:.1f} GTEPS")
 
     # algorithmic bytes per kernel from one untimed pass with the work model
-    model = ibfs_amd.Engine(g, lane_bits=K, work_model=1)
+    model = ibfs_amd.Engine(g, lane_bits=K, work_model=1, **tune)
     scratch = ibfs_amd.DeviceBuffer(4 * K * V)  # keeps the timed outputs intact
     alg = {"expand_td": 0, "expand_bu": 0, "inspect": 0, "emit": 0}
     levels = []
@@ -190,7 +199,7 @@ def main() -> int:
                    "scale": wl["scale"], "edge_factor": 16, "V": V, "E_directed": E,
                    "sources": n, "lane_bits": K, "groups": (n + K - 1) // K,
                    "graph_seed": 1, "source_seed": 2, "permuted_ids": True,
-                   "policy": "alpha_beta(14,24)", "depth_out": "device int32 [sources][V]"},
+                   "policy": f"alpha_beta({eng.cfg.alpha},{eng.cfg.beta})", "depth_out": "device int32 [sources][V]"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},

@@ -166,9 +166,17 @@ class DeviceBuffer:
 class DeviceGraph:
     """A CSR resident in device memory (read-only, shareable by engines)."""
 
-    def __init__(self, csr: HostCsr, device: int = 0):
+    def __init__(self, csr: HostCsr, device: int = 0, hub_count: Optional[int] = None):
         h = C.c_void_p()
-        check(lib().ibfs_graph_from_host(csr._h, device, C.byref(h)))
+        if hub_count is None:
+            check(lib().ibfs_graph_from_host(csr._h, device, C.byref(h)))
+        else:
+            opts = _lib.GraphOpts()
+            check(lib().ibfs_graph_opts_default(C.byref(opts)))
+            opts.hub_count = hub_count
+            beg, adj = csr.arrays()
+            check(lib().ibfs_graph_create_ex(_i64p(beg), len(beg) - 1, _i32p(adj), len(adj),
+                                             device, C.byref(opts), C.byref(h)))
         self._h = h
         self.device = device
         self.vert_count = csr.vert_count

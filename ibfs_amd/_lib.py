@@ -46,7 +46,7 @@ class Config(C.Structure):
         ("policy", C.c_int32), ("sw_level", C.c_int32), ("alpha", C.c_int32),
         ("beta", C.c_int32), ("wg_size", C.c_int32), ("td_chunk", C.c_int32),
         ("ring_levels", C.c_int32), ("kernel_timing", C.c_int32), ("device", C.c_int32),
-        ("work_model", C.c_int32), ("reserved", C.c_int32 * 3),
+        ("work_model", C.c_int32), ("emit_mode", C.c_int32), ("reserved", C.c_int32 * 2),
     ]
 
 
@@ -58,6 +58,10 @@ class LevelInfo(C.Structure):
         ("bu_work", C.c_int64), ("bu_prefix", C.c_int64),
         ("expand_ms", C.c_float), ("inspect_ms", C.c_float),
     ]
+
+
+class GraphOpts(C.Structure):
+    _fields_ = [("hub_count", C.c_int32), ("nthreads", C.c_int32), ("reserved", C.c_int32 * 6)]
 
 
 class Stats(C.Structure):
@@ -97,6 +101,9 @@ SIGNATURES = {
     "ibfs_gen_sources": (C.c_int, [P, C.c_int64, C.c_uint64, I32P]),
     "ibfs_convert_text": (C.c_int, [C.c_char_p, C.c_int32, C.c_int64, C.POINTER(P)]),
     "ibfs_graph_create": (C.c_int, [I64P, C.c_int64, I32P, C.c_int64, C.c_int, C.POINTER(P)]),
+    "ibfs_graph_opts_default": (C.c_int, [C.POINTER(GraphOpts)]),
+    "ibfs_graph_create_ex": (C.c_int, [I64P, C.c_int64, I32P, C.c_int64, C.c_int,
+                                       C.POINTER(GraphOpts), C.POINTER(P)]),
     "ibfs_graph_from_host": (C.c_int, [P, C.c_int, C.POINTER(P)]),
     "ibfs_graph_load_files": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.POINTER(P)]),
     "ibfs_graph_info": (C.c_int, [P, I64P, I64P]),

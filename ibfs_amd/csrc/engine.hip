@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "common.h"
@@ -23,8 +24,11 @@ struct ibfs_graph_s {
   int device = 0;
   int64_t V = 0, E = 0;
   int64_t *d_beg = nullptr;
-  int32_t *d_adj = nullptr;
+  int32_t *d_adj = nullptr;   // adjacency as given (top-down, GroupBy)
+  int32_t *d_tadj = nullptr;  // traversal adjacency: by descending degree, hubs encoded
   int32_t *d_deg = nullptr;
+  int32_t *d_hubs = nullptr;  // the H highest-degree vertices
+  int64_t H = 0;
   int64_t max_deg = 0;
 };
 
@@ -47,6 +51,8 @@ struct ibfs_engine_s {
   size_t d_out_cap = 0;
   u64 *d_count = nullptr;
   u64 *h_count = nullptr;
+  u64 *hs = nullptr;          // hub cache, H * W words
+  int emit_vec4 = 0;          // every lane output of this traversal is 16-byte aligned
   std::vector<ibfs_level_info> log;
   std::vector<hipEvent_t> ev;  // event pool
   size_t ev_used = 0;
@@ -104,6 +110,7 @@ void engine_release(ibfs_engine_s *e) {
   dfree(e->d_out);
   dfree(e->d_count);
   hfree(e->h_count);
+  dfree(e->hs);
   for (auto x : e->ev) (void)hipEventDestroy(x);
   e->ev.clear();
   if (e->st) (void)hipStreamDestroy(e->st);
@@ -134,8 +141,8 @@ int emit_window(ibfs_engine_s *e, int b, int en, int k, ibfs_stats *stats) {
     e0 = next_event(e);
     HIP_TRY(hipEventRecord(e0, e->st));
   }
-  HIP_TRY(launch_emit(e->W, e->ring, e->R, e->g->V, b, en, e->d_lanes, e->d_masks + 2 * e->W,
-                      e->d_masks + e->W, k, e->st));
+  HIP_TRY(launch_emit(e->W, e->cfg.emit_mode, e->emit_vec4, e->ring, e->R, e->g->V, b, en,
+                      e->d_lanes, e->d_masks + 2 * e->W, e->d_masks + e->W, k, e->st));
   if (e->cfg.kernel_timing) {
     e1 = next_event(e);
     HIP_TRY(hipEventRecord(e1, e->st));
@@ -165,6 +172,7 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
     e->h_lanes[i].admit = 0;
     e->h_lanes[i].pad = 0;
   }
+  e->emit_vec4 = (V % 4 == 0) && ((uintptr_t)dout % 16 == 0);
   for (int w = 0; w < W; ++w) e->h_masks[W + w] = e->h_masks[w];
   for (size_t i = 0; i < (size_t)(e->R + 1) * W; ++i) e->h_masks[2 * W + i] = ~0ull;
   HIP_TRY(hipMemcpyAsync(e->d_masks, e->h_masks, sizeof(u64) * masks_words(e),
@@ -233,16 +241,17 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
                                c.q[Q_TD_S], c.q[Q_TD_M], c.q[Q_TD_L], e->chunk_shift, e->st));
       if (stats) stats->n_td_launches++;
     } else {
+      HIP_TRY(launch_hub_refresh(W, slot(e, L - 1), g->d_hubs, g->H, e->hs, e->st));
       if (e->cfg.work_model) {  // instrumentation: exact early-termination prefixes
         HIP_TRY(hipMemsetAsync(e->d_count + 1, 0, 2 * sizeof(u64), e->st));
-        HIP_TRY(launch_bu_model(W, g->d_beg, g->d_adj, slot(e, L - 1), lamp, V, e->d_count + 1,
-                                e->st));
+        HIP_TRY(launch_bu_model(W, g->d_beg, g->d_tadj, slot(e, L - 1), e->hs, lamp, V,
+                                e->d_count + 1, e->st));
         HIP_TRY(hipMemcpyAsync(e->h_count + 1, e->d_count + 1, 2 * sizeof(u64),
                                hipMemcpyDeviceToHost, e->st));
         if (timing) HIP_TRY(hipEventRecord(x0, e->st));
       }
-      HIP_TRY(launch_expand_bu(W, g->d_beg, g->d_adj, slot(e, L - 1), slot(e, L), lamp, e->q,
-                               c.q[Q_BU_S], c.q[Q_BU_M], c.q[Q_BU_L], e->st));
+      HIP_TRY(launch_expand_bu(W, g->d_beg, g->d_tadj, slot(e, L - 1), e->hs, slot(e, L), lamp,
+                               e->q, c.q[Q_BU_S], c.q[Q_BU_M], c.q[Q_BU_L], e->st));
       if (stats) stats->n_bu_launches++;
     }
     if (timing) { x1 = next_event(e); HIP_TRY(hipEventRecord(x1, e->st)); }
@@ -327,13 +336,17 @@ int ibfs_config_default(ibfs_config *cfg) {
   cfg->lrg_shed = 512;
   cfg->policy = IBFS_POLICY_ALPHA_BETA;
   cfg->sw_level = 2;
-  cfg->alpha = 14;
+  // With 128 concurrent lanes a bottom-up vertex rarely covers every active
+  // lane early, so top-down stays cheaper far longer than for one source:
+  // switch when the frontier's edges exceed half of the candidates' edges.
+  cfg->alpha = 2;
   cfg->beta = 24;
   cfg->wg_size = 256;
   cfg->td_chunk = 4096;
   cfg->ring_levels = 32;
   cfg->kernel_timing = 0;
   cfg->device = 0;
+  cfg->emit_mode = 0;
   return IBFS_OK;
 }
 
@@ -369,8 +382,25 @@ int ibfs_memcpy_h2d(void *dst, const void *src, size_t bytes) {
   return IBFS_OK;
 }
 
+int ibfs_graph_opts_default(ibfs_graph_opts *o) {
+  if (!o) return fail(IBFS_ERR_ARG, "null options");
+  std::memset(o, 0, sizeof *o);
+  o->hub_count = 131072;
+  o->nthreads = 0;
+  return IBFS_OK;
+}
+
 int ibfs_graph_create(const int64_t *beg, int64_t V, const int32_t *adj, int64_t E, int device,
                       ibfs_graph_t *out) {
+  return ibfs_graph_create_ex(beg, V, adj, E, device, nullptr, out);
+}
+
+int ibfs_graph_create_ex(const int64_t *beg, int64_t V, const int32_t *adj, int64_t E, int device,
+                         const ibfs_graph_opts *opts_in, ibfs_graph_t *out) {
+  ibfs_graph_opts opts;
+  if (opts_in) opts = *opts_in;
+  else ibfs_graph_opts_default(&opts);
+  if (opts.hub_count < 0) return fail(IBFS_ERR_ARG, "hub_count must be >= 0");
   if (!out || !beg || V < 1 || E < 0 || (E > 0 && !adj))
     return fail(IBFS_ERR_ARG, "ibfs_graph_create: bad arguments (V=%lld, E=%lld)", (long long)V,
                 (long long)E);
@@ -392,21 +422,63 @@ int ibfs_graph_create(const int64_t *beg, int64_t V, const int32_t *adj, int64_t
       return fail(IBFS_ERR_FORMAT, "adjacency entry %lld = %d out of range", (long long)i, adj[i]);
   int rc = check_device(device);
   if (rc) return rc;
+  // Hubs: the H highest-degree vertices (ties by id).  Traversal adjacency:
+  // each list ordered by descending neighbour degree (ties by id), hub
+  // neighbours encoded as 0x80000000 | hub slot.
+  const int64_t H = std::min<int64_t>(opts.hub_count, V);
+  std::vector<int32_t> order((size_t)V);
+  for (int64_t v = 0; v < V; ++v) order[v] = (int32_t)v;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int32_t a, int32_t b) { return deg[a] > deg[b]; });
+  std::vector<int32_t> slot_of((size_t)V, -1);
+  for (int64_t i = 0; i < H; ++i) slot_of[order[i]] = (int32_t)i;
+  std::vector<int32_t> tadj((size_t)std::max<int64_t>(E, 1));
+  struct Ctx {
+    const int64_t *beg;
+    const int32_t *adj, *deg, *slot_of;
+    int32_t *tadj;
+    int64_t V;
+    int nt;
+  } cx{beg, adj, deg.data(), slot_of.data(), tadj.data(), V, 1};
+  {
+    int nt = opts.nthreads > 0 ? opts.nthreads : (int)std::thread::hardware_concurrency();
+    cx.nt = std::max(1, std::min(nt, 64));
+  }
+  parallel_for_threads(cx.nt, [](int t, void *p) {
+    Ctx &c = *(Ctx *)p;
+    for (int64_t v = t; v < c.V; v += c.nt) {
+      int32_t *a = c.tadj + c.beg[v], *e = c.tadj + c.beg[v + 1];
+      std::copy(c.adj + c.beg[v], c.adj + c.beg[v + 1], a);
+      std::sort(a, e, [&](int32_t x, int32_t y) {
+        return c.deg[x] != c.deg[y] ? c.deg[x] > c.deg[y] : x < y;
+      });
+      for (int32_t *x = a; x < e; ++x)
+        if (c.slot_of[*x] >= 0) *x = (int32_t)(0x80000000u | (uint32_t)c.slot_of[*x]);
+    }
+  }, &cx);
   auto *g = new ibfs_graph_s;
   g->device = device;
   g->V = V;
   g->E = E;
+  g->H = H;
   g->max_deg = maxd;
+  const size_t eb = sizeof(int32_t) * (size_t)std::max<int64_t>(E, 1);
   hipError_t err = hipMalloc((void **)&g->d_beg, sizeof(int64_t) * (size_t)(V + 1));
-  if (err == hipSuccess) err = hipMalloc((void **)&g->d_adj, sizeof(int32_t) * (size_t)std::max<int64_t>(E, 1));
+  if (err == hipSuccess) err = hipMalloc((void **)&g->d_adj, eb);
+  if (err == hipSuccess) err = hipMalloc((void **)&g->d_tadj, eb);
   if (err == hipSuccess) err = hipMalloc((void **)&g->d_deg, sizeof(int32_t) * (size_t)V);
+  if (err == hipSuccess) err = hipMalloc((void **)&g->d_hubs, sizeof(int32_t) * (size_t)std::max<int64_t>(H, 1));
   if (err == hipSuccess) err = hipMemcpy(g->d_beg, beg, sizeof(int64_t) * (size_t)(V + 1), hipMemcpyHostToDevice);
   if (err == hipSuccess && E > 0) err = hipMemcpy(g->d_adj, adj, sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice);
+  if (err == hipSuccess && E > 0) err = hipMemcpy(g->d_tadj, tadj.data(), sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipMemcpy(g->d_deg, deg.data(), sizeof(int32_t) * (size_t)V, hipMemcpyHostToDevice);
+  if (err == hipSuccess && H > 0) err = hipMemcpy(g->d_hubs, order.data(), sizeof(int32_t) * (size_t)H, hipMemcpyHostToDevice);
   if (err != hipSuccess) {
     dfree(g->d_beg);
     dfree(g->d_adj);
+    dfree(g->d_tadj);
     dfree(g->d_deg);
+    dfree(g->d_hubs);
     delete g;
     return fail(err == hipErrorOutOfMemory ? IBFS_ERR_NOMEM : IBFS_ERR_HIP,
                 "graph upload failed: %s", hipGetErrorString(err));
@@ -442,7 +514,9 @@ int ibfs_graph_destroy(ibfs_graph_t g) {
   (void)hipSetDevice(g->device);
   dfree(g->d_beg);
   dfree(g->d_adj);
+  dfree(g->d_tadj);
   dfree(g->d_deg);
+  dfree(g->d_hubs);
   delete g;
   return IBFS_OK;
 }
@@ -482,6 +556,8 @@ int ibfs_engine_create(ibfs_graph_t g, const ibfs_config *cfg_in, ibfs_engine_t 
     return fail(IBFS_ERR_ARG, "alpha and beta must be >= 1");
   if (cfg.td_chunk < 256 || (cfg.td_chunk & (cfg.td_chunk - 1)))
     return fail(IBFS_ERR_ARG, "td_chunk must be a power of two >= 256 (got %d)", cfg.td_chunk);
+  if (cfg.emit_mode < 0 || cfg.emit_mode > 2)
+    return fail(IBFS_ERR_ARG, "emit_mode must be 0, 1 or 2 (got %d)", cfg.emit_mode);
   if (cfg.ring_levels < 4 || cfg.ring_levels > 255)
     return fail(IBFS_ERR_ARG, "ring_levels must be in [4, 255] (got %d)", cfg.ring_levels);
   if (cfg.device != g->device)
@@ -521,6 +597,7 @@ int ibfs_engine_create(ibfs_graph_t g, const ibfs_config *cfg_in, ibfs_engine_t 
   H((void **)&e->h_src, sizeof(int32_t) * (size_t)e->K);
   A((void **)&e->d_lanes, sizeof(LaneOut) * (size_t)e->K);
   H((void **)&e->h_lanes, sizeof(LaneOut) * (size_t)e->K);
+  A((void **)&e->hs, sizeof(u64) * (size_t)e->W * (size_t)std::max<int64_t>(g->H, 1));
   A((void **)&e->d_count, 4 * sizeof(u64));
   H((void **)&e->h_count, 4 * sizeof(u64));
   if (err != hipSuccess) {

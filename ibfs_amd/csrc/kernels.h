@@ -49,20 +49,28 @@ hipError_t launch_inspect(int W, const u64 *prev, const u64 *cur, u64 *next,
 hipError_t launch_expand_td(int W, const int64_t *beg, const int32_t *adj, const u64 *prev,
                             u64 *cur, const u64 *lam_prev, Queues q, int64_t ns, int64_t nm,
                             int64_t nl, int chunk_shift, hipStream_t st);
-hipError_t launch_expand_bu(int W, const int64_t *beg, const int32_t *adj, const u64 *prev,
-                            u64 *cur, const u64 *lam_prev, Queues q, int64_t ns, int64_t nm,
-                            int64_t nl, hipStream_t st);
+// tadj: traversal adjacency (neighbours by descending degree, hub entries
+// encoded as 0x80000000 | hub slot); hs: hub cache mirroring prev.
+hipError_t launch_expand_bu(int W, const int64_t *beg, const int32_t *tadj, const u64 *prev,
+                            const u64 *hs, u64 *cur, const u64 *lam_prev, Queues q, int64_t ns,
+                            int64_t nm, int64_t nl, hipStream_t st);
+// hs[i] = S[hubs[i]], i < H.
+hipError_t launch_hub_refresh(int W, const u64 *S, const int32_t *hubs, int64_t H, u64 *hs,
+                              hipStream_t st);
 // Depth emission over the snapshot window [b, e] (ring slots L % R, each
 // V*W words).  level_masks[(L-b)*W + w] = lanes admitted at or before L.
-hipError_t launch_emit(int W, const u64 *ring, int R, int64_t V, int b, int e,
+// mode 0: one vertex per thread, per-lane 4-byte stores; mode 1: LDS-tiled,
+// 16-byte stores when vec4 (all lane outputs 16-byte aligned, V % 4 == 0).
+hipError_t launch_emit(int W, int mode, int vec4, const u64 *ring, int R, int64_t V, int b, int e,
                        const LaneOut *lanes, const u64 *level_masks, const u64 *sel,
                        int k_lanes, hipStream_t st);
 hipError_t launch_count_edges(int W, const u64 *S, const int32_t *deg, int64_t V,
                               const u64 *lanes, u64 *out, hipStream_t st);
 // Bottom-up work model of one level: out[0] += candidates with work,
 // out[1] += sum of their sequential early-termination prefixes.
-hipError_t launch_bu_model(int W, const int64_t *beg, const int32_t *adj, const u64 *prev,
-                           const u64 *lam_prev, int64_t V, u64 *out, hipStream_t st);
+hipError_t launch_bu_model(int W, const int64_t *beg, const int32_t *tadj, const u64 *prev,
+                           const u64 *hs, const u64 *lam_prev, int64_t V, u64 *out,
+                           hipStream_t st);
 // GroupBy (groupby.cu:36-149): key[i] = first neighbour of degree > 128 of a
 // source of degree <= 128, else -1; then the stable placement.
 hipError_t launch_groupby(const int64_t *beg, const int32_t *adj, const int32_t *deg,

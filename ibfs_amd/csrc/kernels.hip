@@ -320,17 +320,45 @@ __global__ __launch_bounds__(BS) void k_expand_td(const int64_t *__restrict__ be
 }
 
 // ---------------------------------------------------------------------------
+// Hub status cache: hs[i] = S[hubs[i]] for the H highest-degree vertices,
+// refreshed after every expansion so that bottom-up gathers of hub words hit
+// a compact, L2-resident array.
+template <int W>
+__global__ __launch_bounds__(BS) void k_hub_refresh(const u64 *__restrict__ S,
+                                                    const int32_t *__restrict__ hubs, int64_t H,
+                                                    u64 *__restrict__ hs) {
+  const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i >= H) return;
+  u64 x[W];
+  load_word<W>(S + (int64_t)hubs[i] * W, x);
+  store_word<W>(hs + i * W, x);
+}
+
+// Word of traversal-adjacency entry a: a >= 0 is a vertex id (read from the
+// snapshot), a < 0 is hub slot a & 0x7fffffff (read from the hub cache).
+template <int W>
+__device__ __forceinline__ void load_nbr(const u64 *__restrict__ prev, const u64 *__restrict__ hs,
+                                         int32_t a, u64 (&y)[W]) {
+  const u64 *p = (a < 0) ? hs + (int64_t)(a & 0x7fffffff) * W : prev + (int64_t)a * W;
+  load_word<W>(p, y);
+}
+
+// ---------------------------------------------------------------------------
 // Bottom-up expansion of level L: a candidate v ORs S_{L-1} of its
 // neighbours (restricted to lam_{L-1}, the only lanes that can still be new)
 // in adjacency order and stops as soon as its word covers lam_{L-1}
-// (bitwise early termination, expander.cu:258-359).  Small candidates: one
-// per thread, four neighbours per step; mid: one per wave, 64 neighbours per
-// step with a wave OR-reduction; large: one per workgroup, 256 neighbours per
-// step with the four wave partials combined through LDS.
+// (bitwise early termination, expander.cu:258-359).  The traversal adjacency
+// `tadj` lists each vertex's neighbours by descending degree (hubs first, so
+// coverage is reached sooner) with hub entries redirected to the hub cache.
+// Small candidates: one per thread, four neighbours per step; mid: one per
+// wave, 64 neighbours per step with a wave OR-reduction; large: one per
+// workgroup, 256 neighbours per step with the four wave partials combined
+// through LDS.
 template <int W>
 __global__ __launch_bounds__(BS) void k_expand_bu(const int64_t *__restrict__ beg,
                                                   const int32_t *__restrict__ adj,
                                                   const u64 *__restrict__ prev,
+                                                  const u64 *__restrict__ hs,
                                                   u64 *__restrict__ cur,
                                                   const u64 *__restrict__ lam_prev, Queues q,
                                                   int64_t ns, int64_t nm, int64_t nl, int64_t nbs,
@@ -359,10 +387,10 @@ __global__ __launch_bounds__(BS) void k_expand_bu(const int64_t *__restrict__ be
       const int32_t a2 = (j + 2 < e) ? adj[j + 2] : a0;
       const int32_t a3 = (j + 3 < e) ? adj[j + 3] : a0;
       u64 s0[W], s1[W], s2[W], s3[W];
-      load_word<W>(prev + (int64_t)a0 * W, s0);
-      load_word<W>(prev + (int64_t)a1 * W, s1);
-      load_word<W>(prev + (int64_t)a2 * W, s2);
-      load_word<W>(prev + (int64_t)a3 * W, s3);
+      load_nbr<W>(prev, hs, a0, s0);
+      load_nbr<W>(prev, hs, a1, s1);
+      load_nbr<W>(prev, hs, a2, s2);
+      load_nbr<W>(prev, hs, a3, s3);
 #pragma unroll
       for (int w = 0; w < W; ++w) x[w] |= (s0[w] | s1[w] | s2[w] | s3[w]) & lam[w];
       j += 4;
@@ -389,7 +417,7 @@ __global__ __launch_bounds__(BS) void k_expand_bu(const int64_t *__restrict__ be
       const int64_t j = r + lane;
       u64 y[W];
       if (j < e) {
-        load_word<W>(prev + (int64_t)adj[j] * W, y);
+        load_nbr<W>(prev, hs, adj[j], y);
       } else {
 #pragma unroll
         for (int w = 0; w < W; ++w) y[w] = 0;
@@ -418,7 +446,7 @@ __global__ __launch_bounds__(BS) void k_expand_bu(const int64_t *__restrict__ be
     const int64_t j = r + t;
     u64 y[W];
     if (j < e) {
-      load_word<W>(prev + (int64_t)adj[j] * W, y);
+      load_nbr<W>(prev, hs, adj[j], y);
     } else {
 #pragma unroll
       for (int w = 0; w < W; ++w) y[w] = 0;
@@ -447,56 +475,167 @@ __global__ __launch_bounds__(BS) void k_expand_bu(const int64_t *__restrict__ be
 // bit is absent (a bit-sliced counter of NS bits), which is the lane's depth
 // relative to its admission level because status words only gain bits.
 // Stores to lane k's array are consecutive across the wave (one 256-byte
-// segment per store instruction).  A lane's first window writes every
-// vertex (-1 where the bit never arrived); later windows write only the
-// vertices whose bit arrived inside them.
+// segment per store instruction).  A lane's first window (admit >= b) writes
+// every vertex (-1 where the bit never arrived); later windows write only the
+// vertices whose bit arrived inside them.  The lane table is staged in LDS:
+// reading it from global memory inside the lane loop would put a memory
+// round trip on every lane of every vertex.
+typedef __attribute__((address_space(1))) int32_t gint32;
+
+__device__ __forceinline__ void put_depth(int32_t *out, int64_t v, int32_t val, bool nt) {
+  gint32 *o = (gint32 *)out;
+  if (nt) __builtin_nontemporal_store(val, o + v);
+  else o[v] = val;
+}
+
+template <int W, int NS>
+__device__ __forceinline__ void emit_count(const u64 *__restrict__ ring, int R, int64_t V, int b,
+                                           int e, int64_t v, const u64 *__restrict__ level_masks,
+                                           u64 (&c)[NS][W], u64 (&fin)[W], u64 (&bs)[W]) {
+  const int64_t slot = V * W;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+#pragma unroll
+    for (int j = 0; j < NS; ++j) c[j][w] = 0;
+  }
+  load_word<W>(ring + (int64_t)(e % R) * slot + v * W, fin);
+  load_word<W>(ring + (int64_t)(b % R) * slot + v * W, bs);
+  for (int L = b; L <= e; ++L) {
+    u64 s[W];
+    load_word<W>(ring + (int64_t)(L % R) * slot + v * W, s);
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      u64 carry = ~s[w] & level_masks[(int64_t)(L - b) * W + w];
+#pragma unroll
+      for (int j = 0; j < NS; ++j) {
+        const u64 nc = c[j][w] & carry;
+        c[j][w] ^= carry;
+        carry = nc;
+      }
+    }
+  }
+}
+
 template <int W, int NS>
 __global__ __launch_bounds__(BS) void k_emit(const u64 *__restrict__ ring, int R, int64_t V, int b,
                                              int e, const LaneOut *__restrict__ lanes,
                                              const u64 *__restrict__ level_masks,
-                                             const u64 *__restrict__ sel, int k_lanes) {
-  const int64_t slot = V * W;
+                                             const u64 *__restrict__ sel, int k_lanes, int nt) {
+  __shared__ int32_t *s_out[4 * 64];
+  __shared__ int32_t s_off[4 * 64];  // b - admit: <= 0 in the lane's first window
+  for (int i = threadIdx.x; i < k_lanes; i += BS) {
+    const bool on = (sel[i >> 6] >> (i & 63)) & 1ull;
+    s_out[i] = on ? lanes[i].out : nullptr;
+    s_off[i] = b - lanes[i].admit;
+  }
+  __syncthreads();
   for (int64_t v = (int64_t)blockIdx.x * BS + threadIdx.x; v < V; v += (int64_t)gridDim.x * BS) {
-    u64 c[NS][W];
-#pragma unroll
-    for (int j = 0; j < NS; ++j)
-#pragma unroll
-      for (int w = 0; w < W; ++w) c[j][w] = 0;
-    u64 fin[W], bs[W];
-    load_word<W>(ring + (int64_t)(e % R) * slot + v * W, fin);
-    load_word<W>(ring + (int64_t)(b % R) * slot + v * W, bs);
-    for (int L = b; L <= e; ++L) {
-      u64 s[W];
-      load_word<W>(ring + (int64_t)(L % R) * slot + v * W, s);
-#pragma unroll
-      for (int w = 0; w < W; ++w) {
-        u64 carry = ~s[w] & level_masks[(int64_t)(L - b) * W + w];
-#pragma unroll
-        for (int j = 0; j < NS; ++j) {
-          const u64 nc = c[j][w] & carry;
-          c[j][w] ^= carry;
-          carry = nc;
-        }
-      }
-    }
+    u64 c[NS][W], fin[W], bs[W];
+    emit_count<W, NS>(ring, R, V, b, e, v, level_masks, c, fin, bs);
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-      const u64 selw = sel[w];
       for (int bit = 0; bit < 64; ++bit) {
         const int k = w * 64 + bit;
         if (k >= k_lanes) break;
-        if (!((selw >> bit) & 1ull)) continue;
-        const LaneOut lo = lanes[k];
-        if (!lo.out) continue;
+        int32_t *out = s_out[k];
+        if (!out) continue;
+        const int off = s_off[k];
         int cnt = 0;
 #pragma unroll
         for (int j = 0; j < NS; ++j) cnt |= (int)((c[j][w] >> bit) & 1ull) << j;
         const bool set = (fin[w] >> bit) & 1ull;
-        if (lo.admit >= b) {
-          lo.out[v] = set ? cnt : -1;
-        } else if (set && !((bs[w] >> bit) & 1ull)) {
-          lo.out[v] = (b - lo.admit) + cnt;
+        if (off <= 0) put_depth(out, v, set ? cnt : -1, nt);
+        else if (set && !((bs[w] >> bit) & 1ull)) put_depth(out, v, off + cnt, nt);
+      }
+    }
+  }
+}
+
+// Tiled depth emission: same values as k_emit, but a workgroup owns 256
+// consecutive vertices and stages 32 lanes at a time through LDS, so that
+// each wave writes one lane's 256 depths as a single 1-KB contiguous segment
+// with 16-byte stores.  Entries a continuation window must not touch carry
+// the sentinel INT_MIN and are skipped.  vec4 = every lane's output is
+// 16-byte aligned and V % 4 == 0.
+template <int W, int NS>
+__global__ __launch_bounds__(BS) void k_emit_tiled(const u64 *__restrict__ ring, int R, int64_t V,
+                                                   int b, int e,
+                                                   const LaneOut *__restrict__ lanes,
+                                                   const u64 *__restrict__ level_masks,
+                                                   const u64 *__restrict__ sel, int k_lanes,
+                                                   int vec4) {
+  constexpr int32_t SKIP = (int32_t)0x80000000;
+  __shared__ int4 tile4[32 * 64];  // [row][64 x int4] = 32 lanes x 256 vertices
+  __shared__ int32_t *s_out[4 * 64];
+  __shared__ int32_t s_off[4 * 64];
+  int32_t *tile = reinterpret_cast<int32_t *>(tile4);
+  const int t = threadIdx.x;
+  for (int i = t; i < k_lanes; i += BS) {
+    const bool on = (sel[i >> 6] >> (i & 63)) & 1ull;
+    s_out[i] = on ? lanes[i].out : nullptr;
+    s_off[i] = b - lanes[i].admit;
+  }
+  __syncthreads();
+  const int64_t nblk = (V + BS - 1) / BS;
+  for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const int64_t v = blk * BS + t;
+    u64 c[NS][W], fin[W], bs[W];
+    if (v < V) {
+      emit_count<W, NS>(ring, R, V, b, e, v, level_masks, c, fin, bs);
+    } else {
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        fin[w] = bs[w] = 0;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) c[j][w] = 0;
+      }
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int lane0 = w * 64 + h * 32;
+        if (lane0 >= k_lanes) break;  // uniform
+        for (int r = 0; r < 32; ++r) {
+          const int k = lane0 + r;
+          const int bit = h * 32 + r;
+          int32_t val = SKIP;
+          if (k < k_lanes && s_out[k]) {
+            const int off = s_off[k];
+            int cnt = 0;
+#pragma unroll
+            for (int j = 0; j < NS; ++j) cnt |= (int)((c[j][w] >> bit) & 1ull) << j;
+            const bool set = (fin[w] >> bit) & 1ull;
+            if (off <= 0) val = set ? cnt : -1;
+            else if (set && !((bs[w] >> bit) & 1ull)) val = off + cnt;
+          }
+          tile[r * BS + t] = val;
         }
+        __syncthreads();
+        for (int m = 0; m < 8; ++m) {
+          const int i = t + BS * m;
+          const int row = i >> 6, col4 = i & 63;
+          const int k = lane0 + row;
+          int32_t *out = (k < k_lanes) ? s_out[k] : nullptr;
+          if (out) {
+            const int4 q = tile4[row * 64 + col4];
+            const int64_t v0 = blk * BS + 4 * col4;
+            const bool full = vec4 && (v0 + 3 < V) && q.x != SKIP && q.y != SKIP &&
+                              q.z != SKIP && q.w != SKIP;
+            gint32 *o = (gint32 *)out;
+            if (full) {
+              typedef int v4i __attribute__((ext_vector_type(4)));
+              v4i x = {q.x, q.y, q.z, q.w};
+              *reinterpret_cast<__attribute__((address_space(1))) v4i *>(o + v0) = x;
+            } else {
+              if (v0 < V && q.x != SKIP) o[v0] = q.x;
+              if (v0 + 1 < V && q.y != SKIP) o[v0 + 1] = q.y;
+              if (v0 + 2 < V && q.z != SKIP) o[v0 + 2] = q.z;
+              if (v0 + 3 < V && q.w != SKIP) o[v0 + 3] = q.w;
+            }
+          }
+        }
+        __syncthreads();
       }
     }
   }
@@ -535,6 +674,7 @@ template <int W>
 __global__ __launch_bounds__(BS) void k_bu_model(const int64_t *__restrict__ beg,
                                                  const int32_t *__restrict__ adj,
                                                  const u64 *__restrict__ prev,
+                                                 const u64 *__restrict__ hs,
                                                  const u64 *__restrict__ lam_prev, int64_t V,
                                                  u64 *out) {
   __shared__ u64 red[2][4];
@@ -551,7 +691,7 @@ __global__ __launch_bounds__(BS) void k_bu_model(const int64_t *__restrict__ beg
     ++nv;
     for (int64_t j = s; j < e; ++j) {
       u64 y[W];
-      load_word<W>(prev + (int64_t)adj[j] * W, y);
+      load_nbr<W>(prev, hs, adj[j], y);
 #pragma unroll
       for (int w = 0; w < W; ++w) x[w] |= y[w] & lam[w];
       ++pref;
@@ -715,47 +855,71 @@ hipError_t launch_expand_td(int W, const int64_t *beg, const int32_t *adj, const
   return hipGetLastError();
 }
 
-hipError_t launch_expand_bu(int W, const int64_t *beg, const int32_t *adj, const u64 *prev,
-                            u64 *cur, const u64 *lam_prev, Queues q, int64_t ns, int64_t nm,
-                            int64_t nl, hipStream_t st) {
+hipError_t launch_expand_bu(int W, const int64_t *beg, const int32_t *tadj, const u64 *prev,
+                            const u64 *hs, u64 *cur, const u64 *lam_prev, Queues q, int64_t ns,
+                            int64_t nm, int64_t nl, hipStream_t st) {
   const int64_t nbs = (ns + BS - 1) / BS, nbm = (nm + 3) / 4;
   const int64_t nb = nbs + nbm + nl;
   if (nb == 0) return hipSuccess;
   if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
   dim3 g((unsigned)nb), blk(BS);
   switch (W) {
-    case 1: hipLaunchKernelGGL(k_expand_bu<1>, g, blk, 0, st, beg, adj, prev, cur, lam_prev, q, ns, nm, nl, nbs, nbm); break;
-    case 2: hipLaunchKernelGGL(k_expand_bu<2>, g, blk, 0, st, beg, adj, prev, cur, lam_prev, q, ns, nm, nl, nbs, nbm); break;
-    case 4: hipLaunchKernelGGL(k_expand_bu<4>, g, blk, 0, st, beg, adj, prev, cur, lam_prev, q, ns, nm, nl, nbs, nbm); break;
+    case 1: hipLaunchKernelGGL(k_expand_bu<1>, g, blk, 0, st, beg, tadj, prev, hs, cur, lam_prev, q, ns, nm, nl, nbs, nbm); break;
+    case 2: hipLaunchKernelGGL(k_expand_bu<2>, g, blk, 0, st, beg, tadj, prev, hs, cur, lam_prev, q, ns, nm, nl, nbs, nbm); break;
+    case 4: hipLaunchKernelGGL(k_expand_bu<4>, g, blk, 0, st, beg, tadj, prev, hs, cur, lam_prev, q, ns, nm, nl, nbs, nbm); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-template <int W>
-static hipError_t emit_w(const u64 *ring, int R, int64_t V, int b, int e, const LaneOut *lanes,
-                         const u64 *masks, const u64 *sel, int k, hipStream_t st) {
-  const int span = e - b + 1;
-  dim3 g(grid_for(V, BS, 8192)), blk(BS);
-  if (span < 16)
-    hipLaunchKernelGGL((k_emit<W, 4>), g, blk, 0, st, ring, R, V, b, e, lanes, masks, sel, k);
-  else if (span < 64)
-    hipLaunchKernelGGL((k_emit<W, 6>), g, blk, 0, st, ring, R, V, b, e, lanes, masks, sel, k);
-  else if (span < 256)
-    hipLaunchKernelGGL((k_emit<W, 8>), g, blk, 0, st, ring, R, V, b, e, lanes, masks, sel, k);
-  else
-    return hipErrorInvalidValue;
+hipError_t launch_hub_refresh(int W, const u64 *S, const int32_t *hubs, int64_t H, u64 *hs,
+                              hipStream_t st) {
+  if (H <= 0) return hipSuccess;
+  dim3 g(grid_for(H, BS, 1 << 30)), blk(BS);
+  switch (W) {
+    case 1: hipLaunchKernelGGL(k_hub_refresh<1>, g, blk, 0, st, S, hubs, H, hs); break;
+    case 2: hipLaunchKernelGGL(k_hub_refresh<2>, g, blk, 0, st, S, hubs, H, hs); break;
+    case 4: hipLaunchKernelGGL(k_hub_refresh<4>, g, blk, 0, st, S, hubs, H, hs); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
-hipError_t launch_emit(int W, const u64 *ring, int R, int64_t V, int b, int e,
+template <int W, int NS>
+static void emit_ns(int mode, int vec4, const u64 *ring, int R, int64_t V, int b, int e,
+                    const LaneOut *lanes, const u64 *masks, const u64 *sel, int k,
+                    hipStream_t st) {
+  if (mode == 1) {
+    dim3 g(grid_for(V, BS, 4096)), blk(BS);
+    hipLaunchKernelGGL((k_emit_tiled<W, NS>), g, blk, 0, st, ring, R, V, b, e, lanes, masks, sel,
+                       k, vec4);
+  } else {
+    dim3 g(grid_for(V, BS, 8192)), blk(BS);
+    hipLaunchKernelGGL((k_emit<W, NS>), g, blk, 0, st, ring, R, V, b, e, lanes, masks, sel, k,
+                       mode == 2 ? 1 : 0);
+  }
+}
+
+template <int W>
+static hipError_t emit_w(int mode, int vec4, const u64 *ring, int R, int64_t V, int b, int e,
+                         const LaneOut *lanes, const u64 *masks, const u64 *sel, int k,
+                         hipStream_t st) {
+  const int span = e - b + 1;
+  if (span < 16) emit_ns<W, 4>(mode, vec4, ring, R, V, b, e, lanes, masks, sel, k, st);
+  else if (span < 64) emit_ns<W, 6>(mode, vec4, ring, R, V, b, e, lanes, masks, sel, k, st);
+  else if (span < 256) emit_ns<W, 8>(mode, vec4, ring, R, V, b, e, lanes, masks, sel, k, st);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_emit(int W, int mode, int vec4, const u64 *ring, int R, int64_t V, int b, int e,
                        const LaneOut *lanes, const u64 *level_masks, const u64 *sel, int k_lanes,
                        hipStream_t st) {
   if (V <= 0 || e < b) return hipSuccess;
   switch (W) {
-    case 1: return emit_w<1>(ring, R, V, b, e, lanes, level_masks, sel, k_lanes, st);
-    case 2: return emit_w<2>(ring, R, V, b, e, lanes, level_masks, sel, k_lanes, st);
-    case 4: return emit_w<4>(ring, R, V, b, e, lanes, level_masks, sel, k_lanes, st);
+    case 1: return emit_w<1>(mode, vec4, ring, R, V, b, e, lanes, level_masks, sel, k_lanes, st);
+    case 2: return emit_w<2>(mode, vec4, ring, R, V, b, e, lanes, level_masks, sel, k_lanes, st);
+    case 4: return emit_w<4>(mode, vec4, ring, R, V, b, e, lanes, level_masks, sel, k_lanes, st);
     default: return hipErrorInvalidValue;
   }
 }
@@ -773,14 +937,15 @@ hipError_t launch_count_edges(int W, const u64 *S, const int32_t *deg, int64_t V
   return hipGetLastError();
 }
 
-hipError_t launch_bu_model(int W, const int64_t *beg, const int32_t *adj, const u64 *prev,
-                           const u64 *lam_prev, int64_t V, u64 *out, hipStream_t st) {
+hipError_t launch_bu_model(int W, const int64_t *beg, const int32_t *tadj, const u64 *prev,
+                           const u64 *hs, const u64 *lam_prev, int64_t V, u64 *out,
+                           hipStream_t st) {
   if (V <= 0) return hipSuccess;
   dim3 g(grid_for(V, BS, 8192)), blk(BS);
   switch (W) {
-    case 1: hipLaunchKernelGGL(k_bu_model<1>, g, blk, 0, st, beg, adj, prev, lam_prev, V, out); break;
-    case 2: hipLaunchKernelGGL(k_bu_model<2>, g, blk, 0, st, beg, adj, prev, lam_prev, V, out); break;
-    case 4: hipLaunchKernelGGL(k_bu_model<4>, g, blk, 0, st, beg, adj, prev, lam_prev, V, out); break;
+    case 1: hipLaunchKernelGGL(k_bu_model<1>, g, blk, 0, st, beg, tadj, prev, hs, lam_prev, V, out); break;
+    case 2: hipLaunchKernelGGL(k_bu_model<2>, g, blk, 0, st, beg, tadj, prev, hs, lam_prev, V, out); break;
+    case 4: hipLaunchKernelGGL(k_bu_model<4>, g, blk, 0, st, beg, tadj, prev, hs, lam_prev, V, out); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

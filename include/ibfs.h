@@ -65,7 +65,9 @@ typedef struct ibfs_config {
   int32_t device;      /* HIP device ordinal                                   */
   int32_t work_model;  /* 1: measure each bottom-up level's early-termination
                           prefix into the level log (instrumentation)        */
-  int32_t reserved[3];
+  int32_t emit_mode;   /* depth emission: 0 per-thread stores, 1 LDS-tiled,
+                          2 per-thread non-temporal stores                   */
+  int32_t reserved[2];
 } ibfs_config;
 
 /* Per-level record of the last traversal (the build's monitoring log, in place
@@ -152,11 +154,24 @@ int ibfs_convert_text(const char *text_path, int32_t reverse,
                       int64_t skip_lines, ibfs_hostcsr_t *out);
 
 /* ---- device graph -------------------------------------------------------- */
-/* Uploads a CSR to the device (int64 beg_pos, int32 adjacency, int32 degree).
- * Replaces the graph upload in the gpu_ibfs constructor (gpu_ibfs.cu:54-116). */
+/* Device-side layout options of a graph (no effect on results). */
+typedef struct ibfs_graph_opts {
+  int32_t hub_count; /* highest-degree vertices whose status words are mirrored
+                        in a compact per-level cache for bottom-up gathers   */
+  int32_t nthreads;  /* host threads for the upload preprocessing (0: all)   */
+  int32_t reserved[6];
+} ibfs_graph_opts;
+int ibfs_graph_opts_default(ibfs_graph_opts *opts);
+/* Uploads a CSR to the device (int64 beg_pos, int32 adjacency, int32 degree,
+ * plus a traversal copy of the adjacency ordered by descending neighbour
+ * degree).  Replaces the graph upload in the gpu_ibfs constructor
+ * (gpu_ibfs.cu:54-116). */
 int ibfs_graph_create(const int64_t *beg_pos, int64_t vert_count,
                       const int32_t *adj, int64_t edge_count, int device,
                       ibfs_graph_t *out);
+int ibfs_graph_create_ex(const int64_t *beg_pos, int64_t vert_count,
+                         const int32_t *adj, int64_t edge_count, int device,
+                         const ibfs_graph_opts *opts, ibfs_graph_t *out);
 int ibfs_graph_from_host(ibfs_hostcsr_t h, int device, ibfs_graph_t *out);
 int ibfs_graph_load_files(const char *beg_path, const char *csr_path,
                           int device, ibfs_graph_t *out);

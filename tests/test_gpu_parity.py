@@ -105,6 +105,32 @@ def test_ring_wrap_long_path(ring):
         assert eng.last_stats.emit_windows > 1
 
 
+@pytest.mark.parametrize("hubs", [0, 3, 1000, None])
+@pytest.mark.parametrize("emit_mode", [0, 1, 2])
+def test_hub_cache_and_emission_variants(hubs, emit_mode):
+    say(f"[gpu] hubs={hubs} emit_mode={emit_mode}")
+    for name in ("bfs_classes1500.npz", "bfs_er300.npz", "bfs_path10.npz"):
+        z = load_golden(name)
+        h = ibfs_amd.HostCsr.from_arrays(z["beg_pos"], z["adj"])
+        g = ibfs_amd.DeviceGraph(h, hub_count=hubs)
+        V = len(z["beg_pos"]) - 1
+        for pol in ("ab", "bu"):
+            for ring in (4, 32):
+                eng = ibfs_amd.Engine(g, emit_mode=emit_mode, ring_levels=ring, **POLICIES[pol])
+                src = np.arange(0, V, max(1, V // 200), dtype=np.int32)
+                np.testing.assert_array_equal(eng.run_groups(src), z["depth"][src],
+                                              err_msg=f"{name} {pol} ring={ring}")
+
+
+@pytest.mark.parametrize("hubs", [0, 64, None])
+def test_s16_hub_cache(s16, hubs):
+    say(f"[gpu] s16 hubs={hubs}")
+    g = ibfs_amd.DeviceGraph(s16["h"], hub_count=hubs)
+    for lb in (128, 256):
+        eng = ibfs_amd.Engine(g, lane_bits=lb, policy=ibfs_amd.POLICY_BU_ONLY)
+        np.testing.assert_array_equal(eng.run_groups(s16["src"]), s16["want"])
+
+
 def test_device_output_and_determinism(s16):
     say("[gpu] device output")
     eng = ibfs_amd.Engine(s16["g"])
