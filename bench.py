@@ -77,6 +77,10 @@ def main() -> int:
     ap.add_argument("--hubs", type=int, default=None)
     ap.add_argument("--alpha", type=int, default=None)
     ap.add_argument("--beta", type=int, default=None)
+    ap.add_argument("--batch", type=int, default=0,
+                    help="contexts of the batched launcher (0: plain engine)")
+    ap.add_argument("--burst", type=int, default=8)
+    ap.add_argument("--dd-grid", type=int, default=0)
     args = ap.parse_args()
     tune = {}
     if args.alpha is not None:
@@ -108,7 +112,12 @@ def main() -> int:
     groupby_ms = (time.perf_counter() - t0) * 1e3
     K = args.lane_bits
     n = len(src)
-    eng = ibfs_amd.Engine(g, lane_bits=K, kernel_timing=1, emit_mode=args.emit_mode, **tune)
+    if args.batch:
+        eng = ibfs_amd.Batch(g, contexts=args.batch, burst=args.burst, lane_bits=K,
+                             emit_mode=args.emit_mode, dd_grid=args.dd_grid, **tune)
+        eng.run_groups = eng.run
+    else:
+        eng = ibfs_amd.Engine(g, lane_bits=K, kernel_timing=1, emit_mode=args.emit_mode, **tune)
     out = ibfs_amd.DeviceBuffer(4 * n * V)
     log(f"engine ready: lanes={K} groups={(n + K - 1) // K} depth buffer {4 * n * V / 2**30:.1f} GiB")
     for i in range(args.warmup):

@@ -291,6 +291,65 @@ class Engine:
             pass
 
 
+class Batch:
+    """Batched launcher: any number of sources packed into lane groups,
+    `contexts` traversals in flight, device-driven levels in captured
+    hipGraphs (ibfs_batch_*)."""
+
+    def __init__(self, g: DeviceGraph, contexts: int = 2, burst: int = 8,
+                 cfg: Optional[Config] = None, **overrides):
+        if cfg is None:
+            cfg = default_config(device=g.device, **overrides)
+        elif overrides:
+            for k, v in overrides.items():
+                setattr(cfg, k, v)
+        h = C.c_void_p()
+        check(lib().ibfs_batch_create(g._h, C.byref(cfg), contexts, burst, C.byref(h)))
+        self._h = h
+        self.graph = g
+        self.cfg = cfg
+        self.last_stats = Stats()
+
+    def run(self, sources, out: Optional[DeviceBuffer] = None) -> Optional[np.ndarray]:
+        s = np.ascontiguousarray(sources, dtype=np.int32)
+        n, V = len(s), self.graph.vert_count
+        st = Stats()
+        if out is None:
+            depth = np.empty((n, V), dtype=np.int32)
+            check(lib().ibfs_batch_run(self._h, _i32p(s), n, depth.ctypes.data, OUT_HOST,
+                                       C.byref(st)))
+            self.last_stats = st
+            return depth
+        if out.nbytes < 4 * n * V:
+            raise ValueError("device buffer too small")
+        check(lib().ibfs_batch_run(self._h, _i32p(s), n, C.c_void_p(out.ptr), OUT_DEVICE,
+                                   C.byref(st)))
+        self.last_stats = st
+        return None
+
+    def closeness(self, sources) -> tuple[np.ndarray, np.ndarray]:
+        """Per-source (sum of depths, reached vertices), int64 each."""
+        s = np.ascontiguousarray(sources, dtype=np.int32)
+        dsum = np.zeros(len(s), dtype=np.int64)
+        reach = np.zeros(len(s), dtype=np.int64)
+        st = Stats()
+        check(lib().ibfs_batch_closeness(self._h, _i32p(s), len(s), _i64p(dsum), _i64p(reach),
+                                         C.byref(st)))
+        self.last_stats = st
+        return dsum, reach
+
+    def close(self) -> None:
+        if self._h:
+            lib().ibfs_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ---------------------------------------------------------------------------
 # Mirror of the reference's host classes (graph.h / graph.cu, gpu_ibfs.cuh).
 

@@ -46,7 +46,8 @@ class Config(C.Structure):
         ("policy", C.c_int32), ("sw_level", C.c_int32), ("alpha", C.c_int32),
         ("beta", C.c_int32), ("wg_size", C.c_int32), ("td_chunk", C.c_int32),
         ("ring_levels", C.c_int32), ("kernel_timing", C.c_int32), ("device", C.c_int32),
-        ("work_model", C.c_int32), ("emit_mode", C.c_int32), ("reserved", C.c_int32 * 2),
+        ("work_model", C.c_int32), ("emit_mode", C.c_int32), ("dd_grid", C.c_int32),
+        ("reserved", C.c_int32 * 1),
     ]
 
 
@@ -116,6 +117,10 @@ SIGNATURES = {
                                         C.POINTER(C.c_int32)]),
     "ibfs_run_groups": (C.c_int, [P, I32P, C.c_int64, P, C.c_int32, C.POINTER(Stats)]),
     "ibfs_traversed_edges": (C.c_int, [P, I32P, I64P]),
+    "ibfs_batch_create": (C.c_int, [P, C.POINTER(Config), C.c_int32, C.c_int32, C.POINTER(P)]),
+    "ibfs_batch_run": (C.c_int, [P, I32P, C.c_int64, P, C.c_int32, C.POINTER(Stats)]),
+    "ibfs_batch_destroy": (C.c_int, [P]),
+    "ibfs_batch_closeness": (C.c_int, [P, I32P, C.c_int64, I64P, I64P, C.POINTER(Stats)]),
 }
 
 _lib = None

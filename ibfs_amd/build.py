@@ -15,7 +15,7 @@ CSRC = PKG / "csrc"
 OUT = PKG / "libibfs.so"
 CLI = PKG / "gpu-ibfs"
 
-HIP_SOURCES = ["kernels.hip", "engine.hip", "batch.hip"]
+HIP_SOURCES = ["kernels_traverse.hip", "kernels_emit.hip", "engine.hip", "batch.hip"]
 CXX_SOURCES = ["hostcsr.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

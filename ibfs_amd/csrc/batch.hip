@@ -1,0 +1,462 @@
+// Batched launcher: an arbitrary source set packed into lane groups, several
+// traversal contexts in flight on their own HIP streams, each level a
+// captured hipGraph of two kernels (expand, inspect) whose direction is
+// decided on the device, so the host only polls once per burst of levels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstddef>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+
+using namespace ibfs;
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess)                                                              \
+      return fail(IBFS_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                  __FILE__, __LINE__);                                                 \
+  } while (0)
+
+namespace {
+
+enum CtxState { IDLE = 0, RUNNING = 1, EMITTING = 2 };
+
+struct Ctx {
+  hipStream_t st = nullptr;
+  u64 *ring = nullptr;
+  LevelCtl *ctl = nullptr;
+  Queues q{};
+  u64 *hs = nullptr;
+  Plan *d_plan = nullptr, *h_plan = nullptr;
+  u64 *d_masks = nullptr, *h_masks = nullptr;  // [0,W) lanes_all, [W,2W) sel, then (R+1)W level masks
+  int32_t *d_src = nullptr, *h_src = nullptr;
+  LaneOut *d_lanes = nullptr, *h_lanes = nullptr;
+  int32_t *d_stage = nullptr;  // host-placed output staging, K*V
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev = nullptr;
+  int state = IDLE;
+  int64_t g0 = 0;
+  int k = 0;
+  int base = 0;
+  int level = 0;  // level the device will expand next (host view)
+  int32_t *out_host = nullptr;
+};
+
+template <class T>
+void dfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+template <class T>
+void hfree(T *&p) {
+  if (p) (void)hipHostFree((void *)p);
+  p = nullptr;
+}
+
+}  // namespace
+
+struct ibfs_batch_s {
+  ibfs_graph_s *g = nullptr;
+  ibfs_config cfg{};
+  int K = 128, W = 2, R = 16, C = 2, burst = 8, chunk_shift = 12, grid = 2048;
+  std::vector<Ctx> ctx;
+  u64 *d_edges = nullptr, *h_edges = nullptr;
+  // closeness runs: per-source depth sums and reach counts instead of depths
+  bool closeness = false;
+  u64 *d_sum = nullptr, *d_reach = nullptr;
+  int64_t sum_cap = 0;
+};
+
+namespace {
+
+size_t masks_words(const ibfs_batch_s *b) { return (size_t)(2 + (b->R + 1)) * b->W; }
+
+u64 *slot(const ibfs_batch_s *b, const Ctx &c, int L) {
+  return c.ring + (size_t)(L % b->R) * (size_t)b->g->V * (size_t)b->W;
+}
+
+void release_ctx(Ctx &c) {
+  if (c.st) (void)hipStreamSynchronize(c.st);
+  if (c.exec) (void)hipGraphExecDestroy(c.exec);
+  if (c.graph) (void)hipGraphDestroy(c.graph);
+  c.exec = nullptr;
+  c.graph = nullptr;
+  dfree(c.ring);
+  dfree(c.ctl);
+  for (int i = 0; i < 6; ++i) dfree(c.q.q[i]);
+  dfree(c.q.chunks);
+  dfree(c.hs);
+  dfree(c.d_plan);
+  hfree(c.h_plan);
+  dfree(c.d_masks);
+  hfree(c.h_masks);
+  dfree(c.d_src);
+  hfree(c.h_src);
+  dfree(c.d_lanes);
+  hfree(c.h_lanes);
+  dfree(c.d_stage);
+  if (c.ev) (void)hipEventDestroy(c.ev);
+  c.ev = nullptr;
+  if (c.st) (void)hipStreamDestroy(c.st);
+  c.st = nullptr;
+}
+
+void release_batch(ibfs_batch_s *b) {
+  if (b->g) (void)hipSetDevice(b->g->device);
+  for (auto &c : b->ctx) release_ctx(c);
+  b->ctx.clear();
+  dfree(b->d_edges);
+  hfree(b->h_edges);
+  dfree(b->d_sum);
+  dfree(b->d_reach);
+  b->sum_cap = 0;
+}
+
+// Output of the snapshot window [lo, hi] of context c: depth arrays, or the
+// closeness sums of the group's sources.
+hipError_t window_out(ibfs_batch_s *b, Ctx &c, int lo, int hi) {
+  if (b->closeness)
+    return launch_close(b->W, c.ring, b->R, b->g->V, lo, hi, c.d_lanes, c.d_masks + 2 * b->W,
+                        c.d_masks + b->W, c.k, b->d_sum + c.g0, b->d_reach + c.g0, c.st);
+  return launch_emit(b->W, b->cfg.emit_mode, 0, c.ring, b->R, b->g->V, lo, hi, c.d_lanes,
+                     c.d_masks + 2 * b->W, c.d_masks + b->W, c.k, c.st);
+}
+
+InspectArgs dd_inspect_args(const ibfs_batch_s *b, const Ctx &c) {
+  InspectArgs a{};
+  a.deg = b->g->d_deg;
+  a.V = b->g->V;
+  a.q = c.q;
+  a.sml = b->cfg.sml_shed;
+  a.lrg = b->cfg.lrg_shed;
+  a.chunk_shift = b->chunk_shift;
+  a.hubs = b->g->d_hubs;
+  a.H = b->g->H;
+  a.hs = c.hs;
+  a.plan = c.d_plan;
+  a.ring = c.ring;
+  a.R = b->R;
+  a.ctl_ring = c.ctl;
+  a.lanes_all = c.d_masks;
+  return a;
+}
+
+int alloc_ctx(ibfs_batch_s *b, Ctx &c) {
+  const int64_t V = b->g->V;
+  const size_t slot_bytes = (size_t)V * (size_t)b->W * sizeof(u64);
+  hipError_t err = hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking);
+  auto A = [&](void **p, size_t bytes) {
+    if (err == hipSuccess) err = hipMalloc(p, bytes ? bytes : 1);
+  };
+  auto H = [&](void **p, size_t bytes) {
+    if (err == hipSuccess) err = hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocDefault);
+  };
+  A((void **)&c.ring, (size_t)b->R * slot_bytes);
+  A((void **)&c.ctl, sizeof(LevelCtl) * (size_t)b->R);
+  for (int i = 0; i < 6; ++i)
+    if (i != Q_TD_L) A((void **)&c.q.q[i], sizeof(int32_t) * (size_t)V);
+  A((void **)&c.q.chunks, sizeof(int2) * ((size_t)V + (size_t)(b->g->E >> b->chunk_shift) + 2));
+  A((void **)&c.hs, sizeof(u64) * (size_t)b->W * (size_t)std::max<int64_t>(b->g->H, 1));
+  A((void **)&c.d_plan, sizeof(Plan));
+  H((void **)&c.h_plan, sizeof(Plan));
+  A((void **)&c.d_masks, sizeof(u64) * masks_words(b));
+  H((void **)&c.h_masks, sizeof(u64) * masks_words(b));
+  A((void **)&c.d_src, sizeof(int32_t) * (size_t)b->K);
+  H((void **)&c.h_src, sizeof(int32_t) * (size_t)b->K);
+  A((void **)&c.d_lanes, sizeof(LaneOut) * (size_t)b->K);
+  H((void **)&c.h_lanes, sizeof(LaneOut) * (size_t)b->K);
+  if (err == hipSuccess) err = hipEventCreateWithFlags(&c.ev, hipEventDisableTiming);
+  if (err != hipSuccess)
+    return fail(err == hipErrorOutOfMemory ? IBFS_ERR_NOMEM : IBFS_ERR_HIP,
+                "batch context allocation failed: %s", hipGetErrorString(err));
+  // capture one level body: device-driven expansion + inspection
+  HIP_TRY(hipStreamBeginCapture(c.st, hipStreamCaptureModeThreadLocal));
+  hipError_t e1 = launch_expand_dd(b->W, b->g->d_beg, b->g->d_adj, b->g->d_tadj, c.ring, b->R, V,
+                                   c.ctl, c.d_plan, c.hs, c.q, b->chunk_shift, b->grid, c.st);
+  hipError_t e2 = launch_inspect(b->W, dd_inspect_args(b, c), c.st);
+  if (e2 == hipSuccess) e2 = launch_plan_decide(c.d_plan, c.ctl, b->R, c.st);
+  hipError_t e3 = hipStreamEndCapture(c.st, &c.graph);
+  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+    return fail(IBFS_ERR_HIP, "level graph capture failed: %s / %s / %s", hipGetErrorString(e1),
+                hipGetErrorString(e2), hipGetErrorString(e3));
+  HIP_TRY(hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0));
+  return IBFS_OK;
+}
+
+// Starts group [g0, g0+k) of src on context c: lane table, masks, S_0 and the
+// level-0 inspection (whose last workgroup plans level 1).
+int start_group(ibfs_batch_s *b, Ctx &c, const int32_t *src, int64_t g0, int k, int32_t *dout) {
+  const int W = b->W;
+  const int64_t V = b->g->V;
+  std::memset(c.h_masks, 0, sizeof(u64) * masks_words(b));
+  for (int i = 0; i < k; ++i) {
+    c.h_masks[i >> 6] |= 1ull << (i & 63);
+    c.h_src[i] = src[g0 + i];
+    c.h_lanes[i].out = dout ? dout + (size_t)i * (size_t)V : nullptr;
+    c.h_lanes[i].admit = 0;
+    c.h_lanes[i].pad = 0;
+  }
+  for (int w = 0; w < W; ++w) c.h_masks[W + w] = c.h_masks[w];
+  for (size_t i = 0; i < (size_t)(b->R + 1) * W; ++i) c.h_masks[2 * W + i] = ~0ull;
+  std::memset(c.h_plan, 0, sizeof(Plan));
+  c.h_plan->policy = b->cfg.policy;
+  c.h_plan->sw_level = b->cfg.sw_level;
+  c.h_plan->alpha = b->cfg.alpha;
+  c.h_plan->beta = b->cfg.beta;
+  c.h_plan->nf_prev = ~0ull;
+  c.h_plan->V = (u64)V;
+  c.h_plan->dir = -1;
+  HIP_TRY(hipMemcpyAsync(c.d_masks, c.h_masks, sizeof(u64) * masks_words(b), hipMemcpyHostToDevice, c.st));
+  HIP_TRY(hipMemcpyAsync(c.d_src, c.h_src, sizeof(int32_t) * k, hipMemcpyHostToDevice, c.st));
+  HIP_TRY(hipMemcpyAsync(c.d_lanes, c.h_lanes, sizeof(LaneOut) * k, hipMemcpyHostToDevice, c.st));
+  HIP_TRY(hipMemcpyAsync(c.d_plan, c.h_plan, sizeof(Plan), hipMemcpyHostToDevice, c.st));
+  HIP_TRY(launch_fill_u64(slot(b, c, 0), V * W, 0ull, c.st));
+  HIP_TRY(launch_fill_u64(reinterpret_cast<u64 *>(c.ctl), (int64_t)b->R * 16, 0ull, c.st));
+  HIP_TRY(launch_init_sources(W, c.d_src, k, 0, slot(b, c, 0), c.st));
+  HIP_TRY(launch_inspect(W, dd_inspect_args(b, c), c.st));
+  HIP_TRY(launch_plan_decide(c.d_plan, c.ctl, b->R, c.st));
+  c.g0 = g0;
+  c.k = k;
+  c.base = 0;
+  c.level = 1;
+  return IBFS_OK;
+}
+
+// Enqueues up to `burst` level bodies (bounded so that no snapshot of the
+// current emission window is overwritten), then a copy of the plan and the
+// context's event.
+int launch_burst(ibfs_batch_s *b, Ctx &c, ibfs_stats *stats) {
+  int B = std::min(b->burst, b->R - 1 - (c.level - c.base));
+  if (B < 1) {  // flush the window of complete snapshots [base, level-1]
+    HIP_TRY(window_out(b, c, c.base, c.level - 1));
+    if (stats) stats->emit_windows++;
+    c.base = c.level - 1;
+    B = std::min(b->burst, b->R - 1 - (c.level - c.base));
+  }
+  for (int i = 0; i < B; ++i) HIP_TRY(hipGraphLaunch(c.exec, c.st));
+  HIP_TRY(hipMemcpyAsync(c.h_plan, c.d_plan, offsetof(Plan, dirlog), hipMemcpyDeviceToHost, c.st));
+  HIP_TRY(hipEventRecord(c.ev, c.st));
+  return IBFS_OK;
+}
+
+int finish_group(ibfs_batch_s *b, Ctx &c, int last, ibfs_stats *stats) {
+  if (last >= c.base) {
+    HIP_TRY(window_out(b, c, c.base, last));
+    if (stats) stats->emit_windows++;
+  }
+  if (stats) {
+    stats->levels = std::max(stats->levels, last + 1);
+    HIP_TRY(launch_count_edges(b->W, slot(b, c, last), b->g->d_deg, b->g->V, c.d_masks,
+                               b->d_edges, c.st));
+  }
+  if (c.out_host)
+    HIP_TRY(hipMemcpyAsync(c.out_host, c.d_stage, sizeof(int32_t) * (size_t)c.k * (size_t)b->g->V,
+                           hipMemcpyDeviceToHost, c.st));
+  HIP_TRY(hipEventRecord(c.ev, c.st));
+  return IBFS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ibfs_batch_create(ibfs_graph_t g, const ibfs_config *cfg_in, int32_t contexts, int32_t burst,
+                      ibfs_batch_t *out) {
+  if (!g || !out) return fail(IBFS_ERR_ARG, "ibfs_batch_create: null argument");
+  if (contexts < 1 || contexts > 16) return fail(IBFS_ERR_ARG, "contexts must be in [1, 16]");
+  if (burst < 1 || burst > 64) return fail(IBFS_ERR_ARG, "burst must be in [1, 64]");
+  ibfs_config cfg;
+  if (cfg_in) cfg = *cfg_in;
+  else ibfs_config_default(&cfg);
+  if (cfg.lane_bits != 64 && cfg.lane_bits != 128 && cfg.lane_bits != 256)
+    return fail(IBFS_ERR_ARG, "lane_bits must be 64, 128 or 256 (got %d)", cfg.lane_bits);
+  if (cfg.sml_shed < 1 || cfg.lrg_shed < cfg.sml_shed)
+    return fail(IBFS_ERR_ARG, "need 1 <= sml_shed <= lrg_shed");
+  if (cfg.policy < IBFS_POLICY_FIXED || cfg.policy > IBFS_POLICY_BU_ONLY)
+    return fail(IBFS_ERR_ARG, "unknown policy %d", cfg.policy);
+  if (cfg.td_chunk < 256 || (cfg.td_chunk & (cfg.td_chunk - 1)))
+    return fail(IBFS_ERR_ARG, "td_chunk must be a power of two >= 256");
+  if (cfg.ring_levels < 4 || cfg.ring_levels > 255)
+    return fail(IBFS_ERR_ARG, "ring_levels must be in [4, 255]");
+  if (cfg.emit_mode < 0 || cfg.emit_mode > 2) return fail(IBFS_ERR_ARG, "bad emit_mode");
+  if (cfg.device != g->device) return fail(IBFS_ERR_ARG, "config device differs from the graph's");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return fail(IBFS_ERR_NODEVICE, "no HIP device available");
+  HIP_TRY(hipSetDevice(g->device));
+  auto *b = new ibfs_batch_s;
+  b->g = g;
+  b->cfg = cfg;
+  b->K = cfg.lane_bits;
+  b->W = cfg.lane_bits / 64;
+  b->C = contexts;
+  b->burst = burst;
+  b->chunk_shift = __builtin_ctz((unsigned)cfg.td_chunk);
+  const size_t slot_bytes = (size_t)g->V * (size_t)b->W * sizeof(u64);
+  int R = cfg.ring_levels;
+  const size_t budget = ((size_t)16 << 30) / (size_t)contexts;
+  while (R > 4 && (size_t)R * slot_bytes > budget) --R;
+  b->R = R;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device) != hipSuccess ||
+      cus <= 0)
+    cus = 256;
+  b->grid = cfg.dd_grid > 0 ? cfg.dd_grid : cus * 8;
+  b->ctx.resize(contexts);
+  hipError_t err = hipMalloc((void **)&b->d_edges, sizeof(u64));
+  if (err == hipSuccess) err = hipHostMalloc((void **)&b->h_edges, sizeof(u64), hipHostMallocDefault);
+  if (err != hipSuccess) {
+    release_batch(b);
+    delete b;
+    return fail(IBFS_ERR_NOMEM, "batch allocation failed");
+  }
+  for (auto &c : b->ctx) {
+    int rc = alloc_ctx(b, c);
+    if (rc) {
+      release_batch(b);
+      delete b;
+      return rc;
+    }
+  }
+  *out = b;
+  return IBFS_OK;
+}
+
+int ibfs_batch_destroy(ibfs_batch_t b) {
+  if (!b) return IBFS_OK;
+  release_batch(b);
+  delete b;
+  return IBFS_OK;
+}
+
+}  // extern "C"
+
+// Scheduler shared by depth runs and closeness runs (b->closeness).
+static int batch_run_impl(ibfs_batch_s *b, const int32_t *src, int64_t n, int32_t *depth_out,
+                          int32_t out_place, ibfs_stats *stats) {
+  const int64_t V = b->g->V;
+  for (int64_t i = 0; i < n; ++i)
+    if (src[i] < 0 || src[i] >= V)
+      return fail(IBFS_ERR_ARG, "source %lld = %d out of range", (long long)i, src[i]);
+  if (stats) std::memset(stats, 0, sizeof *stats);
+  if (n == 0) return IBFS_OK;
+  HIP_TRY(hipSetDevice(b->g->device));
+  if (out_place == IBFS_OUT_HOST) {
+    for (auto &c : b->ctx)
+      if (!c.d_stage) HIP_TRY(hipMalloc((void **)&c.d_stage, sizeof(int32_t) * (size_t)b->K * (size_t)V));
+  }
+  HIP_TRY(hipMemset(b->d_edges, 0, sizeof(u64)));
+  HIP_TRY(hipDeviceSynchronize());
+  const auto t0 = std::chrono::steady_clock::now();
+  const int64_t groups = (n + b->K - 1) / b->K;
+  int64_t next = 0, finished = 0;
+  int rc = IBFS_OK;
+  for (auto &c : b->ctx) c.state = IDLE;
+  while (finished < groups && rc == IBFS_OK) {
+    bool progressed = false;
+    for (auto &c : b->ctx) {
+      if (c.state == IDLE) {
+        if (next >= groups) continue;
+        const int64_t g0 = next * b->K;
+        const int k = (int)std::min<int64_t>(b->K, n - g0);
+        int32_t *dout = nullptr;
+        c.out_host = nullptr;
+        if (!b->closeness) {
+          dout = out_place == IBFS_OUT_DEVICE ? depth_out + (size_t)g0 * (size_t)V : c.d_stage;
+          c.out_host = out_place == IBFS_OUT_HOST ? depth_out + (size_t)g0 * (size_t)V : nullptr;
+        }
+        ++next;
+        rc = start_group(b, c, src, g0, k, dout);
+        if (!rc) rc = launch_burst(b, c, stats);
+        c.state = RUNNING;
+        progressed = true;
+      } else {
+        hipError_t q = hipEventQuery(c.ev);
+        if (q == hipErrorNotReady) continue;
+        if (q != hipSuccess) {
+          rc = fail(IBFS_ERR_HIP, "context event: %s", hipGetErrorString(q));
+          break;
+        }
+        progressed = true;
+        if (c.state == RUNNING) {
+          if (c.h_plan->done) {
+            rc = finish_group(b, c, c.h_plan->last, stats);
+            c.state = EMITTING;
+          } else if (c.h_plan->level > V + 2) {
+            rc = fail(IBFS_ERR_STATE, "traversal exceeded V+2 levels");
+          } else {
+            c.level = c.h_plan->level;
+            rc = launch_burst(b, c, stats);
+          }
+        } else {  // EMITTING finished
+          c.state = IDLE;
+          ++finished;
+        }
+      }
+      if (rc) break;
+    }
+    if (!progressed) std::this_thread::yield();
+  }
+  if (rc) {
+    for (auto &c : b->ctx) (void)hipStreamSynchronize(c.st);
+    return rc;
+  }
+  for (auto &c : b->ctx) HIP_TRY(hipStreamSynchronize(c.st));
+  if (stats) {
+    stats->total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    HIP_TRY(hipMemcpy(b->h_edges, b->d_edges, sizeof(u64), hipMemcpyDeviceToHost));
+    stats->traversed_edges = (int64_t)*b->h_edges;
+  }
+  return IBFS_OK;
+}
+
+extern "C" {
+
+int ibfs_batch_run(ibfs_batch_t b, const int32_t *src, int64_t n, int32_t *depth_out,
+                   int32_t out_place, ibfs_stats *stats) {
+  if (!b || n < 0 || (n > 0 && (!src || !depth_out)))
+    return fail(IBFS_ERR_ARG, "ibfs_batch_run: bad arguments");
+  if (out_place != IBFS_OUT_HOST && out_place != IBFS_OUT_DEVICE)
+    return fail(IBFS_ERR_ARG, "bad out_place %d", out_place);
+  b->closeness = false;
+  return batch_run_impl(b, src, n, depth_out, out_place, stats);
+}
+
+int ibfs_batch_closeness(ibfs_batch_t b, const int32_t *src, int64_t n, int64_t *depth_sum,
+                         int64_t *reach, ibfs_stats *stats) {
+  if (!b || n < 0 || (n > 0 && (!src || !depth_sum || !reach)))
+    return fail(IBFS_ERR_ARG, "ibfs_batch_closeness: bad arguments");
+  if (n == 0) {
+    if (stats) std::memset(stats, 0, sizeof *stats);
+    return IBFS_OK;
+  }
+  HIP_TRY(hipSetDevice(b->g->device));
+  if (b->sum_cap < n) {
+    dfree(b->d_sum);
+    dfree(b->d_reach);
+    b->sum_cap = 0;
+    HIP_TRY(hipMalloc((void **)&b->d_sum, sizeof(u64) * (size_t)n));
+    HIP_TRY(hipMalloc((void **)&b->d_reach, sizeof(u64) * (size_t)n));
+    b->sum_cap = n;
+  }
+  HIP_TRY(hipMemset(b->d_sum, 0, sizeof(u64) * (size_t)n));
+  HIP_TRY(hipMemset(b->d_reach, 0, sizeof(u64) * (size_t)n));
+  b->closeness = true;
+  int rc = batch_run_impl(b, src, n, nullptr, IBFS_OUT_DEVICE, stats);
+  b->closeness = false;
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(depth_sum, b->d_sum, sizeof(u64) * (size_t)n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(reach, b->d_reach, sizeof(u64) * (size_t)n, hipMemcpyDeviceToHost));
+  return IBFS_OK;
+}
+
+}  // extern "C"

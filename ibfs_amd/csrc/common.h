@@ -33,3 +33,16 @@ void parallel_for_threads(int nthreads, void (*fn)(int, void *), void *ctx);
 struct ibfs_hostcsr_s {
   ibfs::HostCsr csr;
 };
+
+// Device-resident graph (read-only after upload).
+struct ibfs_graph_s {
+  int device = 0;
+  int64_t V = 0, E = 0;
+  int64_t *d_beg = nullptr;
+  int32_t *d_adj = nullptr;   // adjacency as given (top-down, GroupBy)
+  int32_t *d_tadj = nullptr;  // traversal adjacency: by descending degree, hubs encoded
+  int32_t *d_deg = nullptr;
+  int32_t *d_hubs = nullptr;  // the H highest-degree vertices
+  int64_t H = 0;
+  int64_t max_deg = 0;
+};

@@ -20,18 +20,6 @@ using namespace ibfs;
                   __FILE__, __LINE__);                                                 \
   } while (0)
 
-struct ibfs_graph_s {
-  int device = 0;
-  int64_t V = 0, E = 0;
-  int64_t *d_beg = nullptr;
-  int32_t *d_adj = nullptr;   // adjacency as given (top-down, GroupBy)
-  int32_t *d_tadj = nullptr;  // traversal adjacency: by descending degree, hubs encoded
-  int32_t *d_deg = nullptr;
-  int32_t *d_hubs = nullptr;  // the H highest-degree vertices
-  int64_t H = 0;
-  int64_t max_deg = 0;
-};
-
 struct ibfs_engine_s {
   ibfs_graph_s *g = nullptr;
   ibfs_config cfg{};
@@ -134,6 +122,29 @@ int decide(const ibfs_engine_s *e, int L, const LevelCtl &c, int dir_prev, u64 n
   return (small && c.n_front < nf_prev) ? 0 : 1;
 }
 
+// Host-driven inspector arguments (the pass also refreshes the hub cache
+// from cur for the next bottom-up level).
+InspectArgs inspect_args(ibfs_engine_s *e, const u64 *prev, const u64 *cur, u64 *next,
+                         const u64 *lam_prev, LevelCtl *ctl, LevelCtl *ctl_zero) {
+  InspectArgs a{};
+  a.prev = prev;
+  a.cur = cur;
+  a.next = next;
+  a.lam_prev = lam_prev;
+  a.ctl = ctl;
+  a.ctl_zero = ctl_zero;
+  a.deg = e->g->d_deg;
+  a.V = e->g->V;
+  a.q = e->q;
+  a.sml = e->cfg.sml_shed;
+  a.lrg = e->cfg.lrg_shed;
+  a.chunk_shift = e->chunk_shift;
+  a.hubs = e->g->d_hubs;
+  a.H = e->g->H;
+  a.hs = e->hs;
+  return a;
+}
+
 // Emits the window [b, en] for the lanes in e->h_lanes (already on device).
 int emit_window(ibfs_engine_s *e, int b, int en, int k, ibfs_stats *stats) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -191,9 +202,9 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
   HIP_TRY(launch_init_sources(W, e->d_src, k, 0, slot(e, 0), e->st));
   hipEvent_t ti0 = nullptr, ti1 = nullptr;
   if (timing) { ti0 = next_event(e); HIP_TRY(hipEventRecord(ti0, e->st)); }
-  HIP_TRY(launch_inspect(W, nullptr, slot(e, 0), slot(e, 1), g->d_deg, V, e->d_masks, nullptr,
-                         nullptr, &e->d_ctl[0], &e->d_ctl[1 % e->R], e->q, e->cfg.sml_shed,
-                         e->cfg.lrg_shed, e->chunk_shift, e->st));
+  HIP_TRY(launch_inspect(W, inspect_args(e, nullptr, slot(e, 0), slot(e, 1), e->d_masks,
+                                         &e->d_ctl[0], &e->d_ctl[1 % e->R]),
+                         e->st));
   if (timing) { ti1 = next_event(e); HIP_TRY(hipEventRecord(ti1, e->st)); }
   HIP_TRY(hipMemcpyAsync(e->h_ctl, &e->d_ctl[0], sizeof(LevelCtl), hipMemcpyDeviceToHost, e->st));
   HIP_TRY(hipStreamSynchronize(e->st));
@@ -241,7 +252,6 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
                                c.q[Q_TD_S], c.q[Q_TD_M], c.q[Q_TD_L], e->chunk_shift, e->st));
       if (stats) stats->n_td_launches++;
     } else {
-      HIP_TRY(launch_hub_refresh(W, slot(e, L - 1), g->d_hubs, g->H, e->hs, e->st));
       if (e->cfg.work_model) {  // instrumentation: exact early-termination prefixes
         HIP_TRY(hipMemsetAsync(e->d_count + 1, 0, 2 * sizeof(u64), e->st));
         HIP_TRY(launch_bu_model(W, g->d_beg, g->d_tadj, slot(e, L - 1), e->hs, lamp, V,
@@ -263,9 +273,9 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
     }
     hipEvent_t y0 = nullptr;
     if (timing) { y0 = next_event(e); HIP_TRY(hipEventRecord(y0, e->st)); }
-    HIP_TRY(launch_inspect(W, slot(e, L - 1), slot(e, L), slot(e, L + 1), g->d_deg, V, lamp,
-                           nullptr, nullptr, &e->d_ctl[L % e->R], &e->d_ctl[(L + 1) % e->R], e->q,
-                           e->cfg.sml_shed, e->cfg.lrg_shed, e->chunk_shift, e->st));
+    HIP_TRY(launch_inspect(W, inspect_args(e, slot(e, L - 1), slot(e, L), slot(e, L + 1), lamp,
+                                           &e->d_ctl[L % e->R], &e->d_ctl[(L + 1) % e->R]),
+                           e->st));
     if (timing) { y1 = next_event(e); HIP_TRY(hipEventRecord(y1, e->st)); }
     HIP_TRY(hipMemcpyAsync(e->h_ctl, &e->d_ctl[L % e->R], sizeof(LevelCtl),
                            hipMemcpyDeviceToHost, e->st));

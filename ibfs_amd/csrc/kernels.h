@@ -24,6 +24,21 @@ static_assert(sizeof(LevelCtl) == 128, "LevelCtl layout");
 
 enum { Q_TD_S = 0, Q_TD_M = 1, Q_TD_L = 2, Q_BU_S = 3, Q_BU_M = 4, Q_BU_L = 5 };
 
+// Device-resident plan of a device-driven traversal: the inspector's last
+// workgroup decides the next level's direction here, the expander reads it.
+struct Plan {
+  int level;            // level to expand next (== level being inspected)
+  int dir;              // direction of `level`: 0 top-down, 1 bottom-up
+  int done;             // 1 once a level produced no change
+  int last;             // deepest level with a change (valid when done)
+  unsigned int blocks_done;  // inspector workgroups finished (tail election)
+  int policy, sw_level, alpha, beta;
+  int pad;
+  u64 nf_prev;          // frontier size of the level before `level`
+  u64 V;
+  int dirlog[256];      // direction per level, indexed level & 255
+};
+
 // Output slot of one lane for the depth emission.
 struct LaneOut {
   int32_t *out;   // depth array of V int32 for this lane (null: lane not emitted)
@@ -36,16 +51,38 @@ struct Queues {
   int2 *chunks;   // (vertex, chunk index) of top-down large vertices
 };
 
+// Arguments of one inspector pass.  Host-driven: prev/cur/next/lam_prev/ctl
+// given explicitly and plan == nullptr.  Device-driven: plan != nullptr and
+// the level, snapshot slots and control blocks are derived from plan->level.
+struct InspectArgs {
+  const u64 *prev;
+  const u64 *cur;
+  u64 *next;
+  const u64 *lam_prev;
+  LevelCtl *ctl;
+  LevelCtl *ctl_zero;
+  const u64 *prev_keep;  // lanes whose previous bits count (null: all)
+  const u64 *next_and;   // mask applied to the copy into next (null: all)
+  const int32_t *deg;
+  int64_t V;
+  Queues q;
+  int sml, lrg, chunk_shift;
+  const int32_t *hubs;  // hub cache refresh: hs[i] = cur[hubs[i]], i < H
+  int64_t H;
+  u64 *hs;
+  Plan *plan;           // device-driven mode
+  u64 *ring;
+  int R;
+  LevelCtl *ctl_ring;
+  const u64 *lanes_all; // lam_prev of level 0
+};
+
 // Launchers (all asynchronous on `st`).  W = status words (u64) per vertex.
 hipError_t launch_fill_u64(u64 *p, int64_t n, u64 value, hipStream_t st);
 hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t value, hipStream_t st);
 hipError_t launch_init_sources(int W, const int32_t *src, int k, int lane0, u64 *S0,
                                hipStream_t st);
-hipError_t launch_inspect(int W, const u64 *prev, const u64 *cur, u64 *next,
-                          const int32_t *deg, int64_t V, const u64 *lam_prev,
-                          const u64 *prev_keep, const u64 *next_and, LevelCtl *ctl_out,
-                          LevelCtl *ctl_zero, Queues q, int sml, int lrg, int chunk_shift,
-                          hipStream_t st);
+hipError_t launch_inspect(int W, const InspectArgs &a, hipStream_t st);
 hipError_t launch_expand_td(int W, const int64_t *beg, const int32_t *adj, const u64 *prev,
                             u64 *cur, const u64 *lam_prev, Queues q, int64_t ns, int64_t nm,
                             int64_t nl, int chunk_shift, hipStream_t st);
@@ -54,16 +91,27 @@ hipError_t launch_expand_td(int W, const int64_t *beg, const int32_t *adj, const
 hipError_t launch_expand_bu(int W, const int64_t *beg, const int32_t *tadj, const u64 *prev,
                             const u64 *hs, u64 *cur, const u64 *lam_prev, Queues q, int64_t ns,
                             int64_t nm, int64_t nl, hipStream_t st);
-// hs[i] = S[hubs[i]], i < H.
-hipError_t launch_hub_refresh(int W, const u64 *S, const int32_t *hubs, int64_t H, u64 *hs,
-                              hipStream_t st);
+// Device-driven expansion of plan->level (either direction), fixed grid.
+hipError_t launch_expand_dd(int W, const int64_t *beg, const int32_t *adj, const int32_t *tadj,
+                            u64 *ring, int R, int64_t V, const LevelCtl *ctl_ring,
+                            const Plan *plan, const u64 *hs, Queues q, int chunk_shift,
+                            int grid, hipStream_t st);
+// Device-driven direction decision for plan->level + 1 (one thread).
+hipError_t launch_plan_decide(Plan *plan, const LevelCtl *ctl_ring, int R, hipStream_t st);
 // Depth emission over the snapshot window [b, e] (ring slots L % R, each
 // V*W words).  level_masks[(L-b)*W + w] = lanes admitted at or before L.
 // mode 0: one vertex per thread, per-lane 4-byte stores; mode 1: LDS-tiled,
-// 16-byte stores when vec4 (all lane outputs 16-byte aligned, V % 4 == 0).
+// 16-byte stores when vec4 (all lane outputs 16-byte aligned, V % 4 == 0);
+// mode 2: as 0 with non-temporal stores.
 hipError_t launch_emit(int W, int mode, int vec4, const u64 *ring, int R, int64_t V, int b, int e,
                        const LaneOut *lanes, const u64 *level_masks, const u64 *sel,
                        int k_lanes, hipStream_t st);
+// Closeness reduction of the window [b, e]: sum_out[k] += sum of depths and
+// reach_out[k] += vertices reached, over the bits of lane k that arrived in
+// the window (lane table and masks as for launch_emit).
+hipError_t launch_close(int W, const u64 *ring, int R, int64_t V, int b, int e,
+                        const LaneOut *lanes, const u64 *level_masks, const u64 *sel, int k_lanes,
+                        u64 *sum_out, u64 *reach_out, hipStream_t st);
 hipError_t launch_count_edges(int W, const u64 *S, const int32_t *deg, int64_t V,
                               const u64 *lanes, u64 *out, hipStream_t st);
 // Bottom-up work model of one level: out[0] += candidates with work,

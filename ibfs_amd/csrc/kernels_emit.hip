@@ -1,0 +1,450 @@
+// Depth emission, TEPS numerator and GroupBy kernels (gfx950, wave64).
+#include <hip/hip_runtime.h>
+
+#include "device_util.h"
+
+namespace ibfs {
+namespace {
+
+using namespace dev;
+
+// ---------------------------------------------------------------------------
+// Depth emission over the snapshot window [b, e]: one thread per vertex
+// counts, per lane, the admitted levels of the window at which the lane's
+// bit is absent (a bit-sliced counter of NS bits), which is the lane's depth
+// relative to its admission level because status words only gain bits.
+// Stores to lane k's array are consecutive across the wave (one 256-byte
+// segment per store instruction).  A lane's first window (admit >= b) writes
+// every vertex (-1 where the bit never arrived); later windows write only the
+// vertices whose bit arrived inside them.  The lane table is staged in LDS:
+// reading it from global memory inside the lane loop would put a memory
+// round trip on every lane of every vertex.
+
+__device__ __forceinline__ void put_depth(int32_t *out, int64_t v, int32_t val, bool nt) {
+  gint32 *o = (gint32 *)out;
+  if (nt) __builtin_nontemporal_store(val, o + v);
+  else o[v] = val;
+}
+
+template <int W, int NS>
+__device__ __forceinline__ void emit_count(const u64 *__restrict__ ring, int R, int64_t V, int b,
+                                           int e, int64_t v, const u64 *__restrict__ level_masks,
+                                           u64 (&c)[NS][W], u64 (&fin)[W], u64 (&bs)[W]) {
+  const int64_t slot = V * W;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+#pragma unroll
+    for (int j = 0; j < NS; ++j) c[j][w] = 0;
+  }
+  load_word<W>(ring + (int64_t)(e % R) * slot + v * W, fin);
+  load_word<W>(ring + (int64_t)(b % R) * slot + v * W, bs);
+  for (int L = b; L <= e; ++L) {
+    u64 s[W];
+    load_word<W>(ring + (int64_t)(L % R) * slot + v * W, s);
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      u64 carry = ~s[w] & level_masks[(int64_t)(L - b) * W + w];
+#pragma unroll
+      for (int j = 0; j < NS; ++j) {
+        const u64 nc = c[j][w] & carry;
+        c[j][w] ^= carry;
+        carry = nc;
+      }
+    }
+  }
+}
+
+template <int W, int NS>
+__global__ __launch_bounds__(BS) void k_emit(const u64 *__restrict__ ring, int R, int64_t V, int b,
+                                             int e, const LaneOut *__restrict__ lanes,
+                                             const u64 *__restrict__ level_masks,
+                                             const u64 *__restrict__ sel, int k_lanes, int nt) {
+  __shared__ int32_t *s_out[4 * 64];
+  __shared__ int32_t s_off[4 * 64];  // b - admit: <= 0 in the lane's first window
+  for (int i = threadIdx.x; i < k_lanes; i += BS) {
+    const bool on = (sel[i >> 6] >> (i & 63)) & 1ull;
+    s_out[i] = on ? lanes[i].out : nullptr;
+    s_off[i] = b - lanes[i].admit;
+  }
+  __syncthreads();
+  for (int64_t v = (int64_t)blockIdx.x * BS + threadIdx.x; v < V; v += (int64_t)gridDim.x * BS) {
+    u64 c[NS][W], fin[W], bs[W];
+    emit_count<W, NS>(ring, R, V, b, e, v, level_masks, c, fin, bs);
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      for (int bit = 0; bit < 64; ++bit) {
+        const int k = w * 64 + bit;
+        if (k >= k_lanes) break;
+        int32_t *out = s_out[k];
+        if (!out) continue;
+        const int off = s_off[k];
+        int cnt = 0;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) cnt |= (int)((c[j][w] >> bit) & 1ull) << j;
+        const bool set = (fin[w] >> bit) & 1ull;
+        if (off <= 0) put_depth(out, v, set ? cnt : -1, nt);
+        else if (set && !((bs[w] >> bit) & 1ull)) put_depth(out, v, off + cnt, nt);
+      }
+    }
+  }
+}
+
+// Tiled depth emission: same values as k_emit, but a workgroup owns 256
+// consecutive vertices and stages 32 lanes at a time through LDS, so that
+// each wave writes one lane's 256 depths as a single 1-KB contiguous segment
+// with 16-byte stores.  Entries a continuation window must not touch carry
+// the sentinel INT_MIN and are skipped.  vec4 = every lane's output is
+// 16-byte aligned and V % 4 == 0.
+template <int W, int NS>
+__global__ __launch_bounds__(BS) void k_emit_tiled(const u64 *__restrict__ ring, int R, int64_t V,
+                                                   int b, int e,
+                                                   const LaneOut *__restrict__ lanes,
+                                                   const u64 *__restrict__ level_masks,
+                                                   const u64 *__restrict__ sel, int k_lanes,
+                                                   int vec4) {
+  constexpr int32_t SKIP = (int32_t)0x80000000;
+  __shared__ int4 tile4[32 * 64];  // [row][64 x int4] = 32 lanes x 256 vertices
+  __shared__ int32_t *s_out[4 * 64];
+  __shared__ int32_t s_off[4 * 64];
+  int32_t *tile = reinterpret_cast<int32_t *>(tile4);
+  const int t = threadIdx.x;
+  for (int i = t; i < k_lanes; i += BS) {
+    const bool on = (sel[i >> 6] >> (i & 63)) & 1ull;
+    s_out[i] = on ? lanes[i].out : nullptr;
+    s_off[i] = b - lanes[i].admit;
+  }
+  __syncthreads();
+  const int64_t nblk = (V + BS - 1) / BS;
+  for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const int64_t v = blk * BS + t;
+    u64 c[NS][W], fin[W], bs[W];
+    if (v < V) {
+      emit_count<W, NS>(ring, R, V, b, e, v, level_masks, c, fin, bs);
+    } else {
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        fin[w] = bs[w] = 0;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) c[j][w] = 0;
+      }
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int lane0 = w * 64 + h * 32;
+        if (lane0 >= k_lanes) break;  // uniform
+        for (int r = 0; r < 32; ++r) {
+          const int k = lane0 + r;
+          const int bit = h * 32 + r;
+          int32_t val = SKIP;
+          if (k < k_lanes && s_out[k]) {
+            const int off = s_off[k];
+            int cnt = 0;
+#pragma unroll
+            for (int j = 0; j < NS; ++j) cnt |= (int)((c[j][w] >> bit) & 1ull) << j;
+            const bool set = (fin[w] >> bit) & 1ull;
+            if (off <= 0) val = set ? cnt : -1;
+            else if (set && !((bs[w] >> bit) & 1ull)) val = off + cnt;
+          }
+          tile[r * BS + t] = val;
+        }
+        __syncthreads();
+        for (int m = 0; m < 8; ++m) {
+          const int i = t + BS * m;
+          const int row = i >> 6, col4 = i & 63;
+          const int k = lane0 + row;
+          int32_t *out = (k < k_lanes) ? s_out[k] : nullptr;
+          if (out) {
+            const int4 q = tile4[row * 64 + col4];
+            const int64_t v0 = blk * BS + 4 * col4;
+            const bool full = vec4 && (v0 + 3 < V) && q.x != SKIP && q.y != SKIP &&
+                              q.z != SKIP && q.w != SKIP;
+            gint32 *o = (gint32 *)out;
+            if (full) {
+              typedef int v4i __attribute__((ext_vector_type(4)));
+              v4i x = {q.x, q.y, q.z, q.w};
+              *reinterpret_cast<__attribute__((address_space(1))) v4i *>(o + v0) = x;
+            } else {
+              if (v0 < V && q.x != SKIP) o[v0] = q.x;
+              if (v0 + 1 < V && q.y != SKIP) o[v0 + 1] = q.y;
+              if (v0 + 2 < V && q.z != SKIP) o[v0 + 2] = q.z;
+              if (v0 + 3 < V && q.w != SKIP) o[v0 + 3] = q.w;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+  }
+}
+
+// Closeness reduction over the snapshot window [b, e]: per lane k, the sum
+// over vertices of depth_k(v) and the number of vertices reached, for the
+// vertices whose bit arrived in the window, without writing depth arrays.
+// With the per-vertex bit-sliced counters c_j (depth = sum_j 2^j c_j), the
+// sum over a wave's 64 vertices is sum_j 2^j popcount(ballot(c_j bit k)),
+// so each lane costs NS+1 ballots per 64 vertices.  Lane k of the window is
+// a first window (off_k <= 0: every set bit counts, depth = count) or a
+// continuation (off_k = b - admit > 0: bits set at e but not at b count,
+// depth = off_k + count).  Wave partials meet in LDS, one atomic per lane
+// and workgroup.
+template <int W, int NS>
+__global__ __launch_bounds__(BS) void k_close(const u64 *__restrict__ ring, int R, int64_t V, int b,
+                                              int e, const LaneOut *__restrict__ lanes,
+                                              const u64 *__restrict__ level_masks,
+                                              const u64 *__restrict__ sel, int k_lanes,
+                                              unsigned long long *__restrict__ sum_out,
+                                              unsigned long long *__restrict__ reach_out) {
+  __shared__ int32_t s_off[4 * 64];
+  __shared__ u64 red[2][4][4 * 64];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  for (int i = t; i < 4 * 64; i += BS) s_off[i] = (i < k_lanes) ? b - lanes[i].admit : 0;
+  __syncthreads();
+  u64 selw[W], cont[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    selw[w] = sel[w];
+    cont[w] = 0;
+    for (int bit = 0; bit < 64 && w * 64 + bit < k_lanes; ++bit)
+      if (s_off[w * 64 + bit] > 0) cont[w] |= 1ull << bit;
+  }
+  u64 acc_sum[W], acc_reach[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) acc_sum[w] = acc_reach[w] = 0;
+  const int64_t nblk = (V + BS - 1) / BS;
+  for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const int64_t v = blk * BS + t;
+    u64 c[NS][W], fin[W], bs[W], M[W];
+    if (v < V) {
+      emit_count<W, NS>(ring, R, V, b, e, v, level_masks, c, fin, bs);
+    } else {
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        fin[w] = bs[w] = 0;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) c[j][w] = 0;
+      }
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      M[w] = fin[w] & ~(bs[w] & cont[w]) & selw[w];
+#pragma unroll
+      for (int j = 0; j < NS; ++j) c[j][w] &= M[w];
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      if (!wave_or(M[w])) continue;  // uniform: no counted bit in this word
+      for (int bit = 0; bit < 64; ++bit) {
+        const int k = w * 64 + bit;
+        if (k >= k_lanes) break;
+        const u64 pm = __popcll(__ballot((M[w] >> bit) & 1ull));
+        if (!pm) continue;
+        u64 s = 0;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) s += (u64)__popcll(__ballot((c[j][w] >> bit) & 1ull)) << j;
+        if (lane == bit) {
+          acc_reach[w] += pm;
+          acc_sum[w] += s + (u64)(s_off[k] > 0 ? s_off[k] : 0) * pm;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    red[0][wv][w * 64 + lane] = acc_sum[w];
+    red[1][wv][w * 64 + lane] = acc_reach[w];
+  }
+  __syncthreads();
+  for (int i = t; i < k_lanes; i += BS) {
+    const u64 s = red[0][0][i] + red[0][1][i] + red[0][2][i] + red[0][3][i];
+    const u64 r = red[1][0][i] + red[1][1][i] + red[1][2][i] + red[1][3][i];
+    if (r) {
+      atomicAdd(&sum_out[i], s);
+      atomicAdd(&reach_out[i], r);
+    }
+  }
+}
+
+// Sum over v of deg(v) * |S[v] & lanes|: the TEPS numerator of all lanes
+// (validate.h:59-76 summed over sources).
+template <int W>
+__global__ __launch_bounds__(BS) void k_count_edges(const u64 *__restrict__ S,
+                                                    const int32_t *__restrict__ deg, int64_t V,
+                                                    const u64 *__restrict__ lanes, u64 *out) {
+  __shared__ u64 red[4];
+  u64 acc = 0;
+  for (int64_t v = (int64_t)blockIdx.x * BS + threadIdx.x; v < V; v += (int64_t)gridDim.x * BS) {
+    u64 x[W];
+    load_word<W>(S + v * W, x);
+    int pc = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) pc += __popcll(x[w] & lanes[w]);
+    acc += (u64)pc * (u64)deg[v];
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const u64 s = red[0] + red[1] + red[2] + red[3];
+    if (s) atomicAdd(out, s);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// GroupBy.  Key of source i: its first neighbour (adjacency order) of degree
+// > 128 if deg(src[i]) <= 128, else -1 (groupby.cu:49-66); one wave per
+// source scans 64 neighbours per step and takes the lowest set ballot lane.
+__global__ __launch_bounds__(BS) void k_groupby_key(const int64_t *__restrict__ beg,
+                                                    const int32_t *__restrict__ adj,
+                                                    const int32_t *__restrict__ deg,
+                                                    const int32_t *__restrict__ src, int64_t n,
+                                                    int32_t *__restrict__ key) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= n) return;
+  const int32_t s = src[i];
+  int32_t h = -1;
+  if (deg[s] <= 128) {
+    const int64_t e = beg[s + 1];
+    for (int64_t j0 = beg[s]; j0 < e; j0 += 64) {
+      const int64_t j = j0 + lane;
+      const bool hub = (j < e) && (deg[adj[j]] > 128);
+      const u64 m = __ballot(hub);
+      if (m) {
+        h = adj[j0 + __ffsll((unsigned long long)m) - 1];
+        break;
+      }
+    }
+  }
+  if (lane == 0) key[i] = h;
+}
+
+// Stable placement (groupby.cu:69-115): an eligible source goes after every
+// eligible source with a smaller hub id or the same hub and a smaller input
+// index; ineligible sources follow all eligible ones in input order.  Each
+// thread counts its rank over all keys staged through LDS in tiles.
+__global__ __launch_bounds__(BS) void k_groupby_place(const int32_t *__restrict__ key,
+                                                      const int32_t *__restrict__ src, int64_t n,
+                                                      int32_t *__restrict__ out) {
+  __shared__ int32_t tile[1024];
+  const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  const int32_t ki = (i < n) ? key[i] : 0;
+  int64_t less = 0, elig = 0, inel_before = 0;
+  for (int64_t base = 0; base < n; base += 1024) {
+    for (int r = threadIdx.x; r < 1024; r += BS)
+      tile[r] = (base + r < n) ? key[base + r] : 0;
+    __syncthreads();
+    const int m = (n - base) < 1024 ? (int)(n - base) : 1024;
+    for (int jj = 0; jj < m; ++jj) {
+      const int32_t kj = tile[jj];
+      const int64_t j = base + jj;
+      if (kj >= 0) {
+        ++elig;
+        if (ki >= 0 && (kj < ki || (kj == ki && j < i))) ++less;
+      } else if (j < i) {
+        ++inel_before;
+      }
+    }
+    __syncthreads();
+  }
+  if (i < n) out[(ki >= 0) ? less : (elig + inel_before)] = src[i];
+}
+
+}  // namespace
+
+template <int W, int NS>
+static void emit_ns(int mode, int vec4, const u64 *ring, int R, int64_t V, int b, int e,
+                    const LaneOut *lanes, const u64 *masks, const u64 *sel, int k,
+                    hipStream_t st) {
+  if (mode == 1) {
+    dim3 g(grid_for(V, BS, 4096)), blk(BS);
+    hipLaunchKernelGGL((k_emit_tiled<W, NS>), g, blk, 0, st, ring, R, V, b, e, lanes, masks, sel,
+                       k, vec4);
+  } else {
+    dim3 g(grid_for(V, BS, 8192)), blk(BS);
+    hipLaunchKernelGGL((k_emit<W, NS>), g, blk, 0, st, ring, R, V, b, e, lanes, masks, sel, k,
+                       mode == 2 ? 1 : 0);
+  }
+}
+
+template <int W>
+static hipError_t emit_w(int mode, int vec4, const u64 *ring, int R, int64_t V, int b, int e,
+                         const LaneOut *lanes, const u64 *masks, const u64 *sel, int k,
+                         hipStream_t st) {
+  const int span = e - b + 1;
+  if (span < 16) emit_ns<W, 4>(mode, vec4, ring, R, V, b, e, lanes, masks, sel, k, st);
+  else if (span < 64) emit_ns<W, 6>(mode, vec4, ring, R, V, b, e, lanes, masks, sel, k, st);
+  else if (span < 256) emit_ns<W, 8>(mode, vec4, ring, R, V, b, e, lanes, masks, sel, k, st);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_emit(int W, int mode, int vec4, const u64 *ring, int R, int64_t V, int b, int e,
+                       const LaneOut *lanes, const u64 *level_masks, const u64 *sel, int k_lanes,
+                       hipStream_t st) {
+  if (V <= 0 || e < b) return hipSuccess;
+  switch (W) {
+    case 1: return emit_w<1>(mode, vec4, ring, R, V, b, e, lanes, level_masks, sel, k_lanes, st);
+    case 2: return emit_w<2>(mode, vec4, ring, R, V, b, e, lanes, level_masks, sel, k_lanes, st);
+    case 4: return emit_w<4>(mode, vec4, ring, R, V, b, e, lanes, level_masks, sel, k_lanes, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_count_edges(int W, const u64 *S, const int32_t *deg, int64_t V,
+                              const u64 *lanes, u64 *out, hipStream_t st) {
+  if (V <= 0) return hipSuccess;
+  dim3 g(grid_for(V, BS * 4, 4096)), blk(BS);
+  switch (W) {
+    case 1: hipLaunchKernelGGL(k_count_edges<1>, g, blk, 0, st, S, deg, V, lanes, out); break;
+    case 2: hipLaunchKernelGGL(k_count_edges<2>, g, blk, 0, st, S, deg, V, lanes, out); break;
+    case 4: hipLaunchKernelGGL(k_count_edges<4>, g, blk, 0, st, S, deg, V, lanes, out); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int W>
+static hipError_t close_w(const u64 *ring, int R, int64_t V, int b, int e, const LaneOut *lanes,
+                          const u64 *masks, const u64 *sel, int k, u64 *sum, u64 *reach,
+                          hipStream_t st) {
+  const int span = e - b + 1;
+  dim3 g(grid_for(V, BS * 8, 256)), blk(BS);
+  if (span < 16)
+    hipLaunchKernelGGL((k_close<W, 4>), g, blk, 0, st, ring, R, V, b, e, lanes, masks, sel, k, sum, reach);
+  else if (span < 64)
+    hipLaunchKernelGGL((k_close<W, 6>), g, blk, 0, st, ring, R, V, b, e, lanes, masks, sel, k, sum, reach);
+  else if (span < 256)
+    hipLaunchKernelGGL((k_close<W, 8>), g, blk, 0, st, ring, R, V, b, e, lanes, masks, sel, k, sum, reach);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_close(int W, const u64 *ring, int R, int64_t V, int b, int e,
+                        const LaneOut *lanes, const u64 *level_masks, const u64 *sel, int k_lanes,
+                        u64 *sum_out, u64 *reach_out, hipStream_t st) {
+  if (V <= 0 || e < b) return hipSuccess;
+  switch (W) {
+    case 1: return close_w<1>(ring, R, V, b, e, lanes, level_masks, sel, k_lanes, sum_out, reach_out, st);
+    case 2: return close_w<2>(ring, R, V, b, e, lanes, level_masks, sel, k_lanes, sum_out, reach_out, st);
+    case 4: return close_w<4>(ring, R, V, b, e, lanes, level_masks, sel, k_lanes, sum_out, reach_out, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_groupby(const int64_t *beg, const int32_t *adj, const int32_t *deg,
+                          const int32_t *src, int64_t n, int32_t *key, int32_t *out,
+                          hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_groupby_key, dim3(grid_for(n, 4, 1 << 30)), dim3(BS), 0, st, beg, adj, deg,
+                     src, n, key);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  hipLaunchKernelGGL(k_groupby_place, dim3(grid_for(n, BS, 1 << 30)), dim3(BS), 0, st, key, src, n,
+                     out);
+  return hipGetLastError();
+}
+
+}  // namespace ibfs

@@ -1,0 +1,647 @@
+// Traversal kernels (gfx950, wave64): lane init, inspector, top-down and
+// bottom-up expanders (host-driven and device-driven), bottom-up work model.
+#include <hip/hip_runtime.h>
+
+#include "../../include/ibfs.h"
+#include "device_util.h"
+
+namespace ibfs {
+namespace {
+
+using namespace dev;
+
+__global__ void k_fill_u64(u64 *p, int64_t n, u64 value) {
+  for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
+    p[i] = value;
+}
+
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t value) {
+  for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
+    p[i] = value;
+}
+
+// Lane init: lane lane0+i gets bit of src[i].  OR, so duplicate sources
+// and zero-degree sources keep their bit (the reference stores and skips
+// zero-degree sources, expander.cu:63,70-71).
+template <int W>
+__global__ void k_init_sources(const int32_t *src, int k, int lane0, u64 *S0) {
+  int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= k) return;
+  int lane = lane0 + i;
+  atomicOr(&S0[(int64_t)src[i] * W + (lane >> 6)], 1ull << (lane & 63));
+}
+
+// Direction of level L+1 from the counters of level L (same rule as the
+// host-driven driver): fixed switch level, or alpha/beta.
+__device__ void plan_decide(Plan *p, const LevelCtl *c, int L) {
+  const u64 nch = atomic_read(&c->n_changed);
+  const u64 nf = atomic_read(&c->n_front);
+  const u64 mf = atomic_read(&c->m_front);
+  const u64 mc = atomic_read(&c->m_cand);
+  p->blocks_done = 0;
+  if (nch == 0) {
+    p->done = 1;
+    p->last = L - 1;
+    return;
+  }
+  const int Ln = L + 1;
+  const int dir_prev = (L == 0) ? -1 : p->dir;
+  int dir;
+  switch (p->policy) {
+    case IBFS_POLICY_TD_ONLY: dir = 0; break;
+    case IBFS_POLICY_BU_ONLY: dir = 1; break;
+    case IBFS_POLICY_FIXED: dir = (Ln <= p->sw_level) ? 0 : 1; break;
+    default:
+      if (dir_prev != 1) dir = (mf * (u64)p->alpha > mc) ? 1 : 0;
+      else dir = (nf * (u64)p->beta < p->V && nf < p->nf_prev) ? 0 : 1;
+      break;
+  }
+  p->nf_prev = nf;
+  p->dir = dir;
+  p->dirlog[Ln & 255] = dir;
+  p->level = Ln;
+}
+
+// ---------------------------------------------------------------------------
+// Inspector of level L: one pass over all vertices.  Diffs cur (S_L) against
+// prev (S_{L-1}), copies S_L into next (S_{L+1}, the slot the expanders of
+// L+1 update), refreshes the hub cache from S_L, accumulates the lanes that
+// advanced (lam of L), and compacts, per degree class, both the top-down
+// frontier (changed, degree > 0) and the bottom-up candidates (degree > 0 and
+// missing a lane of lam_prev) of level L+1.  Replaces td/sw_frontier_count +
+// insp_scan + td/sw_frontier_gather + extract_depth of inspector.cu:37-500
+// and scan.cuh:455-904 with a single kernel: per-tile flags in LDS, a
+// workgroup scan and one atomic per class and tile.  Queue order is
+// ascending vertex order within a tile.  Device-driven mode: the level and
+// its slots come from the plan; k_plan_decide then picks the next direction.
+template <int W, int VPT>
+__global__ __launch_bounds__(BS) void k_inspect(InspectArgs a) {
+  constexpr int TILE = BS * VPT;
+  __shared__ uint8_t fl[TILE];
+  __shared__ unsigned wsum[4][6];
+  __shared__ unsigned qbase[6];
+  __shared__ u64 red[4][5 + W];
+
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const u64 *prev = a.prev, *cur = a.cur;
+  u64 *next = a.next;
+  const u64 *lam_prev = a.lam_prev;
+  LevelCtl *ctl = a.ctl, *ctl_zero = a.ctl_zero;
+  if (a.plan) {
+    if (a.plan->done) return;
+    const int L = a.plan->level;
+    const int64_t slot = a.V * W;
+    prev = L > 0 ? a.ring + (int64_t)((L - 1) % a.R) * slot : nullptr;
+    cur = a.ring + (int64_t)(L % a.R) * slot;
+    next = a.ring + (int64_t)((L + 1) % a.R) * slot;
+    ctl = a.ctl_ring + L % a.R;
+    ctl_zero = a.ctl_ring + (L + 1) % a.R;
+    lam_prev = L > 0 ? a.ctl_ring[(L - 1) % a.R].lam : a.lanes_all;
+  }
+  if (blockIdx.x == 0 && ctl_zero && t < (int)(sizeof(LevelCtl) / 8))
+    reinterpret_cast<u64 *>(ctl_zero)[t] = 0;
+  for (int64_t i = (int64_t)blockIdx.x * BS + t; i < a.H; i += (int64_t)gridDim.x * BS) {
+    u64 x[W];
+    load_word<W>(cur + (int64_t)a.hubs[i] * W, x);
+    store_word<W>(a.hs + i * W, x);
+  }
+
+  u64 lp[W], keep[W], nand[W], lamacc[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    lp[w] = lam_prev[w];
+    keep[w] = a.prev_keep ? a.prev_keep[w] : ~0ull;
+    nand[w] = a.next_and ? a.next_and[w] : ~0ull;
+    lamacc[w] = 0;
+  }
+  u64 n_changed = 0, n_front = 0, m_front = 0, n_cand = 0, m_cand = 0;
+  const int64_t V = a.V;
+  const int sml = a.sml, lrg = a.lrg, cs = a.chunk_shift;
+
+  const int64_t ntiles = (V + TILE - 1) / TILE;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t t0 = tile * TILE;
+#pragma unroll
+    for (int i = 0; i < VPT; ++i) {
+      const int64_t v = t0 + i * BS + t;
+      uint8_t f = 0;
+      if (v < V) {
+        u64 c[W], p[W], n[W];
+        load_word<W>(cur + v * W, c);
+        if (prev) {
+          load_word<W>(prev + v * W, p);
+        } else {
+#pragma unroll
+          for (int w = 0; w < W; ++w) p[w] = 0;
+        }
+        bool ch = false, cover = true;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          const u64 d = c[w] & ~(p[w] & keep[w]);
+          lamacc[w] |= d;
+          ch |= (d != 0);
+          n[w] = c[w] & nand[w];
+          cover &= ((c[w] & lp[w]) == lp[w]);
+        }
+        store_word<W>(next + v * W, n);
+        const int dg = a.deg[v];
+        if (ch) {
+          ++n_changed;
+          if (dg > 0) {
+            ++n_front;
+            m_front += (u64)dg;
+            f |= (uint8_t)degree_class(dg, sml, lrg);
+          }
+        }
+        if (dg > 0 && !cover) {
+          ++n_cand;
+          m_cand += (u64)dg;
+          f |= (uint8_t)(degree_class(dg, sml, lrg) << 2);
+        }
+      }
+      fl[i * BS + t] = f;
+    }
+    __syncthreads();
+
+    unsigned cnt[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < VPT; ++j) {
+      const uint8_t f = fl[t * VPT + j];
+      const int tc = f & 3, bc = (f >> 2) & 3;
+      if (tc == 1) {
+        ++cnt[Q_TD_S];
+      } else if (tc == 2) {
+        ++cnt[Q_TD_M];
+      } else if (tc == 3) {
+        const int dg = a.deg[t0 + t * VPT + j];
+        cnt[Q_TD_L] += (unsigned)((dg + (1 << cs) - 1) >> cs);
+      }
+      if (bc) ++cnt[Q_BU_S + bc - 1];
+    }
+    unsigned ex[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const unsigned inc = wave_incl_scan(cnt[k], lane);
+      ex[k] = inc - cnt[k];
+      if (lane == 63) wsum[wv][k] = inc;
+    }
+    __syncthreads();
+    if (t < 6) {
+      const unsigned tot = wsum[0][t] + wsum[1][t] + wsum[2][t] + wsum[3][t];
+      qbase[t] = tot ? atomicAdd(&ctl->q[t], tot) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      unsigned add = qbase[k];
+      for (int w = 0; w < wv; ++w) add += wsum[w][k];
+      ex[k] += add;
+    }
+#pragma unroll
+    for (int j = 0; j < VPT; ++j) {
+      const uint8_t f = fl[t * VPT + j];
+      if (!f) continue;
+      const int32_t v = (int32_t)(t0 + t * VPT + j);
+      const int tc = f & 3, bc = (f >> 2) & 3;
+      if (tc == 1) {
+        a.q.q[Q_TD_S][ex[Q_TD_S]++] = v;
+      } else if (tc == 2) {
+        a.q.q[Q_TD_M][ex[Q_TD_M]++] = v;
+      } else if (tc == 3) {
+        const int nch = (a.deg[v] + (1 << cs) - 1) >> cs;
+        for (int c = 0; c < nch; ++c) a.q.chunks[ex[Q_TD_L]++] = make_int2(v, c);
+      }
+      if (bc == 1) a.q.q[Q_BU_S][ex[Q_BU_S]++] = v;
+      else if (bc == 2) a.q.q[Q_BU_M][ex[Q_BU_M]++] = v;
+      else if (bc == 3) a.q.q[Q_BU_L][ex[Q_BU_L]++] = v;
+    }
+    __syncthreads();
+  }
+
+  // workgroup reduction of the level counters, one atomic each
+  u64 vals[5 + W] = {n_changed, n_front, m_front, n_cand, m_cand};
+#pragma unroll
+  for (int w = 0; w < W; ++w) vals[5 + w] = lamacc[w];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) vals[i] = wave_sum(vals[i]);
+#pragma unroll
+  for (int w = 0; w < W; ++w) vals[5 + w] = wave_or(vals[5 + w]);
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < 5 + W; ++i) red[wv][i] = vals[i];
+  }
+  __syncthreads();
+  if (t < 5) {
+    const u64 s = red[0][t] + red[1][t] + red[2][t] + red[3][t];
+    if (s) atomicAdd(&(&ctl->n_changed)[t], s);
+  } else if (t < 5 + W) {
+    const u64 s = red[0][t] | red[1][t] | red[2][t] | red[3][t];
+    if (s) atomicOr(&ctl->lam[t - 5], s);
+  }
+}
+
+// Device-driven decision after the inspection of plan->level (one thread).
+// A separate launch instead of a last-workgroup election inside the
+// inspector: on this multi-XCD part each workgroup's agent-scope fence
+// would write back and invalidate its L2 in the middle of the pass.
+__global__ void k_plan_decide(Plan *plan, const LevelCtl *ctl_ring, int R) {
+  if (threadIdx.x != 0 || plan->done) return;
+  const int L = plan->level;
+  plan_decide(plan, ctl_ring + L % R, L);
+}
+
+// ---------------------------------------------------------------------------
+// Top-down: a frontier vertex u pushes S_{L-1}[u] & lam_{L-1} into S_L of its
+// neighbours with 64-bit atomic ORs, skipping neighbours that already hold
+// the bits (expander.cu:100-252).
+template <int W>
+__device__ __forceinline__ void td_push(const int64_t *__restrict__ beg,
+                                        const int32_t *__restrict__ adj,
+                                        const u64 *__restrict__ prev, u64 *cur,
+                                        const u64 (&lam)[W], int32_t u, int64_t j0, int64_t j1,
+                                        int stride) {
+  u64 p[W];
+  load_word<W>(prev + (int64_t)u * W, p);
+#pragma unroll
+  for (int w = 0; w < W; ++w) p[w] &= lam[w];
+  for (int64_t j = j0; j < j1; j += stride) {
+    const int64_t v = adj[j];
+    u64 c[W];
+    load_word<W>(cur + v * W, c);
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+      if (p[w] & ~c[w]) atomicOr(&cur[v * W + w], p[w]);
+  }
+}
+
+// chunk i of a large top-down vertex: edges [lo, hi)
+__device__ __forceinline__ void td_chunk_range(const int64_t *__restrict__ beg, int2 e, int cs,
+                                               int64_t &lo, int64_t &hi) {
+  lo = beg[e.x] + ((int64_t)e.y << cs);
+  const int64_t end = beg[e.x + 1];
+  hi = (lo + (1ll << cs)) < end ? (lo + (1ll << cs)) : end;
+}
+
+// Bottom-up: a candidate v ORs S_{L-1} of its neighbours (restricted to
+// lam_{L-1}, the only lanes that can still be new) in traversal-adjacency
+// order (descending degree, hubs first) and stops as soon as its word
+// covers lam_{L-1} (bitwise early termination, expander.cu:258-359).
+// One candidate per thread, four neighbours per step.
+template <int W>
+__device__ __forceinline__ void bu_thread(const int64_t *__restrict__ beg,
+                                          const int32_t *__restrict__ tadj,
+                                          const u64 *__restrict__ prev,
+                                          const u64 *__restrict__ hs, u64 *__restrict__ cur,
+                                          const u64 (&lam)[W], int64_t v) {
+  u64 x[W], x0[W];
+  load_word<W>(cur + v * W, x);
+#pragma unroll
+  for (int w = 0; w < W; ++w) x0[w] = x[w];
+  if (covers<W>(x, lam)) return;
+  int64_t j = beg[v];
+  const int64_t e = beg[v + 1];
+  while (j < e) {
+    const int32_t a0 = tadj[j];
+    const int32_t a1 = (j + 1 < e) ? tadj[j + 1] : a0;
+    const int32_t a2 = (j + 2 < e) ? tadj[j + 2] : a0;
+    const int32_t a3 = (j + 3 < e) ? tadj[j + 3] : a0;
+    u64 s0[W], s1[W], s2[W], s3[W];
+    load_nbr<W>(prev, hs, a0, s0);
+    load_nbr<W>(prev, hs, a1, s1);
+    load_nbr<W>(prev, hs, a2, s2);
+    load_nbr<W>(prev, hs, a3, s3);
+#pragma unroll
+    for (int w = 0; w < W; ++w) x[w] |= (s0[w] | s1[w] | s2[w] | s3[w]) & lam[w];
+    j += 4;
+    if (covers<W>(x, lam)) break;
+  }
+  if (differs<W>(x, x0)) store_word<W>(cur + v * W, x);
+}
+
+// One candidate per wave: 64 neighbours per step, wave OR-reduction.
+template <int W>
+__device__ __forceinline__ void bu_wave(const int64_t *__restrict__ beg,
+                                        const int32_t *__restrict__ tadj,
+                                        const u64 *__restrict__ prev, const u64 *__restrict__ hs,
+                                        u64 *__restrict__ cur, const u64 (&lam)[W], int64_t v,
+                                        int lane) {
+  u64 x[W], x0[W];
+  load_word<W>(cur + v * W, x);
+#pragma unroll
+  for (int w = 0; w < W; ++w) x0[w] = x[w];
+  if (covers<W>(x, lam)) return;
+  const int64_t s = beg[v], e = beg[v + 1];
+  for (int64_t r = s; r < e; r += 64) {
+    const int64_t j = r + lane;
+    u64 y[W];
+    if (j < e) {
+      load_nbr<W>(prev, hs, tadj[j], y);
+    } else {
+#pragma unroll
+      for (int w = 0; w < W; ++w) y[w] = 0;
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w) x[w] |= wave_or(y[w] & lam[w]);
+    if (covers<W>(x, lam)) break;
+  }
+  if (lane == 0 && differs<W>(x, x0)) store_word<W>(cur + v * W, x);
+}
+
+// One candidate per workgroup: 256 neighbours per step, the four wave
+// partials combined through LDS.  Must be called by the whole workgroup.
+template <int W>
+__device__ __forceinline__ void bu_block(const int64_t *__restrict__ beg,
+                                         const int32_t *__restrict__ tadj,
+                                         const u64 *__restrict__ prev, const u64 *__restrict__ hs,
+                                         u64 *__restrict__ cur, const u64 (&lam)[W], int64_t v,
+                                         u64 (*part)[W]) {
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  u64 x[W], x0[W];
+  load_word<W>(cur + v * W, x);
+#pragma unroll
+  for (int w = 0; w < W; ++w) x0[w] = x[w];
+  if (covers<W>(x, lam)) return;
+  const int64_t s = beg[v], e = beg[v + 1];
+  for (int64_t r = s; r < e; r += BS) {
+    const int64_t j = r + t;
+    u64 y[W];
+    if (j < e) {
+      load_nbr<W>(prev, hs, tadj[j], y);
+    } else {
+#pragma unroll
+      for (int w = 0; w < W; ++w) y[w] = 0;
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w) y[w] = wave_or(y[w] & lam[w]);
+    if (lane == 0) {
+#pragma unroll
+      for (int w = 0; w < W; ++w) part[wv][w] = y[w];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < W; ++w) x[w] |= part[0][w] | part[1][w] | part[2][w] | part[3][w];
+    __syncthreads();
+    if (covers<W>(x, lam)) break;
+  }
+  if (t == 0 && differs<W>(x, x0)) store_word<W>(cur + v * W, x);
+}
+
+// Host-driven top-down: workgroups [0, nbs) take small vertices one per
+// thread, [nbs, nbs+nbm) mid vertices one per wave, the rest one chunk of
+// 2^chunk_shift edges of a large vertex per workgroup.
+template <int W>
+__global__ __launch_bounds__(BS) void k_expand_td(const int64_t *__restrict__ beg,
+                                                  const int32_t *__restrict__ adj,
+                                                  const u64 *__restrict__ prev, u64 *cur,
+                                                  const u64 *__restrict__ lam_prev, Queues q,
+                                                  int64_t ns, int64_t nm, int64_t nl, int64_t nbs,
+                                                  int64_t nbm, int cs) {
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int64_t b = blockIdx.x;
+  u64 lam[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) lam[w] = lam_prev[w];
+  if (b < nbs) {
+    const int64_t i = b * BS + t;
+    if (i >= ns) return;
+    const int32_t u = q.q[Q_TD_S][i];
+    td_push<W>(beg, adj, prev, cur, lam, u, beg[u], beg[u + 1], 1);
+  } else if (b < nbs + nbm) {
+    const int64_t i = (b - nbs) * 4 + wv;
+    if (i >= nm) return;
+    const int32_t u = q.q[Q_TD_M][i];
+    td_push<W>(beg, adj, prev, cur, lam, u, beg[u] + lane, beg[u + 1], 64);
+  } else {
+    const int64_t i = b - nbs - nbm;
+    if (i >= nl) return;
+    const int2 e = q.chunks[i];
+    int64_t lo, hi;
+    td_chunk_range(beg, e, cs, lo, hi);
+    td_push<W>(beg, adj, prev, cur, lam, e.x, lo + t, hi, BS);
+  }
+}
+
+// Host-driven bottom-up, same workgroup ranges as k_expand_td.
+template <int W>
+__global__ __launch_bounds__(BS) void k_expand_bu(const int64_t *__restrict__ beg,
+                                                  const int32_t *__restrict__ tadj,
+                                                  const u64 *__restrict__ prev,
+                                                  const u64 *__restrict__ hs,
+                                                  u64 *__restrict__ cur,
+                                                  const u64 *__restrict__ lam_prev, Queues q,
+                                                  int64_t ns, int64_t nm, int64_t nl, int64_t nbs,
+                                                  int64_t nbm) {
+  __shared__ u64 part[4][W];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int64_t b = blockIdx.x;
+  u64 lam[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) lam[w] = lam_prev[w];
+  if (b < nbs) {
+    const int64_t i = b * BS + t;
+    if (i < ns) bu_thread<W>(beg, tadj, prev, hs, cur, lam, q.q[Q_BU_S][i]);
+  } else if (b < nbs + nbm) {
+    const int64_t i = (b - nbs) * 4 + wv;
+    if (i < nm) bu_wave<W>(beg, tadj, prev, hs, cur, lam, q.q[Q_BU_M][i], lane);
+  } else {
+    const int64_t i = b - nbs - nbm;
+    if (i < nl) bu_block<W>(beg, tadj, prev, hs, cur, lam, q.q[Q_BU_L][i], part);
+  }
+}
+
+// Device-driven expansion of level plan->level: a fixed grid strides over
+// the three class queues of the direction the inspector chose.
+template <int W>
+__global__ __launch_bounds__(BS) void k_expand_dd(const int64_t *__restrict__ beg,
+                                                  const int32_t *__restrict__ adj,
+                                                  const int32_t *__restrict__ tadj, u64 *ring,
+                                                  int R, int64_t V,
+                                                  const LevelCtl *__restrict__ ctl_ring,
+                                                  const Plan *__restrict__ plan,
+                                                  const u64 *__restrict__ hs, Queues q, int cs) {
+  __shared__ u64 part[4][W];
+  if (plan->done || plan->level < 1) return;
+  const int L = plan->level, dir = plan->dir;
+  const LevelCtl *c = ctl_ring + (L - 1) % R;
+  const int64_t slot = V * W;
+  const u64 *prev = ring + (int64_t)((L - 1) % R) * slot;
+  u64 *cur = ring + (int64_t)(L % R) * slot;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int64_t G = gridDim.x;
+  u64 lam[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) lam[w] = c->lam[w];
+  if (dir == 0) {
+    const int64_t ns = c->q[Q_TD_S], nm = c->q[Q_TD_M], nl = c->q[Q_TD_L];
+    for (int64_t i = (int64_t)blockIdx.x * BS + t; i < ns; i += G * BS) {
+      const int32_t u = q.q[Q_TD_S][i];
+      td_push<W>(beg, adj, prev, cur, lam, u, beg[u], beg[u + 1], 1);
+    }
+    for (int64_t i = (int64_t)blockIdx.x * 4 + wv; i < nm; i += G * 4) {
+      const int32_t u = q.q[Q_TD_M][i];
+      td_push<W>(beg, adj, prev, cur, lam, u, beg[u] + lane, beg[u + 1], 64);
+    }
+    for (int64_t i = blockIdx.x; i < nl; i += G) {
+      const int2 e = q.chunks[i];
+      int64_t lo, hi;
+      td_chunk_range(beg, e, cs, lo, hi);
+      td_push<W>(beg, adj, prev, cur, lam, e.x, lo + t, hi, BS);
+    }
+  } else {
+    const int64_t ns = c->q[Q_BU_S], nm = c->q[Q_BU_M], nl = c->q[Q_BU_L];
+    for (int64_t i = (int64_t)blockIdx.x * BS + t; i < ns; i += G * BS)
+      bu_thread<W>(beg, tadj, prev, hs, cur, lam, q.q[Q_BU_S][i]);
+    for (int64_t i = (int64_t)blockIdx.x * 4 + wv; i < nm; i += G * 4)
+      bu_wave<W>(beg, tadj, prev, hs, cur, lam, q.q[Q_BU_M][i], lane);
+    for (int64_t i = blockIdx.x; i < nl; i += G)
+      bu_block<W>(beg, tadj, prev, hs, cur, lam, q.q[Q_BU_L][i], part);
+  }
+}
+
+// Work model of a bottom-up level (instrumentation, not on the timed path):
+// out[0] = vertices of degree > 0 whose S_{L-1} word misses a lane of
+// lam_{L-1}; out[1] = sum over them of the sequential traversal-adjacency
+// prefix after which the word covers lam_{L-1} (the whole list if never).
+template <int W>
+__global__ __launch_bounds__(BS) void k_bu_model(const int64_t *__restrict__ beg,
+                                                 const int32_t *__restrict__ tadj,
+                                                 const u64 *__restrict__ prev,
+                                                 const u64 *__restrict__ hs,
+                                                 const u64 *__restrict__ lam_prev, int64_t V,
+                                                 u64 *out) {
+  __shared__ u64 red[2][4];
+  u64 lam[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) lam[w] = lam_prev[w];
+  u64 nv = 0, pref = 0;
+  for (int64_t v = (int64_t)blockIdx.x * BS + threadIdx.x; v < V; v += (int64_t)gridDim.x * BS) {
+    const int64_t s = beg[v], e = beg[v + 1];
+    if (s == e) continue;
+    u64 x[W];
+    load_word<W>(prev + v * W, x);
+    if (covers<W>(x, lam)) continue;
+    ++nv;
+    for (int64_t j = s; j < e; ++j) {
+      u64 y[W];
+      load_nbr<W>(prev, hs, tadj[j], y);
+#pragma unroll
+      for (int w = 0; w < W; ++w) x[w] |= y[w] & lam[w];
+      ++pref;
+      if (covers<W>(x, lam)) break;
+    }
+  }
+  nv = wave_sum(nv);
+  pref = wave_sum(pref);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = nv;
+    red[1][threadIdx.x >> 6] = pref;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    const u64 s = red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] +
+                  red[threadIdx.x][3];
+    if (s) atomicAdd(&out[threadIdx.x], s);
+  }
+}
+
+#define IBFS_DISPATCH_W(W, STMT)             \
+  switch (W) {                               \
+    case 1: { constexpr int WW = 1; STMT; break; } \
+    case 2: { constexpr int WW = 2; STMT; break; } \
+    case 4: { constexpr int WW = 4; STMT; break; } \
+    default: return hipErrorInvalidValue;    \
+  }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+hipError_t launch_fill_u64(u64 *p, int64_t n, u64 value, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, BS * 8, 8192)), dim3(BS), 0, st, p, n, value);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t value, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n, BS * 8, 8192)), dim3(BS), 0, st, p, n, value);
+  return hipGetLastError();
+}
+
+hipError_t launch_init_sources(int W, const int32_t *src, int k, int lane0, u64 *S0,
+                               hipStream_t st) {
+  if (k <= 0) return hipSuccess;
+  dim3 g(grid_for(k, BS, 1 << 20)), blk(BS);
+  IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_init_sources<WW>, g, blk, 0, st, src, k, lane0, S0));
+  return hipGetLastError();
+}
+
+template <int W>
+static void inspect_w(const InspectArgs &a, hipStream_t st) {
+  // vertices per thread: 16 when that still gives >= 1024 tiles, else 4, else 1
+  const int64_t V = a.V;
+  if (V >= (int64_t)BS * 16 * 1024)
+    hipLaunchKernelGGL((k_inspect<W, 16>), dim3(grid_for(V, BS * 16, 4096)), dim3(BS), 0, st, a);
+  else if (V >= (int64_t)BS * 4 * 512)
+    hipLaunchKernelGGL((k_inspect<W, 4>), dim3(grid_for(V, BS * 4, 4096)), dim3(BS), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_inspect<W, 1>), dim3(grid_for(V, BS, 4096)), dim3(BS), 0, st, a);
+}
+
+hipError_t launch_inspect(int W, const InspectArgs &a, hipStream_t st) {
+  if (a.V <= 0) return hipSuccess;
+  IBFS_DISPATCH_W(W, inspect_w<WW>(a, st));
+  return hipGetLastError();
+}
+
+hipError_t launch_expand_td(int W, const int64_t *beg, const int32_t *adj, const u64 *prev,
+                            u64 *cur, const u64 *lam_prev, Queues q, int64_t ns, int64_t nm,
+                            int64_t nl, int chunk_shift, hipStream_t st) {
+  const int64_t nbs = (ns + BS - 1) / BS, nbm = (nm + 3) / 4;
+  const int64_t nb = nbs + nbm + nl;
+  if (nb == 0) return hipSuccess;
+  if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
+  dim3 g((unsigned)nb), blk(BS);
+  IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_expand_td<WW>, g, blk, 0, st, beg, adj, prev, cur,
+                                        lam_prev, q, ns, nm, nl, nbs, nbm, chunk_shift));
+  return hipGetLastError();
+}
+
+hipError_t launch_expand_bu(int W, const int64_t *beg, const int32_t *tadj, const u64 *prev,
+                            const u64 *hs, u64 *cur, const u64 *lam_prev, Queues q, int64_t ns,
+                            int64_t nm, int64_t nl, hipStream_t st) {
+  const int64_t nbs = (ns + BS - 1) / BS, nbm = (nm + 3) / 4;
+  const int64_t nb = nbs + nbm + nl;
+  if (nb == 0) return hipSuccess;
+  if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
+  dim3 g((unsigned)nb), blk(BS);
+  IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_expand_bu<WW>, g, blk, 0, st, beg, tadj, prev, hs, cur,
+                                        lam_prev, q, ns, nm, nl, nbs, nbm));
+  return hipGetLastError();
+}
+
+hipError_t launch_expand_dd(int W, const int64_t *beg, const int32_t *adj, const int32_t *tadj,
+                            u64 *ring, int R, int64_t V, const LevelCtl *ctl_ring,
+                            const Plan *plan, const u64 *hs, Queues q, int chunk_shift,
+                            int grid, hipStream_t st) {
+  dim3 g(grid > 0 ? grid : 1), blk(BS);
+  IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_expand_dd<WW>, g, blk, 0, st, beg, adj, tadj, ring, R,
+                                        V, ctl_ring, plan, hs, q, chunk_shift));
+  return hipGetLastError();
+}
+
+hipError_t launch_plan_decide(Plan *plan, const LevelCtl *ctl_ring, int R, hipStream_t st) {
+  hipLaunchKernelGGL(k_plan_decide, dim3(1), dim3(64), 0, st, plan, ctl_ring, R);
+  return hipGetLastError();
+}
+
+hipError_t launch_bu_model(int W, const int64_t *beg, const int32_t *tadj, const u64 *prev,
+                           const u64 *hs, const u64 *lam_prev, int64_t V, u64 *out,
+                           hipStream_t st) {
+  if (V <= 0) return hipSuccess;
+  dim3 g(grid_for(V, BS, 8192)), blk(BS);
+  IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_bu_model<WW>, g, blk, 0, st, beg, tadj, prev, hs,
+                                        lam_prev, V, out));
+  return hipGetLastError();
+}
+
+}  // namespace ibfs

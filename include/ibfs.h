@@ -67,7 +67,8 @@ typedef struct ibfs_config {
                           prefix into the level log (instrumentation)        */
   int32_t emit_mode;   /* depth emission: 0 per-thread stores, 1 LDS-tiled,
                           2 per-thread non-temporal stores                   */
-  int32_t reserved[2];
+  int32_t dd_grid;     /* workgroups of the device-driven expander (0: auto)  */
+  int32_t reserved[1];
 } ibfs_config;
 
 /* Per-level record of the last traversal (the build's monitoring log, in place
@@ -205,6 +206,27 @@ int ibfs_engine_level_log(ibfs_engine_t e, ibfs_level_info *out, int32_t cap,
  * main loop, main.cu:76-90); depth_out is [src_count][V]. */
 int ibfs_run_groups(ibfs_engine_t e, const int32_t *src, int64_t src_count,
                     int32_t *depth_out, int32_t out_place, ibfs_stats *stats);
+
+/* ---- batched launcher ---------------------------------------------------- */
+/* Runs an arbitrary source set packed into lane groups of cfg->lane_bits with
+ * `contexts` traversals in flight on separate HIP streams.  Each level is a
+ * captured hipGraph of two kernels (expand, inspect) whose direction and
+ * queue sizes are decided on the device; the host polls once per `burst`
+ * levels.  depth_out is [n][V] as for ibfs_run_groups.  No reference
+ * counterpart: the reference runs one 128-source group at a time with ~13
+ * host synchronisations per level (main.cu:76-90, traversal.cu:101-130). */
+typedef struct ibfs_batch_s *ibfs_batch_t;
+int ibfs_batch_create(ibfs_graph_t g, const ibfs_config *cfg, int32_t contexts,
+                      int32_t burst, ibfs_batch_t *out);
+int ibfs_batch_run(ibfs_batch_t b, const int32_t *src, int64_t n,
+                   int32_t *depth_out, int32_t out_place, ibfs_stats *stats);
+int ibfs_batch_destroy(ibfs_batch_t b);
+/* Closeness inputs for every source without materialising depth arrays:
+ * depth_sum[i] = sum of depth_i(v) over reached v, reach[i] = number of
+ * vertices reached (source included).  Host arrays of n int64.  The
+ * reference lists closeness as an intended use (readme.md:46-51). */
+int ibfs_batch_closeness(ibfs_batch_t b, const int32_t *src, int64_t n,
+                         int64_t *depth_sum, int64_t *reach, ibfs_stats *stats);
 
 /* TEPS numerator of one depth array: sum of deg(v) over reached v
  * (validate.h:59-76 report()). depth is a host array. */

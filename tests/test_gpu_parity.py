@@ -131,6 +131,66 @@ def test_s16_hub_cache(s16, hubs):
         np.testing.assert_array_equal(eng.run_groups(s16["src"]), s16["want"])
 
 
+@pytest.mark.parametrize("lane_bits", [64, 128, 256])
+@pytest.mark.parametrize("contexts,burst", [(1, 1), (2, 3), (4, 8)])
+def test_batch_s16(s16, lane_bits, contexts, burst):
+    say(f"[gpu] batch s16 lanes={lane_bits} ctx={contexts} burst={burst}")
+    b = ibfs_amd.Batch(s16["g"], contexts=contexts, burst=burst, lane_bits=lane_bits)
+    src = np.concatenate([s16["src"], s16["src"][:37]])  # uneven: 165 sources
+    want = np.concatenate([s16["want"], s16["want"][:37]])
+    np.testing.assert_array_equal(b.run(src), want)
+    st = b.last_stats
+    assert st.traversed_edges == sum(oracle.traversed_edges(s16["beg"], d) for d in want)
+    V = s16["h"].vert_count
+    buf = ibfs_amd.DeviceBuffer(4 * len(src) * V)
+    b.run(src, out=buf)
+    np.testing.assert_array_equal(buf.to_host().reshape(len(src), V), want)
+
+
+@pytest.mark.parametrize("pol", list(POLICIES))
+def test_batch_golden_policies_and_ring_wrap(pol):
+    say(f"[gpu] batch golden {pol}")
+    for name in ("bfs_classes1500.npz", "bfs_twocomp.npz", "bfs_grid6x7.npz"):
+        z = load_golden(name)
+        g = ibfs_amd.DeviceGraph(ibfs_amd.HostCsr.from_arrays(z["beg_pos"], z["adj"]))
+        V = len(z["beg_pos"]) - 1
+        src = np.arange(V, dtype=np.int32)
+        for ring in (4, 7, 32):
+            b = ibfs_amd.Batch(g, contexts=3, burst=5, ring_levels=ring, **POLICIES[pol])
+            np.testing.assert_array_equal(b.run(src), z["depth"], err_msg=f"{name} ring={ring}")
+
+
+def _close_ref(depth):
+    d = depth.astype(np.int64)
+    reached = d >= 0
+    return np.where(reached, d, 0).sum(axis=1), reached.sum(axis=1)
+
+
+@pytest.mark.parametrize("lane_bits", [64, 128, 256])
+def test_closeness_matches_depth_arrays_s16(s16, lane_bits):
+    say(f"[gpu] closeness s16 lanes={lane_bits}")
+    b = ibfs_amd.Batch(s16["g"], contexts=2, burst=4, lane_bits=lane_bits)
+    src = np.concatenate([s16["src"], s16["src"][:21]])
+    dsum, reach = b.closeness(src)
+    want_sum, want_reach = _close_ref(np.concatenate([s16["want"], s16["want"][:21]]))
+    np.testing.assert_array_equal(reach, want_reach)
+    np.testing.assert_array_equal(dsum, want_sum)
+
+
+@pytest.mark.parametrize("ring", [4, 6, 32])
+def test_closeness_golden_all_sources(ring):
+    say(f"[gpu] closeness golden ring={ring}")
+    for name in ("bfs_classes1500.npz", "bfs_twocomp.npz", "bfs_er300.npz"):
+        z = load_golden(name)
+        g = ibfs_amd.DeviceGraph(ibfs_amd.HostCsr.from_arrays(z["beg_pos"], z["adj"]))
+        V = len(z["beg_pos"]) - 1
+        b = ibfs_amd.Batch(g, contexts=2, burst=3, ring_levels=ring)
+        dsum, reach = b.closeness(np.arange(V, dtype=np.int32))
+        want_sum, want_reach = _close_ref(z["depth"])
+        np.testing.assert_array_equal(reach, want_reach, err_msg=name)
+        np.testing.assert_array_equal(dsum, want_sum, err_msg=name)
+
+
 def test_device_output_and_determinism(s16):
     say("[gpu] device output")
     eng = ibfs_amd.Engine(s16["g"])
