@@ -1,13 +1,19 @@
-"""Aggregate GTEPS of concurrent multi-source BFS on one MI355X.
+"""Aggregate GTEPS of concurrent multi-source BFS on one MI355X, plain vs
+batched vs served.
 
-Default workload: R-MAT scale 22, edge factor 16 (a,b,c = .57,.19,.19,
-seeded, vertex ids permuted, cleaned and symmetrised), 1,024 seeded sources
-of degree > 0, GroupBy-ordered, run as 8 groups of 128 lanes with the
-direction-optimising policy.  One step = the 8 groups (all 1,024 depth arrays
-written to device memory).  GTEPS = sum over sources of deg(v) over reached v
-(directed CSR entries) / wall time of the step.
+Plain (the headline `value`): R-MAT scale 22, edge factor 16 (a,b,c =
+.57,.19,.19, seeded, vertex ids permuted, cleaned and symmetrised), 1,024
+seeded sources of degree > 0, GroupBy-ordered, run as 8 groups of 128 lanes
+one after another with the direction-optimising policy; one step = the 8
+groups with all 1,024 depth arrays written to device memory.  GTEPS = sum
+over sources of deg(v) over reached v (directed CSR entries) / wall time.
 
-Prints one JSON line (see the README of the result fields in DESIGN.md).
+Batched: the same 1,024 sources through the batched launcher (wider lane
+words, several traversal contexts in flight, device-driven levels).
+Served: Poisson-arriving queries of 1-256 sources (log-uniform) against the
+R-MAT s22 and a Chung-Lu graph, both resident; p50/p99 latency per rate.
+
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -15,6 +21,7 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
 from pathlib import Path
 
@@ -63,6 +70,145 @@ def alg_bytes_per_kernel(log_levels, V, W, k):
     return out
 
 
+def plain_leg(g, src, K, steps, warmup, tune, emit_mode, out):
+    eng = ibfs_amd.Engine(g, lane_bits=K, kernel_timing=1, emit_mode=emit_mode, **tune)
+    for i in range(warmup):
+        eng.run_groups(src, out=out)
+    kern = {"expand_td": 0.0, "expand_bu": 0.0, "inspect": 0.0, "emit": 0.0}
+    launches = dict.fromkeys(kern, 0)
+    edges = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.run_groups(src, out=out)
+        st = eng.last_stats
+        edges += st.traversed_edges
+        kern["expand_td"] += st.expand_td_ms
+        kern["expand_bu"] += st.expand_bu_ms
+        kern["inspect"] += st.inspect_ms
+        kern["emit"] += st.emit_ms
+        launches["expand_td"] += st.n_td_launches
+        launches["expand_bu"] += st.n_bu_launches
+        launches["inspect"] += st.n_inspect_launches
+        launches["emit"] += st.n_emit_launches
+    wall = time.perf_counter() - t0
+    return dict(gteps=edges / wall / 1e9, ms_per_step=wall * 1e3 / steps, edges_per_step=edges // steps,
+                kern_ms={k: v / steps for k, v in kern.items()},
+                launches={k: v // steps for k, v in launches.items()}, cfg=eng.cfg)
+
+
+def roofline_leg(g, src, K, V, tune, kern_ms, launches, steps):
+    model = ibfs_amd.Engine(g, lane_bits=K, work_model=1, **tune)
+    scratch = ibfs_amd.DeviceBuffer(4 * K * V)
+    alg = {"expand_td": 0, "expand_bu": 0, "inspect": 0, "emit": 0}
+    levels = []
+    for g0 in range(0, len(src), K):
+        grp = src[g0:g0 + K]
+        model.traversal(grp, out=scratch)
+        lv = model.level_log()
+        levels.append([(x["level"], x["direction"]) for x in lv])
+        for kname, b in alg_bytes_per_kernel(lv, V, K // 8, len(grp)).items():
+            alg[kname] += b
+    dom = max(kern_ms, key=kern_ms.get)
+    achieved = alg[dom] / (kern_ms[dom] * 1e-3) / 1e9 if kern_ms[dom] > 0 else 0.0
+    kernels = {k: {"ms_per_step": round(kern_ms[k], 4), "launches_per_step": launches[k],
+                   "alg_GB_per_step": round(alg[k] / 1e9, 4),
+                   "alg_GBps": round(alg[k] / (kern_ms[k] * 1e-3) / 1e9, 1) if kern_ms[k] > 0 else None}
+               for k in kern_ms}
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
+    return roof, kernels, levels
+
+
+def batch_leg(g, src, K, contexts, burst, steps, warmup, tune, emit_mode, out):
+    b = ibfs_amd.Batch(g, contexts=contexts, burst=burst, lane_bits=K, emit_mode=emit_mode, **tune)
+    for _ in range(warmup):
+        b.run(src, out=out)
+    edges = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.run(src, out=out)
+        edges += b.last_stats.traversed_edges
+    wall = time.perf_counter() - t0
+    return dict(gteps=edges / wall / 1e9, ms_per_step=wall * 1e3 / steps, lane_bits=K,
+                contexts=contexts, burst=burst)
+
+
+def edges_of(g, sources, chunk=256):
+    """Traversed edges of each source set (untimed, through the batched form)."""
+    b = ibfs_amd.Batch(g, contexts=1, burst=8, lane_bits=256)
+    buf = ibfs_amd.DeviceBuffer(4 * chunk * g.vert_count)
+    total = 0
+    for i in range(0, len(sources), chunk):
+        b.run(sources[i:i + chunk], out=buf)
+        total += b.last_stats.traversed_edges
+    return total
+
+
+def served_leg(graphs, rates, n_queries, seed, K):
+    """Poisson arrivals at each rate; every query names one graph and carries
+    a log-uniform 1..256 source set.  Latency = done - submit."""
+    rng = np.random.default_rng(seed)
+    servers = [ibfs_amd.Server(gr["g"], lane_bits=K, status_bytes=gr["status_bytes"]) for gr in graphs]
+    results = []
+    for lam in rates:
+        gi = rng.integers(0, len(graphs), size=n_queries)
+        sizes = np.floor(np.exp(rng.uniform(0, np.log(256), size=n_queries))).astype(int)
+        qs = [rng.choice(graphs[i]["pool"], size=int(n)).astype(np.int32) for i, n in zip(gi, sizes)]
+        gaps = rng.exponential(1.0 / lam, size=n_queries)
+        ids = [None] * n_queries
+        # outputs allocated before the arrivals start: no allocation or free
+        # inside the measured window
+        bufs = [ibfs_amd.DeviceBuffer(4 * len(q) * graphs[i]["g"].vert_count) for q, i in zip(qs, gi)]
+        lat = [None] * n_queries
+
+        def producer():
+            t_next = time.perf_counter()
+            for i in range(n_queries):
+                t_next += gaps[i]
+                d = t_next - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                ids[i] = servers[gi[i]].submit(qs[i], bufs[i])
+
+        def consumer():
+            for i in range(n_queries):
+                while ids[i] is None:
+                    time.sleep(0.0002)
+                info = servers[gi[i]].wait(ids[i], timeout_ms=120000)
+                lat[i] = info
+                servers[gi[i]].release(ids[i])
+
+        tp, tc = threading.Thread(target=producer), threading.Thread(target=consumer)
+        tp.start()
+        tc.start()
+        tp.join()
+        tc.join()
+        for b in bufs:
+            b.free()
+        t_first = min(x["submit_ms"] for x in lat)
+        t_last = max(x["done_ms"] for x in lat)
+        lats = np.array([x["latency_ms"] for x in lat])
+        edges = 0
+        for k, gr in enumerate(graphs):
+            sel = [q for q, i in zip(qs, gi) if i == k]
+            if sel:
+                edges += edges_of(gr["g"], np.concatenate(sel))
+        span_s = (t_last - t_first) / 1e3
+        results.append({"lambda_qps": lam, "queries": n_queries,
+                        "sources": int(sum(len(q) for q in qs)),
+                        "p50_ms": round(float(np.percentile(lats, 50)), 3),
+                        "p99_ms": round(float(np.percentile(lats, 99)), 3),
+                        "mean_ms": round(float(lats.mean()), 3),
+                        "gteps": round(edges / span_s / 1e9, 3), "span_ms": round(span_s * 1e3, 2)})
+        log(f"served lambda={lam}/s: p50 {results[-1]['p50_ms']} ms p99 {results[-1]['p99_ms']} ms "
+            f"{results[-1]['gteps']} GTEPS over {results[-1]['span_ms']} ms")
+    info = {"graphs": [gr["name"] for gr in graphs], "lane_bits": K,
+            "contexts": [s.contexts for s in servers], "rates": results}
+    for s in servers:
+        s.close()
+    return info
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,10 +223,13 @@ def main() -> int:
     ap.add_argument("--hubs", type=int, default=None)
     ap.add_argument("--alpha", type=int, default=None)
     ap.add_argument("--beta", type=int, default=None)
-    ap.add_argument("--batch", type=int, default=0,
-                    help="contexts of the batched launcher (0: plain engine)")
+    ap.add_argument("--batch-contexts", type=int, default=2)
+    ap.add_argument("--batch-lanes", type=int, default=256)
     ap.add_argument("--burst", type=int, default=8)
-    ap.add_argument("--dd-grid", type=int, default=0)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-queries", type=int, default=120)
+    ap.add_argument("--serve-rates", default="200,800")
     args = ap.parse_args()
     tune = {}
     if args.alpha is not None:
@@ -105,67 +254,28 @@ def main() -> int:
     log(f"graph V={V} E={E} in {time.perf_counter() - t0:.1f}s")
     src0 = h.gen_sources(wl["sources"], seed=2)
     g = ibfs_amd.DeviceGraph(h, device=0, hub_count=args.hubs)
-    # GroupBy on the device (timed separately, outside the step)
     g.groupby(src0)
     t0 = time.perf_counter()
     src = g.groupby(src0)
     groupby_ms = (time.perf_counter() - t0) * 1e3
     K = args.lane_bits
     n = len(src)
-    if args.batch:
-        eng = ibfs_amd.Batch(g, contexts=args.batch, burst=args.burst, lane_bits=K,
-                             emit_mode=args.emit_mode, dd_grid=args.dd_grid, **tune)
-        eng.run_groups = eng.run
-    else:
-        eng = ibfs_amd.Engine(g, lane_bits=K, kernel_timing=1, emit_mode=args.emit_mode, **tune)
     out = ibfs_amd.DeviceBuffer(4 * n * V)
-    log(f"engine ready: lanes={K} groups={(n + K - 1) // K} depth buffer {4 * n * V / 2**30:.1f} GiB")
-    for i in range(args.warmup):
-        eng.run_groups(src, out=out)
-        log(f"warmup {i + 1}/{args.warmup}: {eng.last_stats.total_ms:.2f} ms device")
-    kern = {"expand_td": 0.0, "expand_bu": 0.0, "inspect": 0.0, "emit": 0.0}
-    launches = {"expand_td": 0, "expand_bu": 0, "inspect": 0, "emit": 0}
-    edges = 0
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        eng.run_groups(src, out=out)
-        st = eng.last_stats
-        edges += st.traversed_edges
-        kern["expand_td"] += st.expand_td_ms
-        kern["expand_bu"] += st.expand_bu_ms
-        kern["inspect"] += st.inspect_ms
-        kern["emit"] += st.emit_ms
-        launches["expand_td"] += st.n_td_launches
-        launches["expand_bu"] += st.n_bu_launches
-        launches["inspect"] += st.n_inspect_launches
-        launches["emit"] += st.n_emit_launches
-    wall = time.perf_counter() - t0
-    gteps = edges / wall / 1e9
-    ms_per_step = wall * 1e3 / args.steps
-    log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, {gteps:.1f} GTEPS")
 
-    # algorithmic bytes per kernel from one untimed pass with the work model
-    model = ibfs_amd.Engine(g, lane_bits=K, work_model=1, **tune)
-    scratch = ibfs_amd.DeviceBuffer(4 * K * V)  # keeps the timed outputs intact
-    alg = {"expand_td": 0, "expand_bu": 0, "inspect": 0, "emit": 0}
-    levels = []
-    for g0 in range(0, n, K):
-        grp = src[g0:g0 + K]
-        model.traversal(grp, out=scratch)
-        lv = model.level_log()
-        levels.append([(x["level"], x["direction"]) for x in lv])
-        for kname, b in alg_bytes_per_kernel(lv, V, K // 8, len(grp)).items():
-            alg[kname] += b
-    per_step_ms = {k: v / args.steps for k, v in kern.items()}
-    dom = max(per_step_ms, key=per_step_ms.get)
-    achieved = alg[dom] / (per_step_ms[dom] * 1e-3) / 1e9 if per_step_ms[dom] > 0 else 0.0
-    kernels = {k: {"ms_per_step": round(per_step_ms[k], 4),
-                   "launches_per_step": launches[k] // args.steps,
-                   "alg_GB_per_step": round(alg[k] / 1e9, 4),
-                   "alg_GBps": round(alg[k] / (per_step_ms[k] * 1e-3) / 1e9, 1)
-                   if per_step_ms[k] > 0 else None}
-               for k in kern}
-    log(f"dominant kernel {dom}: {achieved:.0f} GB/s algorithmic")
+    plain = plain_leg(g, src, K, args.steps, args.warmup, tune, args.emit_mode, out)
+    log(f"plain: {plain['ms_per_step']:.2f} ms/step, {plain['gteps']:.1f} GTEPS")
+    roof, kernels, levels = roofline_leg(g, src, K, V, tune, plain["kern_ms"], plain["launches"],
+                                         args.steps)
+    log(f"dominant kernel {roof['kernel']}: {roof['achieved']:.0f} GB/s algorithmic")
+
+    batched = None
+    if not args.no_batch:
+        batched = batch_leg(g, src, args.batch_lanes, args.batch_contexts, args.burst, args.steps,
+                            args.warmup, tune, args.emit_mode, out)
+        batched["gteps"] = round(batched["gteps"], 3)
+        batched["vs_plain"] = round(batched["gteps"] / plain["gteps"], 3)
+        log(f"batched: {batched['ms_per_step']:.2f} ms/step, {batched['gteps']} GTEPS "
+            f"({batched['vs_plain']}x plain)")
 
     cpu = None
     verified = 0
@@ -190,15 +300,32 @@ def main() -> int:
             return 4
         verified = len(sample)
         log(f"cpu {cpu['value']} GTEPS; {verified} device depth arrays verified bit-exact")
+    del out
+
+    served = None
+    if not args.no_serve:
+        log("generating Chung-Lu graph (3.07M vertices, gamma 2.2, mean 76, cap 33000)")
+        t0 = time.perf_counter()
+        hc = ibfs_amd.HostCsr.chung_lu(3_070_000, 2.2, 76.0, 33000.0, seed=1, threads=threads)
+        gc = ibfs_amd.DeviceGraph(hc, device=0)
+        log(f"Chung-Lu V={hc.vert_count} E={hc.edge_count} in {time.perf_counter() - t0:.1f}s")
+        graphs = [
+            {"name": f"rmat_s{wl['scale']}", "g": g, "pool": h.gen_sources(4096, seed=3),
+             "status_bytes": 2 * 32 * V * 32},
+            {"name": "chung_lu_3.07M", "g": gc, "pool": hc.gen_sources(4096, seed=3),
+             "status_bytes": 2 * 32 * hc.vert_count * 32},
+        ]
+        rates = [float(x) for x in args.serve_rates.split(",") if x]
+        served = served_leg(graphs, rates, args.serve_queries, seed=3, K=256)
 
     res = {
         "metric": METRIC,
-        "value": round(gteps, 3),
+        "value": round(plain["gteps"], 3),
         "unit": "GTEPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": round(plain["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -208,14 +335,15 @@ def main() -> int:
                    "scale": wl["scale"], "edge_factor": 16, "V": V, "E_directed": E,
                    "sources": n, "lane_bits": K, "groups": (n + K - 1) // K,
                    "graph_seed": 1, "source_seed": 2, "permuted_ids": True,
-                   "policy": f"alpha_beta({eng.cfg.alpha},{eng.cfg.beta})", "depth_out": "device int32 [sources][V]"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                   "policy": f"alpha_beta({plain['cfg'].alpha},{plain['cfg'].beta})",
+                   "depth_out": "device int32 [sources][V]"},
+        "roofline": roof,
         "cpu_baseline": cpu,
-        "graph500_gteps": round(gteps / 2, 3),
+        "graph500_gteps": round(plain["gteps"] / 2, 3),
         "groupby_ms": round(groupby_ms, 3),
         "kernels": kernels,
+        "batched": batched,
+        "served": served,
         "levels_per_group": levels,
         "verified_sources": verified,
     }

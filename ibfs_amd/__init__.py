@@ -291,6 +291,29 @@ class Engine:
             pass
 
 
+def autotune(g: DeviceGraph, sample_sources, passes: int = 2, reps: int = 2,
+             start: Optional[Config] = None, save_to: Optional[str] = None) -> tuple[Config, dict]:
+    """Tunes the engine configuration for graph g on the sample sources;
+    returns (config, report).  save_to writes the report as JSON."""
+    import json
+    s = np.ascontiguousarray(sample_sources, dtype=np.int32)
+    best = Config()
+    buf = C.create_string_buffer(1 << 20)
+    st = start if start is not None else default_config(device=g.device)
+    check(lib().ibfs_autotune(g._h, _i32p(s), len(s), C.byref(st), passes, reps, C.byref(best),
+                              buf, len(buf)))
+    report = json.loads(buf.value.decode())
+    if save_to:
+        with open(save_to, "w") as f:
+            json.dump(report, f, indent=1)
+    return best, report
+
+
+def config_from_dict(d: dict, device: int = 0) -> Config:
+    """Config from the "best" record of an autotune report."""
+    return default_config(device=device, **{k: int(v) for k, v in d.items()})
+
+
 class Batch:
     """Batched launcher: any number of sources packed into lane groups,
     `contexts` traversals in flight, device-driven levels in captured
@@ -341,6 +364,56 @@ class Batch:
     def close(self) -> None:
         if self._h:
             lib().ibfs_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching BFS server over one DeviceGraph (ibfs_server_*).
+    submit() is thread-safe and returns a query id; the query's depth arrays
+    are written to the given device buffer ([n][V] int32)."""
+
+    def __init__(self, g: DeviceGraph, status_bytes: int = 0, cfg: Optional[Config] = None,
+                 **overrides):
+        if cfg is None:
+            overrides.setdefault("lane_bits", 256)
+            cfg = default_config(device=g.device, **overrides)
+        h = C.c_void_p()
+        check(lib().ibfs_server_create(g._h, C.byref(cfg), status_bytes, C.byref(h)))
+        self._h = h
+        self.graph = g
+        self.cfg = cfg
+        c, l = C.c_int32(), C.c_int32()
+        check(lib().ibfs_server_contexts(h, C.byref(c), C.byref(l)))
+        self.contexts, self.lanes = c.value, l.value
+
+    def submit(self, sources, out: DeviceBuffer) -> int:
+        s = np.ascontiguousarray(sources, dtype=np.int32)
+        if out.nbytes < 4 * len(s) * self.graph.vert_count:
+            raise ValueError("device buffer too small")
+        qid = C.c_int64()
+        check(lib().ibfs_server_submit(self._h, _i32p(s), len(s), C.c_void_p(out.ptr),
+                                       C.byref(qid)))
+        return qid.value
+
+    def wait(self, qid: int, timeout_ms: float = -1.0) -> dict:
+        info = _lib.QueryInfo()
+        check(lib().ibfs_server_wait(self._h, qid, timeout_ms, C.byref(info)))
+        return {"id": info.id, "n_sources": info.n_sources, "done": bool(info.done),
+                "submit_ms": info.submit_ms, "admit_ms": info.admit_ms,
+                "done_ms": info.done_ms, "latency_ms": info.done_ms - info.submit_ms}
+
+    def release(self, qid: int) -> None:
+        check(lib().ibfs_server_release(self._h, qid))
+
+    def close(self) -> None:
+        if self._h:
+            lib().ibfs_server_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

@@ -65,6 +65,11 @@ class GraphOpts(C.Structure):
     _fields_ = [("hub_count", C.c_int32), ("nthreads", C.c_int32), ("reserved", C.c_int32 * 6)]
 
 
+class QueryInfo(C.Structure):
+    _fields_ = [("id", C.c_int64), ("n_sources", C.c_int32), ("done", C.c_int32),
+                ("submit_ms", C.c_double), ("admit_ms", C.c_double), ("done_ms", C.c_double)]
+
+
 class Stats(C.Structure):
     _fields_ = [
         ("levels", C.c_int32), ("emit_windows", C.c_int32), ("traversed_edges", C.c_int64),
@@ -121,6 +126,14 @@ SIGNATURES = {
     "ibfs_batch_run": (C.c_int, [P, I32P, C.c_int64, P, C.c_int32, C.POINTER(Stats)]),
     "ibfs_batch_destroy": (C.c_int, [P]),
     "ibfs_batch_closeness": (C.c_int, [P, I32P, C.c_int64, I64P, I64P, C.POINTER(Stats)]),
+    "ibfs_server_create": (C.c_int, [P, C.POINTER(Config), C.c_int64, C.POINTER(P)]),
+    "ibfs_server_contexts": (C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "ibfs_server_submit": (C.c_int, [P, I32P, C.c_int32, P, C.POINTER(C.c_int64)]),
+    "ibfs_server_wait": (C.c_int, [P, C.c_int64, C.c_double, C.POINTER(QueryInfo)]),
+    "ibfs_server_release": (C.c_int, [P, C.c_int64]),
+    "ibfs_server_destroy": (C.c_int, [P]),
+    "ibfs_autotune": (C.c_int, [P, I32P, C.c_int64, C.POINTER(Config), C.c_int32, C.c_int32,
+                                C.POINTER(Config), C.c_char_p, C.c_size_t]),
 }
 
 _lib = None

@@ -15,8 +15,9 @@ CSRC = PKG / "csrc"
 OUT = PKG / "libibfs.so"
 CLI = PKG / "gpu-ibfs"
 
-HIP_SOURCES = ["kernels_traverse.hip", "kernels_emit.hip", "engine.hip", "batch.hip"]
-CXX_SOURCES = ["hostcsr.cpp"]
+HIP_SOURCES = ["kernels_traverse.hip", "kernels_emit.hip", "engine.hip", "batch.hip",
+               "serve.hip"]
+CXX_SOURCES = ["hostcsr.cpp", "autotune.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 forms are disabled for all device code on this target.

@@ -82,6 +82,9 @@ hipError_t launch_fill_u64(u64 *p, int64_t n, u64 value, hipStream_t st);
 hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t value, hipStream_t st);
 hipError_t launch_init_sources(int W, const int32_t *src, int k, int lane0, u64 *S0,
                                hipStream_t st);
+// S[src[i]] |= bit lane[i], i < n.
+hipError_t launch_admit(int W, const int32_t *src, const int32_t *lane, int n, u64 *S,
+                        hipStream_t st);
 hipError_t launch_inspect(int W, const InspectArgs &a, hipStream_t st);
 hipError_t launch_expand_td(int W, const int64_t *beg, const int32_t *adj, const u64 *prev,
                             u64 *cur, const u64 *lam_prev, Queues q, int64_t ns, int64_t nm,

@@ -31,6 +31,15 @@ __global__ void k_init_sources(const int32_t *src, int k, int lane0, u64 *S0) {
   atomicOr(&S0[(int64_t)src[i] * W + (lane >> 6)], 1ull << (lane & 63));
 }
 
+// Admission: bit lane[i] set in S[src[i]] (lanes of any position).
+template <int W>
+__global__ void k_admit(const int32_t *src, const int32_t *lane, int n, u64 *S) {
+  int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= n) return;
+  const int k = lane[i];
+  atomicOr(&S[(int64_t)src[i] * W + (k >> 6)], 1ull << (k & 63));
+}
+
 // Direction of level L+1 from the counters of level L (same rule as the
 // host-driven driver): fixed switch level, or alpha/beta.
 __device__ void plan_decide(Plan *p, const LevelCtl *c, int L) {
@@ -572,6 +581,14 @@ hipError_t launch_init_sources(int W, const int32_t *src, int k, int lane0, u64 
   if (k <= 0) return hipSuccess;
   dim3 g(grid_for(k, BS, 1 << 20)), blk(BS);
   IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_init_sources<WW>, g, blk, 0, st, src, k, lane0, S0));
+  return hipGetLastError();
+}
+
+hipError_t launch_admit(int W, const int32_t *src, const int32_t *lane, int n, u64 *S,
+                        hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  dim3 g(grid_for(n, BS, 1 << 20)), blk(BS);
+  IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_admit<WW>, g, blk, 0, st, src, lane, n, S));
   return hipGetLastError();
 }
 

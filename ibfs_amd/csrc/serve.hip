@@ -1,0 +1,567 @@
+// Serving loop: continuous batching of BFS queries (1-256 sources each)
+// against a device-resident graph.
+//
+// Every source of a query occupies one status lane of one traversal context
+// from its admission level a until its BFS ends; its depths are L - a.  A
+// worker thread owns the GPU: at each level boundary of a context it
+// (1) emits and frees the lanes whose BFS ended (no vertex gained the lane at
+// this level), (2) issues the next expansion, (3) admits waiting sources into
+// clean lanes, (4) issues the inspection that also clears the freed lanes in
+// the next snapshot.  Lanes are independent bits, so a query's depths do not
+// depend on what shares the context with it.  Contexts are multiplexed with
+// events, so one context's level boundary overlaps another's kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+
+using namespace ibfs;
+
+#define HIP_CHK(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess)                                                              \
+      return fail(IBFS_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                  __FILE__, __LINE__);                                                 \
+  } while (0)
+
+namespace {
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+struct Query {
+  int64_t id = 0;
+  int n = 0;
+  const int32_t *src = nullptr;  // owned copy in `srcbuf`
+  std::vector<int32_t> srcbuf;
+  int32_t *out = nullptr;        // device [n][V]
+  int remaining = 0;             // sources not finished
+  double t_submit = 0, t_admit = -1, t_done = -1;
+  bool done = false;
+};
+
+struct PendingSrc {
+  int64_t qid;
+  int idx;
+};
+
+enum LaneState { CLEAN = 0, ACTIVE = 1, DIRTY = 2 };
+
+struct Lane {
+  int state = CLEAN;
+  int64_t qid = -1;
+  int admit = 0;
+  int freed_at = 0;
+};
+
+struct SCtx {
+  hipStream_t st = nullptr;
+  u64 *ring = nullptr;
+  LevelCtl *ctl = nullptr;
+  LevelCtl *h_ctl = nullptr;
+  Queues q{};
+  u64 *hs = nullptr;
+  u64 *d_mask = nullptr, *h_mask = nullptr;
+  int32_t *d_adm = nullptr, *h_adm = nullptr;  // [0,K) sources, [K,2K) lanes
+  LaneOut *d_lanes = nullptr, *h_lanes = nullptr;
+  hipEvent_t ev = nullptr;
+  std::vector<Lane> lanes;
+  bool running = false;
+  int L = 0, base = 0, dir_prev = -1;
+  u64 nf_prev = ~0ull;
+  std::vector<int64_t> done_pending;
+};
+
+template <class T>
+void dfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+template <class T>
+void hfree(T *&p) {
+  if (p) (void)hipHostFree((void *)p);
+  p = nullptr;
+}
+
+}  // namespace
+
+struct ibfs_server_s {
+  ibfs_graph_s *g = nullptr;
+  ibfs_config cfg{};
+  int K = 256, W = 4, R = 32, C = 1, chunk_shift = 12;
+  std::vector<SCtx> ctx;
+  std::thread worker;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  bool stop = false;
+  int error = IBFS_OK;
+  std::string error_msg;
+  int64_t next_id = 1;
+  std::unordered_map<int64_t, Query> queries;
+  std::deque<PendingSrc> incoming;  // submitted, not yet seen by the worker
+  std::deque<PendingSrc> pending;   // worker-owned FIFO of sources to admit
+  int64_t levels_issued = 0, lanes_admitted = 0;
+};
+
+namespace {
+
+// mask staging (words): E1 [sel W | levels (R+1)W], E2 same, I [filter W | keep W | nand W]
+size_t emit_words(const ibfs_server_s *s) { return (size_t)(s->R + 2) * s->W; }
+size_t mask_words(const ibfs_server_s *s) { return 2 * emit_words(s) + 3 * (size_t)s->W; }
+
+u64 *slot(const ibfs_server_s *s, const SCtx &c, int L) {
+  return c.ring + (size_t)(L % s->R) * (size_t)s->g->V * (size_t)s->W;
+}
+
+void release_ctx(SCtx &c) {
+  if (c.st) (void)hipStreamSynchronize(c.st);
+  dfree(c.ring);
+  dfree(c.ctl);
+  hfree(c.h_ctl);
+  for (int i = 0; i < 6; ++i) dfree(c.q.q[i]);
+  dfree(c.q.chunks);
+  dfree(c.hs);
+  dfree(c.d_mask);
+  hfree(c.h_mask);
+  dfree(c.d_adm);
+  hfree(c.h_adm);
+  dfree(c.d_lanes);
+  hfree(c.h_lanes);
+  if (c.ev) (void)hipEventDestroy(c.ev);
+  c.ev = nullptr;
+  if (c.st) (void)hipStreamDestroy(c.st);
+  c.st = nullptr;
+}
+
+int alloc_ctx(ibfs_server_s *s, SCtx &c) {
+  const int64_t V = s->g->V;
+  hipError_t err = hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking);
+  auto A = [&](void **p, size_t bytes) {
+    if (err == hipSuccess) err = hipMalloc(p, bytes ? bytes : 1);
+  };
+  auto H = [&](void **p, size_t bytes) {
+    if (err == hipSuccess) err = hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocDefault);
+  };
+  A((void **)&c.ring, (size_t)s->R * (size_t)V * (size_t)s->W * sizeof(u64));
+  A((void **)&c.ctl, sizeof(LevelCtl) * (size_t)s->R);
+  H((void **)&c.h_ctl, sizeof(LevelCtl));
+  for (int i = 0; i < 6; ++i)
+    if (i != Q_TD_L) A((void **)&c.q.q[i], sizeof(int32_t) * (size_t)V);
+  A((void **)&c.q.chunks, sizeof(int2) * ((size_t)V + (size_t)(s->g->E >> s->chunk_shift) + 2));
+  A((void **)&c.hs, sizeof(u64) * (size_t)s->W * (size_t)std::max<int64_t>(s->g->H, 1));
+  A((void **)&c.d_mask, sizeof(u64) * mask_words(s));
+  H((void **)&c.h_mask, sizeof(u64) * mask_words(s));
+  A((void **)&c.d_adm, sizeof(int32_t) * 2 * (size_t)s->K);
+  H((void **)&c.h_adm, sizeof(int32_t) * 2 * (size_t)s->K);
+  A((void **)&c.d_lanes, sizeof(LaneOut) * (size_t)s->K);
+  H((void **)&c.h_lanes, sizeof(LaneOut) * (size_t)s->K);
+  if (err == hipSuccess) err = hipEventCreateWithFlags(&c.ev, hipEventDisableTiming);
+  if (err != hipSuccess)
+    return fail(err == hipErrorOutOfMemory ? IBFS_ERR_NOMEM : IBFS_ERR_HIP,
+                "server context allocation failed: %s", hipGetErrorString(err));
+  std::memset(c.h_lanes, 0, sizeof(LaneOut) * (size_t)s->K);
+  c.lanes.assign(s->K, Lane{});
+  return IBFS_OK;
+}
+
+void set_bit(u64 *m, int k) { m[k >> 6] |= 1ull << (k & 63); }
+bool has_bit(const u64 *m, int k) { return (m[k >> 6] >> (k & 63)) & 1ull; }
+
+// Emission of lanes `sel` over [b, e] using staging slot `which` (0/1).
+int emit_lanes(ibfs_server_s *s, SCtx &c, const std::vector<int> &lanes, int b, int e, int which) {
+  if (lanes.empty() || e < b) return IBFS_OK;
+  const int W = s->W;
+  u64 *h = c.h_mask + (size_t)which * emit_words(s);
+  u64 *d = c.d_mask + (size_t)which * emit_words(s);
+  std::memset(h, 0, sizeof(u64) * emit_words(s));
+  for (int k : lanes) set_bit(h, k);
+  for (int L = b; L <= e; ++L)
+    for (int k : lanes)
+      if (c.lanes[k].admit <= L) set_bit(h + W + (size_t)(L - b) * W, k);
+  HIP_CHK(hipMemcpyAsync(d, h, sizeof(u64) * emit_words(s), hipMemcpyHostToDevice, c.st));
+  HIP_CHK(launch_emit(W, s->cfg.emit_mode, 0, c.ring, s->R, s->g->V, b, e, c.d_lanes, d + W, d,
+                      s->K, c.st));
+  return IBFS_OK;
+}
+
+// Admits up to the free clean lanes' worth of pending sources at level L
+// (their bits go into slot L); returns the admitted lane mask in A.
+int admit(ibfs_server_s *s, SCtx &c, int L, u64 *A, int *n_adm) {
+  int n = 0;
+  for (int k = 0; k < s->K && !s->pending.empty(); ++k) {
+    if (c.lanes[k].state != CLEAN) continue;
+    const PendingSrc p = s->pending.front();
+    s->pending.pop_front();
+    Query &q = s->queries[p.qid];
+    c.lanes[k].state = ACTIVE;
+    c.lanes[k].qid = p.qid;
+    c.lanes[k].admit = L;
+    c.h_adm[n] = q.src[p.idx];
+    c.h_adm[s->K + n] = k;
+    c.h_lanes[k].out = q.out + (size_t)p.idx * (size_t)s->g->V;
+    c.h_lanes[k].admit = L;
+    if (q.t_admit < 0) q.t_admit = now_ms();
+    set_bit(A, k);
+    ++n;
+  }
+  *n_adm = n;
+  if (n == 0) return IBFS_OK;
+  s->lanes_admitted += n;
+  HIP_CHK(hipMemcpyAsync(c.d_adm, c.h_adm, sizeof(int32_t) * 2 * (size_t)s->K,
+                         hipMemcpyHostToDevice, c.st));
+  HIP_CHK(hipMemcpyAsync(c.d_lanes, c.h_lanes, sizeof(LaneOut) * (size_t)s->K,
+                         hipMemcpyHostToDevice, c.st));
+  HIP_CHK(launch_admit(s->W, c.d_adm, c.d_adm + s->K, n, slot(s, c, L), c.st));
+  return IBFS_OK;
+}
+
+// Inspection of level L (masks in staging slot I) + control-block copy + event.
+int inspect_level(ibfs_server_s *s, SCtx &c, int L, const u64 *filter, const u64 *keep,
+                  const u64 *nand) {
+  const int W = s->W;
+  u64 *h = c.h_mask + 2 * emit_words(s);
+  u64 *d = c.d_mask + 2 * emit_words(s);
+  for (int w = 0; w < W; ++w) {
+    h[w] = filter[w];
+    h[W + w] = keep[w];
+    h[2 * W + w] = nand[w];
+  }
+  HIP_CHK(hipMemcpyAsync(d, h, sizeof(u64) * 3 * W, hipMemcpyHostToDevice, c.st));
+  InspectArgs a{};
+  a.prev = L > 0 ? slot(s, c, L - 1) : nullptr;
+  a.cur = slot(s, c, L);
+  a.next = slot(s, c, L + 1);
+  a.lam_prev = d;
+  a.prev_keep = d + W;
+  a.next_and = d + 2 * W;
+  a.ctl = c.ctl + L % s->R;
+  a.ctl_zero = c.ctl + (L + 1) % s->R;
+  a.deg = s->g->d_deg;
+  a.V = s->g->V;
+  a.q = c.q;
+  a.sml = s->cfg.sml_shed;
+  a.lrg = s->cfg.lrg_shed;
+  a.chunk_shift = s->chunk_shift;
+  a.hubs = s->g->d_hubs;
+  a.H = s->g->H;
+  a.hs = c.hs;
+  HIP_CHK(launch_inspect(W, a, c.st));
+  HIP_CHK(hipMemcpyAsync(c.h_ctl, c.ctl + L % s->R, sizeof(LevelCtl), hipMemcpyDeviceToHost, c.st));
+  HIP_CHK(hipEventRecord(c.ev, c.st));
+  c.L = L;
+  ++s->levels_issued;
+  return IBFS_OK;
+}
+
+int start_ctx(ibfs_server_s *s, SCtx &c) {
+  const int W = s->W;
+  for (auto &l : c.lanes) l = Lane{};
+  c.base = 0;
+  c.dir_prev = -1;
+  c.nf_prev = ~0ull;
+  HIP_CHK(launch_fill_u64(slot(s, c, 0), s->g->V * W, 0ull, c.st));
+  HIP_CHK(launch_fill_u64(reinterpret_cast<u64 *>(c.ctl), (int64_t)s->R * 16, 0ull, c.st));
+  std::vector<u64> A(W, 0), keep(W, ~0ull), nand(W, ~0ull);
+  int n = 0;
+  int rc = admit(s, c, 0, A.data(), &n);
+  if (rc) return rc;
+  c.running = true;
+  return inspect_level(s, c, 0, A.data(), keep.data(), nand.data());
+}
+
+int decide(const ibfs_server_s *s, int Ln, const LevelCtl &lc, int dir_prev, u64 nf_prev) {
+  switch (s->cfg.policy) {
+    case IBFS_POLICY_TD_ONLY: return 0;
+    case IBFS_POLICY_BU_ONLY: return 1;
+    case IBFS_POLICY_FIXED: return Ln <= s->cfg.sw_level ? 0 : 1;
+    default: break;
+  }
+  if (dir_prev != 1) return (lc.m_front * (u64)s->cfg.alpha > lc.m_cand) ? 1 : 0;
+  return (lc.n_front * (u64)s->cfg.beta < (u64)s->g->V && lc.n_front < nf_prev) ? 0 : 1;
+}
+
+void mark_done(ibfs_server_s *s, SCtx &c) {
+  if (c.done_pending.empty()) return;
+  const double t = now_ms();
+  for (int64_t id : c.done_pending) {
+    auto it = s->queries.find(id);
+    if (it != s->queries.end()) {
+      it->second.done = true;
+      it->second.t_done = t;
+    }
+  }
+  c.done_pending.clear();
+  s->cv_done.notify_all();
+}
+
+// Level L of context c has completed (its control block is in h_ctl).
+int process_level(ibfs_server_s *s, SCtx &c) {
+  const int W = s->W;
+  const int L = c.L;
+  const LevelCtl lc = *c.h_ctl;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    mark_done(s, c);  // emissions issued before this level's event are complete
+  }
+  // 1. lanes whose BFS ended: emit [base, L-1], free them
+  std::vector<int> fin;
+  std::vector<u64> finmask(W, 0);
+  for (int k = 0; k < s->K; ++k) {
+    Lane &l = c.lanes[k];
+    if (l.state == ACTIVE && l.admit < L && !has_bit(lc.lam, k)) {
+      fin.push_back(k);
+      set_bit(finmask.data(), k);
+    }
+  }
+  int rc = emit_lanes(s, c, fin, c.base, L - 1, 0);
+  if (rc) return rc;
+  bool any_active = false;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (int k : fin) {
+      Lane &l = c.lanes[k];
+      Query &q = s->queries[l.qid];
+      if (--q.remaining == 0) c.done_pending.push_back(l.qid);
+      l.state = DIRTY;
+      l.freed_at = L;
+      l.qid = -1;
+    }
+    for (auto &l : c.lanes) {
+      if (l.state == DIRTY && l.freed_at <= L - 1) l.state = CLEAN;  // cleared in S_{L+1}
+      any_active |= (l.state == ACTIVE);
+    }
+    if (!any_active && s->pending.empty()) {
+      c.running = false;
+      HIP_CHK(hipStreamSynchronize(c.st));
+      mark_done(s, c);
+      return IBFS_OK;
+    }
+  }
+  // 2. expansion of L+1
+  const int Ln = L + 1;
+  const int dir = decide(s, Ln, lc, c.dir_prev, c.nf_prev);
+  const u64 *lamp = reinterpret_cast<const u64 *>(c.ctl + L % s->R);
+  if (dir == 0)
+    HIP_CHK(launch_expand_td(W, s->g->d_beg, s->g->d_adj, slot(s, c, L), slot(s, c, Ln), lamp,
+                             c.q, lc.q[Q_TD_S], lc.q[Q_TD_M], lc.q[Q_TD_L], s->chunk_shift, c.st));
+  else
+    HIP_CHK(launch_expand_bu(W, s->g->d_beg, s->g->d_tadj, slot(s, c, L), c.hs, slot(s, c, Ln),
+                             lamp, c.q, lc.q[Q_BU_S], lc.q[Q_BU_M], lc.q[Q_BU_L], c.st));
+  c.dir_prev = dir;
+  c.nf_prev = lc.n_front;
+  // 3. the next copy (slot Ln+1) would overwrite the window base: flush all
+  //    active lanes over [base, Ln]
+  if ((Ln + 1) - c.base >= s->R) {
+    std::vector<int> act;
+    for (int k = 0; k < s->K; ++k)
+      if (c.lanes[k].state == ACTIVE) act.push_back(k);
+    rc = emit_lanes(s, c, act, c.base, Ln, 1);
+    if (rc) return rc;
+    c.base = Ln;
+  }
+  // 4. admission at Ln
+  std::vector<u64> A(W, 0), filter(W), keep(W), nand(W);
+  int n_adm = 0;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    rc = admit(s, c, Ln, A.data(), &n_adm);
+  }
+  if (rc) return rc;
+  for (int w = 0; w < W; ++w) {
+    filter[w] = lc.lam[w] | A[w];
+    keep[w] = ~A[w];
+    nand[w] = ~finmask[w];
+  }
+  // 5. inspection of Ln (clears the lanes freed at L in slot Ln+1)
+  return inspect_level(s, c, Ln, filter.data(), keep.data(), nand.data());
+}
+
+void worker_main(ibfs_server_s *s) {
+  (void)hipSetDevice(s->g->device);
+  while (true) {
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      bool busy = false;
+      for (auto &c : s->ctx) busy |= c.running;
+      if (!busy)
+        s->cv_work.wait(lk, [&] { return s->stop || !s->incoming.empty(); });
+      if (s->stop) break;
+      while (!s->incoming.empty()) {
+        s->pending.push_back(s->incoming.front());
+        s->incoming.pop_front();
+      }
+    }
+    bool progressed = false;
+    for (auto &c : s->ctx) {
+      int rc = IBFS_OK;
+      if (!c.running) {
+        bool have;
+        {
+          std::lock_guard<std::mutex> lk(s->mu);
+          have = !s->pending.empty();
+          if (have) rc = start_ctx(s, c);
+        }
+        progressed |= have;
+      } else {
+        hipError_t q = hipEventQuery(c.ev);
+        if (q == hipErrorNotReady) continue;
+        if (q != hipSuccess) rc = fail(IBFS_ERR_HIP, "server event: %s", hipGetErrorString(q));
+        else rc = process_level(s, c);
+        progressed = true;
+      }
+      if (rc) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->error = rc;
+        s->error_msg = last_error();
+        s->stop = true;
+        s->cv_done.notify_all();
+        return;
+      }
+    }
+    if (!progressed) std::this_thread::yield();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ibfs_server_create(ibfs_graph_t g, const ibfs_config *cfg_in, int64_t status_bytes,
+                       ibfs_server_t *out) {
+  if (!g || !out) return fail(IBFS_ERR_ARG, "ibfs_server_create: null argument");
+  ibfs_config cfg;
+  if (cfg_in) cfg = *cfg_in;
+  else ibfs_config_default(&cfg);
+  if (cfg.lane_bits != 64 && cfg.lane_bits != 128 && cfg.lane_bits != 256)
+    return fail(IBFS_ERR_ARG, "lane_bits must be 64, 128 or 256");
+  if (cfg.ring_levels < 8 || cfg.ring_levels > 255)
+    return fail(IBFS_ERR_ARG, "serving needs ring_levels in [8, 255]");
+  if (cfg.td_chunk < 256 || (cfg.td_chunk & (cfg.td_chunk - 1)))
+    return fail(IBFS_ERR_ARG, "td_chunk must be a power of two >= 256");
+  if (cfg.emit_mode < 0 || cfg.emit_mode > 2) return fail(IBFS_ERR_ARG, "bad emit_mode");
+  if (cfg.device != g->device) return fail(IBFS_ERR_ARG, "config device differs from the graph's");
+  int nd = 0;
+  if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0)
+    return fail(IBFS_ERR_NODEVICE, "no HIP device available");
+  HIP_CHK(hipSetDevice(g->device));
+  // admission capacity from the status-array budget: one context holds
+  // lane_bits lanes in ring_levels snapshots of V * lane_bits / 8 bytes
+  const size_t per_ctx = (size_t)cfg.ring_levels * (size_t)g->V * (size_t)(cfg.lane_bits / 8);
+  int C = status_bytes > 0 ? (int)((size_t)status_bytes / per_ctx) : 1;
+  if (C < 1)
+    return fail(IBFS_ERR_NOMEM, "status budget %lld below one context (%zu bytes)",
+                (long long)status_bytes, per_ctx);
+  C = std::min(C, 16);
+  auto *s = new ibfs_server_s;
+  s->g = g;
+  s->cfg = cfg;
+  s->K = cfg.lane_bits;
+  s->W = cfg.lane_bits / 64;
+  s->R = cfg.ring_levels;
+  s->C = C;
+  s->chunk_shift = __builtin_ctz((unsigned)cfg.td_chunk);
+  s->ctx.resize(C);
+  for (auto &c : s->ctx) {
+    int rc = alloc_ctx(s, c);
+    if (rc) {
+      for (auto &x : s->ctx) release_ctx(x);
+      delete s;
+      return rc;
+    }
+  }
+  s->worker = std::thread(worker_main, s);
+  *out = s;
+  return IBFS_OK;
+}
+
+int ibfs_server_contexts(ibfs_server_t s, int32_t *contexts, int32_t *lanes) {
+  if (!s) return fail(IBFS_ERR_ARG, "null server");
+  if (contexts) *contexts = s->C;
+  if (lanes) *lanes = s->C * s->K;
+  return IBFS_OK;
+}
+
+int ibfs_server_submit(ibfs_server_t s, const int32_t *src, int32_t n, int32_t *depth_out,
+                       int64_t *id) {
+  if (!s || !src || !depth_out || !id || n < 1) return fail(IBFS_ERR_ARG, "ibfs_server_submit: bad arguments");
+  for (int i = 0; i < n; ++i)
+    if (src[i] < 0 || src[i] >= s->g->V)
+      return fail(IBFS_ERR_ARG, "source %d = %d out of range", i, src[i]);
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->error) return fail(s->error, "server stopped: %s", s->error_msg.c_str());
+  Query q;
+  q.id = s->next_id++;
+  q.n = n;
+  q.srcbuf.assign(src, src + n);
+  q.out = depth_out;
+  q.remaining = n;
+  q.t_submit = now_ms();
+  auto &ref = s->queries[q.id] = std::move(q);
+  ref.src = ref.srcbuf.data();
+  for (int i = 0; i < n; ++i) s->incoming.push_back(PendingSrc{ref.id, i});
+  *id = ref.id;
+  s->cv_work.notify_one();
+  return IBFS_OK;
+}
+
+int ibfs_server_wait(ibfs_server_t s, int64_t id, double timeout_ms, ibfs_query_info *info) {
+  if (!s) return fail(IBFS_ERR_ARG, "null server");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->queries.find(id);
+  if (it == s->queries.end()) return fail(IBFS_ERR_ARG, "unknown query %lld", (long long)id);
+  auto ready = [&] { return s->error || s->queries[id].done; };
+  if (timeout_ms < 0) s->cv_done.wait(lk, ready);
+  else if (!s->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), ready))
+    return fail(IBFS_ERR_STATE, "query %lld not finished within %.1f ms", (long long)id, timeout_ms);
+  if (s->error) return fail(s->error, "server stopped: %s", s->error_msg.c_str());
+  const Query &q = s->queries[id];
+  if (info) {
+    info->id = q.id;
+    info->n_sources = q.n;
+    info->done = q.done;
+    info->submit_ms = q.t_submit;
+    info->admit_ms = q.t_admit;
+    info->done_ms = q.t_done;
+  }
+  return IBFS_OK;
+}
+
+int ibfs_server_release(ibfs_server_t s, int64_t id) {
+  if (!s) return fail(IBFS_ERR_ARG, "null server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->queries.find(id);
+  if (it == s->queries.end()) return fail(IBFS_ERR_ARG, "unknown query %lld", (long long)id);
+  if (!it->second.done) return fail(IBFS_ERR_STATE, "query %lld still running", (long long)id);
+  s->queries.erase(it);
+  return IBFS_OK;
+}
+
+int ibfs_server_destroy(ibfs_server_t s) {
+  if (!s) return IBFS_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+    s->cv_work.notify_all();
+  }
+  if (s->worker.joinable()) s->worker.join();
+  (void)hipSetDevice(s->g->device);
+  for (auto &c : s->ctx) release_ctx(c);
+  delete s;
+  return IBFS_OK;
+}
+
+}  // extern "C"

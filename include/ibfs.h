@@ -228,6 +228,46 @@ int ibfs_batch_destroy(ibfs_batch_t b);
 int ibfs_batch_closeness(ibfs_batch_t b, const int32_t *src, int64_t n,
                          int64_t *depth_sum, int64_t *reach, ibfs_stats *stats);
 
+/* ---- serving: continuous batching ---------------------------------------- */
+/* A worker thread serves BFS queries of 1..any sources against one device
+ * graph.  Each source takes one status lane of one of the server's traversal
+ * contexts from its admission level until its BFS ends; lanes free up as
+ * sources finish and are reused two levels later, so queries arriving while
+ * others run are admitted as lanes free and share levels with them.  The
+ * number of contexts follows from status_bytes (ring_levels * V *
+ * lane_bits/8 bytes each, at least one).  depth_out of a query is a device
+ * buffer of n*V int32 (row i = source i, -1 unreached), valid once the query
+ * is done.  No reference counterpart (the reference is a batch CLI,
+ * main.cu:76-90). */
+typedef struct ibfs_query_info {
+  int64_t id;
+  int32_t n_sources;
+  int32_t done;
+  double submit_ms, admit_ms, done_ms; /* steady-clock milliseconds */
+} ibfs_query_info;
+int ibfs_server_create(ibfs_graph_t g, const ibfs_config *cfg, int64_t status_bytes,
+                       ibfs_server_t *out);
+int ibfs_server_contexts(ibfs_server_t s, int32_t *contexts, int32_t *lanes);
+int ibfs_server_submit(ibfs_server_t s, const int32_t *src, int32_t n,
+                       int32_t *depth_out, int64_t *id);
+/* Blocks until query id is done (timeout_ms < 0: no limit). */
+int ibfs_server_wait(ibfs_server_t s, int64_t id, double timeout_ms,
+                     ibfs_query_info *info);
+/* Forgets a finished query. */
+int ibfs_server_release(ibfs_server_t s, int64_t id);
+int ibfs_server_destroy(ibfs_server_t s);
+
+/* ---- autotuner ------------------------------------------------------------ */
+/* Coordinate descent over lane width, alpha/beta, degree-class thresholds,
+ * top-down chunk size and emission mode, scored by aggregate GTEPS of the
+ * sample sources (best of `reps` timed runs per candidate, `passes` sweeps).
+ * *best receives the chosen configuration; report (optional) a JSON record
+ * of every trial.  No reference counterpart (thresholds are constants in
+ * the reference, main.cu:56-57). */
+int ibfs_autotune(ibfs_graph_t g, const int32_t *src, int64_t n,
+                  const ibfs_config *start, int32_t passes, int32_t reps,
+                  ibfs_config *best, char *report, size_t report_len);
+
 /* TEPS numerator of one depth array: sum of deg(v) over reached v
  * (validate.h:59-76 report()). depth is a host array. */
 int ibfs_traversed_edges(ibfs_hostcsr_t h, const int32_t *depth,

@@ -191,6 +191,73 @@ def test_closeness_golden_all_sources(ring):
         np.testing.assert_array_equal(dsum, want_sum, err_msg=name)
 
 
+def _serve(server, queries, V, gap_s=0.0):
+    import threading
+    import time
+    bufs = [ibfs_amd.DeviceBuffer(4 * len(q) * V) for q in queries]
+    ids = [None] * len(queries)
+
+    def producer():
+        for i, q in enumerate(queries):
+            ids[i] = server.submit(q, bufs[i])
+            if gap_s:
+                time.sleep(gap_s)
+
+    th = threading.Thread(target=producer)
+    th.start()
+    th.join()
+    infos = [server.wait(i, timeout_ms=60000) for i in ids]
+    outs = [b.to_host().reshape(len(q), V) for b, q in zip(bufs, queries)]
+    return outs, infos
+
+
+@pytest.mark.parametrize("lane_bits", [64, 256])
+def test_server_mixed_equals_alone_s16(s16, lane_bits):
+    say(f"[gpu] server s16 lanes={lane_bits}")
+    rng = np.random.default_rng(3)
+    V = s16["h"].vert_count
+    pool = s16["h"].gen_sources(2000, seed=11)
+    sizes = np.floor(np.exp(rng.uniform(0, np.log(256), size=24))).astype(int)
+    queries = [rng.choice(pool, size=int(n), replace=True).astype(np.int32) for n in sizes]
+    srv = ibfs_amd.Server(s16["g"], status_bytes=0, lane_bits=lane_bits)
+    outs, infos = _serve(srv, queries, V, gap_s=0.002)
+    for q, o, inf in zip(queries, outs, infos):
+        assert inf["done"] and inf["latency_ms"] >= 0
+        np.testing.assert_array_equal(o, oracle.bfs_many(s16["beg"], s16["adj"], q, 8))
+    # one query alone on a fresh server gives the same arrays as in the mix
+    alone = ibfs_amd.Server(s16["g"], lane_bits=lane_bits)
+    o2, _ = _serve(alone, [queries[5]], V)
+    np.testing.assert_array_equal(o2[0], outs[5])
+    srv.close()
+    alone.close()
+
+
+@pytest.mark.parametrize("ring", [8, 9, 32])
+def test_server_ring_flush_and_recycling(ring):
+    say(f"[gpu] server ring={ring}")
+    z = load_golden("bfs_classes1500.npz")
+    g = ibfs_amd.DeviceGraph(ibfs_amd.HostCsr.from_arrays(z["beg_pos"], z["adj"]))
+    V = len(z["beg_pos"]) - 1
+    rng = np.random.default_rng(ring)
+    queries = [rng.integers(0, V, size=int(n)).astype(np.int32) for n in (1, 7, 64, 3, 100, 2, 40)]
+    srv = ibfs_amd.Server(g, lane_bits=64, ring_levels=ring)  # 64 lanes < 217 sources
+    outs, infos = _serve(srv, queries, V, gap_s=0.001)
+    for q, o in zip(queries, outs):
+        np.testing.assert_array_equal(o, z["depth"][q])
+    srv.close()
+
+
+def test_autotune_returns_a_valid_faster_or_equal_config(s16, tmp_path):
+    say("[gpu] autotune")
+    cfg, rep = ibfs_amd.autotune(s16["g"], s16["src"][:128], passes=1, reps=1,
+                                 save_to=str(tmp_path / "tune.json"))
+    assert rep["best_gteps"] >= rep["trials"][0]["gteps"]
+    assert cfg.lane_bits in (64, 128, 256)
+    eng = ibfs_amd.Engine(s16["g"], cfg)
+    np.testing.assert_array_equal(eng.run_groups(s16["src"]), s16["want"])
+    assert (tmp_path / "tune.json").exists()
+
+
 def test_device_output_and_determinism(s16):
     say("[gpu] device output")
     eng = ibfs_amd.Engine(s16["g"])
