@@ -339,3 +339,16 @@ def test_engine_argument_errors(s16):
         eng.traversal(np.array([70000], dtype=np.int32))
     # the engine still works after rejected calls
     np.testing.assert_array_equal(eng.traversal(s16["src"][:3]), s16["want"][:3])
+
+
+def test_cli_runs_reference_arguments(tmp_path):
+    say("[gpu] cli")
+    import subprocess
+    from conftest import ROOT
+    h = ibfs_amd.HostCsr.rmat(12, 16, seed=4)
+    h.save(str(tmp_path / "g_beg_pos.bin"), str(tmp_path / "g_csr.bin"))
+    r = subprocess.run([str(ROOT / "ibfs_amd" / "gpu-ibfs"), str(tmp_path / "g_beg_pos.bin"),
+                        str(tmp_path / "g_csr.bin"), "128", "2", "256"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("Traversal-time:") == 2 and "Total time:" in r.stdout

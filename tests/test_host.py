@@ -187,3 +187,14 @@ def test_c_abi_argument_checks():
     assert l.ibfs_graph_info(None, None, None) == _lib.IBFS_ERR_ARG
     assert "null" in _lib.last_error()
     assert l.ibfs_graph_destroy(None) == 0 and l.ibfs_engine_destroy(None) == 0
+
+
+def test_cli_usage_and_errors(tmp_path):
+    cli = ROOT / "ibfs_amd" / "gpu-ibfs"
+    if not cli.exists():
+        pytest.skip("CLI not built")
+    r = subprocess.run([str(cli)], capture_output=True, text=True)
+    assert r.returncode != 0 and "Wrong input" in r.stdout
+    r = subprocess.run([str(cli), str(tmp_path / "nope"), str(tmp_path / "nope2"), "128", "2",
+                        "128"], capture_output=True, text=True)
+    assert r.returncode != 0 and "cannot open" in r.stdout
