@@ -46,27 +46,29 @@ def log(msg: str) -> None:
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
-def alg_bytes_per_kernel(log_levels, V, W, k):
+def alg_bytes_per_kernel(log_levels, V, Vt, W, k):
     """Algorithmic bytes of each kernel launch of one traversal, from the
-    level log (B=8 offset, A=4 adjacency, Q=4 queue entry, W status bytes)."""
-    B, A, Q = 8, 4, 4
+    level log (B=8 offset, A=4 adjacency, Q=4 queue entry, W status bytes,
+    F=1 changed flag; status words exist for the Vt vertices with an edge)."""
+    B, A, Q, F = 8, 4, 4, 1
     out = {"expand_td": 0, "expand_bu": 0, "inspect": 0, "emit": 0}
     for i, li in enumerate(log_levels):
         L = li["level"]
         q = sum(li["q_td"][:2]) * Q + li["q_td"][2] * 8 + sum(li["q_bu"]) * Q
-        reads = 2 if L > 0 else 1
-        out["inspect"] += V * (4 + (reads + 1) * W) + q
+        # flag + degree + S_L read + S_{L+1} written for every vertex; S_{L-1}
+        # read and the flag cleared for the changed ones
+        out["inspect"] += Vt * (F + 4 + 2 * W) + li["n_changed"] * ((W if L > 0 else 0) + F) + q
         if L == 0:
             continue
         prev = log_levels[i - 1]
         if li["direction"] == 0:
             out["expand_td"] += (prev["n_frontier"] * (2 * B + W) + prev["m_frontier"] * (A + W)
-                                 + W * li["n_changed"])
+                                 + (W + F) * li["n_changed"])
         else:
             out["expand_bu"] += (li["bu_work"] * (2 * B + W) + li["bu_prefix"] * (A + W)
-                                 + W * li["n_changed"])
+                                 + (W + F) * li["n_changed"])
     nsnap = len(log_levels) - 1  # snapshots 0..last
-    out["emit"] = nsnap * W * V + 4 * k * V
+    out["emit"] = nsnap * W * Vt + 4 * V + 4 * k * V  # snapshots, vertex map, depths
     return out
 
 
@@ -97,6 +99,7 @@ def plain_leg(g, src, K, steps, warmup, tune, emit_mode, out):
 
 
 def roofline_leg(g, src, K, V, tune, kern_ms, launches, steps):
+    Vt = g.traversal_vertices
     model = ibfs_amd.Engine(g, lane_bits=K, work_model=1, **tune)
     scratch = ibfs_amd.DeviceBuffer(4 * K * V)
     alg = {"expand_td": 0, "expand_bu": 0, "inspect": 0, "emit": 0}
@@ -106,7 +109,7 @@ def roofline_leg(g, src, K, V, tune, kern_ms, launches, steps):
         model.traversal(grp, out=scratch)
         lv = model.level_log()
         levels.append([(x["level"], x["direction"]) for x in lv])
-        for kname, b in alg_bytes_per_kernel(lv, V, K // 8, len(grp)).items():
+        for kname, b in alg_bytes_per_kernel(lv, V, Vt, K // 8, len(grp)).items():
             alg[kname] += b
     dom = max(kern_ms, key=kern_ms.get)
     achieved = alg[dom] / (kern_ms[dom] * 1e-3) / 1e9 if kern_ms[dom] > 0 else 0.0
@@ -223,6 +226,7 @@ def main() -> int:
     ap.add_argument("--hubs", type=int, default=None)
     ap.add_argument("--alpha", type=int, default=None)
     ap.add_argument("--beta", type=int, default=None)
+    ap.add_argument("--inspect-vpt", type=int, default=None)
     ap.add_argument("--batch-contexts", type=int, default=2)
     ap.add_argument("--batch-lanes", type=int, default=256)
     ap.add_argument("--burst", type=int, default=8)
@@ -230,12 +234,16 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-queries", type=int, default=120)
     ap.add_argument("--serve-rates", default="200,800")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device 4 times (HBM counter yardstick)")
     args = ap.parse_args()
     tune = {}
     if args.alpha is not None:
         tune["alpha"] = args.alpha
     if args.beta is not None:
         tune["beta"] = args.beta
+    if args.inspect_vpt is not None:
+        tune["inspect_vpt"] = args.inspect_vpt
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
               f"--gpus {args.gpus}); source groups would shard across GPUs with no exchange.",
@@ -244,6 +252,21 @@ def main() -> int:
     if ibfs_amd.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 3
+    copy_cal = None
+    if args.copy_calibration > 0:
+        # plain device copy of known size: the yardstick for HBM byte counters
+        nb = args.copy_calibration << 20
+        a, b = ibfs_amd.DeviceBuffer(nb), ibfs_amd.DeviceBuffer(nb)
+        b.copy_from(a)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            b.copy_from(a)
+        dt = (time.perf_counter() - t0) / 3
+        copy_cal = {"bytes": nb, "copies": 4, "ms": round(dt * 1e3, 3),
+                    "GBps_read_plus_write": round(2 * nb / dt / 1e9, 1)}
+        log(f"copy calibration: {nb >> 20} MiB in {dt * 1e3:.3f} ms")
+        a.free()
+        b.free()
     wl = WORKLOADS[args.workload]
     threads = min(16, os.cpu_count() or 1)
     log(f"generating R-MAT scale {wl['scale']} ({threads} threads)")
@@ -333,6 +356,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"rmat_s{wl['scale']}_ef16_{n}src_{K}lane_groups_groupby",
                    "scale": wl["scale"], "edge_factor": 16, "V": V, "E_directed": E,
+                   "V_with_edges": g.traversal_vertices,
                    "sources": n, "lane_bits": K, "groups": (n + K - 1) // K,
                    "graph_seed": 1, "source_seed": 2, "permuted_ids": True,
                    "policy": f"alpha_beta({plain['cfg'].alpha},{plain['cfg'].beta})",
@@ -347,6 +371,8 @@ def main() -> int:
         "levels_per_group": levels,
         "verified_sources": verified,
     }
+    if copy_cal:
+        res["copy_calibration"] = copy_cal
     print(json.dumps(res))
     return 0
 

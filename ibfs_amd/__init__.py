@@ -151,6 +151,13 @@ class DeviceBuffer:
         check(lib().ibfs_memcpy_d2h(out.ctypes.data, C.c_void_p(self.ptr), out.nbytes))
         return out
 
+    def copy_from(self, other: "DeviceBuffer", nbytes: Optional[int] = None) -> None:
+        """Device-to-device copy of nbytes (default: all of `other`)."""
+        n = other.nbytes if nbytes is None else nbytes
+        if n > self.nbytes or n > other.nbytes:
+            raise ValueError("copy larger than a buffer")
+        check(lib().ibfs_memcpy_d2d(C.c_void_p(self.ptr), C.c_void_p(other.ptr), n))
+
     def free(self) -> None:
         if self.ptr:
             lib().ibfs_device_free(C.c_void_p(self.ptr))
@@ -181,6 +188,10 @@ class DeviceGraph:
         self.device = device
         self.vert_count = csr.vert_count
         self.edge_count = csr.edge_count
+        vt, hubs = C.c_int64(), C.c_int64()
+        check(lib().ibfs_graph_layout(h, C.byref(vt), C.byref(hubs)))
+        self.traversal_vertices = vt.value  # vertices with an edge (status words)
+        self.hub_count = hubs.value
 
     @classmethod
     def load_files(cls, beg_file: str, csr_file: str, device: int = 0) -> "DeviceGraph":

@@ -47,7 +47,7 @@ class Config(C.Structure):
         ("beta", C.c_int32), ("wg_size", C.c_int32), ("td_chunk", C.c_int32),
         ("ring_levels", C.c_int32), ("kernel_timing", C.c_int32), ("device", C.c_int32),
         ("work_model", C.c_int32), ("emit_mode", C.c_int32), ("dd_grid", C.c_int32),
-        ("reserved", C.c_int32 * 1),
+        ("inspect_vpt", C.c_int32),
     ]
 
 
@@ -94,6 +94,7 @@ SIGNATURES = {
     "ibfs_device_free": (C.c_int, [P]),
     "ibfs_memcpy_d2h": (C.c_int, [P, P, C.c_size_t]),
     "ibfs_memcpy_h2d": (C.c_int, [P, P, C.c_size_t]),
+    "ibfs_memcpy_d2d": (C.c_int, [P, P, C.c_size_t]),
     "ibfs_hostcsr_load": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(P)]),
     "ibfs_hostcsr_save": (C.c_int, [P, C.c_char_p, C.c_char_p]),
     "ibfs_hostcsr_create": (C.c_int, [I64P, C.c_int64, I32P, C.c_int64, C.POINTER(P)]),
@@ -113,6 +114,7 @@ SIGNATURES = {
     "ibfs_graph_from_host": (C.c_int, [P, C.c_int, C.POINTER(P)]),
     "ibfs_graph_load_files": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.POINTER(P)]),
     "ibfs_graph_info": (C.c_int, [P, I64P, I64P]),
+    "ibfs_graph_layout": (C.c_int, [P, I64P, I64P]),
     "ibfs_graph_destroy": (C.c_int, [P]),
     "ibfs_groupby": (C.c_int, [P, I32P, C.c_int64, I32P]),
     "ibfs_engine_create": (C.c_int, [P, C.POINTER(Config), C.POINTER(P)]),

@@ -89,6 +89,7 @@ int ibfs_autotune(ibfs_graph_t g, const int32_t *src, int64_t n, const ibfs_conf
       {"lrg_shed", &ibfs_config::lrg_shed, {128, 256, 512, 1024, 4096}},
       {"td_chunk", &ibfs_config::td_chunk, {1024, 2048, 4096, 8192}},
       {"emit_mode", &ibfs_config::emit_mode, {0, 1, 2}},
+      {"inspect_vpt", &ibfs_config::inspect_vpt, {0, 4, 8, 16}},
   };
   std::vector<Trial> trials;
   double best_g = 0, best_ms = 0;

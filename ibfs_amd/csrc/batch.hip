@@ -34,6 +34,7 @@ struct Ctx {
   LevelCtl *ctl = nullptr;
   Queues q{};
   u64 *hs = nullptr;
+  uint8_t *flags = nullptr;  // changed-at-this-level byte per traversal vertex
   Plan *d_plan = nullptr, *h_plan = nullptr;
   u64 *d_masks = nullptr, *h_masks = nullptr;  // [0,W) lanes_all, [W,2W) sel, then (R+1)W level masks
   int32_t *d_src = nullptr, *h_src = nullptr;
@@ -80,7 +81,7 @@ namespace {
 size_t masks_words(const ibfs_batch_s *b) { return (size_t)(2 + (b->R + 1)) * b->W; }
 
 u64 *slot(const ibfs_batch_s *b, const Ctx &c, int L) {
-  return c.ring + (size_t)(L % b->R) * (size_t)b->g->V * (size_t)b->W;
+  return c.ring + (size_t)(L % b->R) * (size_t)b->g->Vt * (size_t)b->W;
 }
 
 void release_ctx(Ctx &c) {
@@ -94,6 +95,7 @@ void release_ctx(Ctx &c) {
   for (int i = 0; i < 6; ++i) dfree(c.q.q[i]);
   dfree(c.q.chunks);
   dfree(c.hs);
+  dfree(c.flags);
   dfree(c.d_plan);
   hfree(c.h_plan);
   dfree(c.d_masks);
@@ -123,17 +125,18 @@ void release_batch(ibfs_batch_s *b) {
 // Output of the snapshot window [lo, hi] of context c: depth arrays, or the
 // closeness sums of the group's sources.
 hipError_t window_out(ibfs_batch_s *b, Ctx &c, int lo, int hi) {
+  const EmitSpace sp{b->g->d_cid, b->g->V, b->g->Vt};
   if (b->closeness)
-    return launch_close(b->W, c.ring, b->R, b->g->V, lo, hi, c.d_lanes, c.d_masks + 2 * b->W,
+    return launch_close(b->W, c.ring, b->R, sp, lo, hi, c.d_lanes, c.d_masks + 2 * b->W,
                         c.d_masks + b->W, c.k, b->d_sum + c.g0, b->d_reach + c.g0, c.st);
-  return launch_emit(b->W, b->cfg.emit_mode, 0, c.ring, b->R, b->g->V, lo, hi, c.d_lanes,
+  return launch_emit(b->W, b->cfg.emit_mode, 0, c.ring, b->R, sp, lo, hi, c.d_lanes,
                      c.d_masks + 2 * b->W, c.d_masks + b->W, c.k, c.st);
 }
 
 InspectArgs dd_inspect_args(const ibfs_batch_s *b, const Ctx &c) {
   InspectArgs a{};
   a.deg = b->g->d_deg;
-  a.V = b->g->V;
+  a.V = b->g->Vt;
   a.q = c.q;
   a.sml = b->cfg.sml_shed;
   a.lrg = b->cfg.lrg_shed;
@@ -146,12 +149,14 @@ InspectArgs dd_inspect_args(const ibfs_batch_s *b, const Ctx &c) {
   a.R = b->R;
   a.ctl_ring = c.ctl;
   a.lanes_all = c.d_masks;
+  a.flags = c.flags;
+  a.vpt = b->cfg.inspect_vpt;
   return a;
 }
 
 int alloc_ctx(ibfs_batch_s *b, Ctx &c) {
-  const int64_t V = b->g->V;
-  const size_t slot_bytes = (size_t)V * (size_t)b->W * sizeof(u64);
+  const int64_t Vt = b->g->Vt, Vq = std::max<int64_t>(Vt, 1);
+  const size_t slot_bytes = (size_t)Vq * (size_t)b->W * sizeof(u64);
   hipError_t err = hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking);
   auto A = [&](void **p, size_t bytes) {
     if (err == hipSuccess) err = hipMalloc(p, bytes ? bytes : 1);
@@ -162,9 +167,10 @@ int alloc_ctx(ibfs_batch_s *b, Ctx &c) {
   A((void **)&c.ring, (size_t)b->R * slot_bytes);
   A((void **)&c.ctl, sizeof(LevelCtl) * (size_t)b->R);
   for (int i = 0; i < 6; ++i)
-    if (i != Q_TD_L) A((void **)&c.q.q[i], sizeof(int32_t) * (size_t)V);
-  A((void **)&c.q.chunks, sizeof(int2) * ((size_t)V + (size_t)(b->g->E >> b->chunk_shift) + 2));
+    if (i != Q_TD_L) A((void **)&c.q.q[i], sizeof(int32_t) * (size_t)Vq);
+  A((void **)&c.q.chunks, sizeof(int2) * ((size_t)Vq + (size_t)(b->g->E >> b->chunk_shift) + 2));
   A((void **)&c.hs, sizeof(u64) * (size_t)b->W * (size_t)std::max<int64_t>(b->g->H, 1));
+  A((void **)&c.flags, sizeof(u64) * (size_t)((Vq + 7) / 8));
   A((void **)&c.d_plan, sizeof(Plan));
   H((void **)&c.h_plan, sizeof(Plan));
   A((void **)&c.d_masks, sizeof(u64) * masks_words(b));
@@ -179,7 +185,8 @@ int alloc_ctx(ibfs_batch_s *b, Ctx &c) {
                 "batch context allocation failed: %s", hipGetErrorString(err));
   // capture one level body: device-driven expansion + inspection
   HIP_TRY(hipStreamBeginCapture(c.st, hipStreamCaptureModeThreadLocal));
-  hipError_t e1 = launch_expand_dd(b->W, b->g->d_beg, b->g->d_adj, b->g->d_tadj, c.ring, b->R, V,
+  hipError_t e1 = launch_expand_dd(b->W, b->g->d_beg, b->g->d_adj, b->g->d_tadj, c.ring, c.flags,
+                                   b->R, Vt,
                                    c.ctl, c.d_plan, c.hs, c.q, b->chunk_shift, b->grid, c.st);
   hipError_t e2 = launch_inspect(b->W, dd_inspect_args(b, c), c.st);
   if (e2 == hipSuccess) e2 = launch_plan_decide(c.d_plan, c.ctl, b->R, c.st);
@@ -199,10 +206,10 @@ int start_group(ibfs_batch_s *b, Ctx &c, const int32_t *src, int64_t g0, int k, 
   std::memset(c.h_masks, 0, sizeof(u64) * masks_words(b));
   for (int i = 0; i < k; ++i) {
     c.h_masks[i >> 6] |= 1ull << (i & 63);
-    c.h_src[i] = src[g0 + i];
+    c.h_src[i] = b->g->h_cid[src[g0 + i]];
     c.h_lanes[i].out = dout ? dout + (size_t)i * (size_t)V : nullptr;
     c.h_lanes[i].admit = 0;
-    c.h_lanes[i].pad = 0;
+    c.h_lanes[i].src = src[g0 + i];
   }
   for (int w = 0; w < W; ++w) c.h_masks[W + w] = c.h_masks[w];
   for (size_t i = 0; i < (size_t)(b->R + 1) * W; ++i) c.h_masks[2 * W + i] = ~0ull;
@@ -218,9 +225,10 @@ int start_group(ibfs_batch_s *b, Ctx &c, const int32_t *src, int64_t g0, int k, 
   HIP_TRY(hipMemcpyAsync(c.d_src, c.h_src, sizeof(int32_t) * k, hipMemcpyHostToDevice, c.st));
   HIP_TRY(hipMemcpyAsync(c.d_lanes, c.h_lanes, sizeof(LaneOut) * k, hipMemcpyHostToDevice, c.st));
   HIP_TRY(hipMemcpyAsync(c.d_plan, c.h_plan, sizeof(Plan), hipMemcpyHostToDevice, c.st));
-  HIP_TRY(launch_fill_u64(slot(b, c, 0), V * W, 0ull, c.st));
+  HIP_TRY(launch_fill_u64(slot(b, c, 0), b->g->Vt * W, 0ull, c.st));
+  HIP_TRY(launch_fill_u64(reinterpret_cast<u64 *>(c.flags), (b->g->Vt + 7) / 8, 0ull, c.st));
   HIP_TRY(launch_fill_u64(reinterpret_cast<u64 *>(c.ctl), (int64_t)b->R * 16, 0ull, c.st));
-  HIP_TRY(launch_init_sources(W, c.d_src, k, 0, slot(b, c, 0), c.st));
+  HIP_TRY(launch_init_sources(W, c.d_src, k, 0, slot(b, c, 0), c.flags, c.st));
   HIP_TRY(launch_inspect(W, dd_inspect_args(b, c), c.st));
   HIP_TRY(launch_plan_decide(c.d_plan, c.ctl, b->R, c.st));
   c.g0 = g0;
@@ -254,7 +262,7 @@ int finish_group(ibfs_batch_s *b, Ctx &c, int last, ibfs_stats *stats) {
   }
   if (stats) {
     stats->levels = std::max(stats->levels, last + 1);
-    HIP_TRY(launch_count_edges(b->W, slot(b, c, last), b->g->d_deg, b->g->V, c.d_masks,
+    HIP_TRY(launch_count_edges(b->W, slot(b, c, last), b->g->d_deg, b->g->Vt, c.d_masks,
                                b->d_edges, c.st));
   }
   if (c.out_host)
@@ -287,6 +295,7 @@ int ibfs_batch_create(ibfs_graph_t g, const ibfs_config *cfg_in, int32_t context
   if (cfg.ring_levels < 4 || cfg.ring_levels > 255)
     return fail(IBFS_ERR_ARG, "ring_levels must be in [4, 255]");
   if (cfg.emit_mode < 0 || cfg.emit_mode > 2) return fail(IBFS_ERR_ARG, "bad emit_mode");
+  if (!valid_inspect_vpt(cfg.inspect_vpt)) return fail(IBFS_ERR_ARG, "bad inspect_vpt");
   if (cfg.device != g->device) return fail(IBFS_ERR_ARG, "config device differs from the graph's");
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
@@ -300,7 +309,7 @@ int ibfs_batch_create(ibfs_graph_t g, const ibfs_config *cfg_in, int32_t context
   b->C = contexts;
   b->burst = burst;
   b->chunk_shift = __builtin_ctz((unsigned)cfg.td_chunk);
-  const size_t slot_bytes = (size_t)g->V * (size_t)b->W * sizeof(u64);
+  const size_t slot_bytes = (size_t)std::max<int64_t>(g->Vt, 1) * (size_t)b->W * sizeof(u64);
   int R = cfg.ring_levels;
   const size_t budget = ((size_t)16 << 30) / (size_t)contexts;
   while (R > 4 && (size_t)R * slot_bytes > budget) --R;
@@ -456,6 +465,9 @@ int ibfs_batch_closeness(ibfs_batch_t b, const int32_t *src, int64_t n, int64_t 
   if (rc) return rc;
   HIP_TRY(hipMemcpy(depth_sum, b->d_sum, sizeof(u64) * (size_t)n, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(reach, b->d_reach, sizeof(u64) * (size_t)n, hipMemcpyDeviceToHost));
+  // an isolated source reaches itself (depth 0) and has no status word
+  for (int64_t i = 0; i < n; ++i)
+    if (b->g->h_cid[src[i]] < 0) reach[i] += 1;
   return IBFS_OK;
 }
 

@@ -25,6 +25,9 @@ struct HostCsr {
   std::vector<int32_t> adj;  // E
 };
 
+// inspector chunking override of ibfs_config (0 = by vertex count)
+inline bool valid_inspect_vpt(int v) { return v == 0 || v == 1 || v == 4 || v == 8 || v == 16; }
+
 // Parallel helper: runs fn(t) on nthreads std::threads.
 void parallel_for_threads(int nthreads, void (*fn)(int, void *), void *ctx);
 
@@ -35,14 +38,25 @@ struct ibfs_hostcsr_s {
 };
 
 // Device-resident graph (read-only after upload).
+//
+// Traversal vertex space: the Vt vertices with at least one edge (out or
+// in), numbered in their original order (cid[v] = traversal id of original
+// vertex v, -1 when v is isolated).  Status words, queues and the CSR below
+// use traversal ids; an isolated vertex can only ever be reached as its own
+// source, so it needs no status word.  Depth arrays and the ABI use original
+// ids (V of them).  An isolated vertex has no out-edges, so the CSR offsets
+// of the traversal vertices are the original offsets.
 struct ibfs_graph_s {
   int device = 0;
   int64_t V = 0, E = 0;
-  int64_t *d_beg = nullptr;
-  int32_t *d_adj = nullptr;   // adjacency as given (top-down, GroupBy)
+  int64_t Vt = 0;
+  int64_t *d_beg = nullptr;   // Vt+1, traversal ids
+  int32_t *d_adj = nullptr;   // adjacency in input order (top-down, GroupBy)
   int32_t *d_tadj = nullptr;  // traversal adjacency: by descending degree, hubs encoded
-  int32_t *d_deg = nullptr;
+  int32_t *d_deg = nullptr;   // Vt
   int32_t *d_hubs = nullptr;  // the H highest-degree vertices
+  int32_t *d_cid = nullptr;   // V: original id -> traversal id or -1
+  std::vector<int32_t> h_cid;
   int64_t H = 0;
   int64_t max_deg = 0;
 };

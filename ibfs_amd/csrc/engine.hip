@@ -40,6 +40,7 @@ struct ibfs_engine_s {
   u64 *d_count = nullptr;
   u64 *h_count = nullptr;
   u64 *hs = nullptr;          // hub cache, H * W words
+  uint8_t *flags = nullptr;   // changed-at-this-level byte per traversal vertex
   int emit_vec4 = 0;          // every lane output of this traversal is 16-byte aligned
   std::vector<ibfs_level_info> log;
   std::vector<hipEvent_t> ev;  // event pool
@@ -99,6 +100,7 @@ void engine_release(ibfs_engine_s *e) {
   dfree(e->d_count);
   hfree(e->h_count);
   dfree(e->hs);
+  dfree(e->flags);
   for (auto x : e->ev) (void)hipEventDestroy(x);
   e->ev.clear();
   if (e->st) (void)hipStreamDestroy(e->st);
@@ -106,7 +108,19 @@ void engine_release(ibfs_engine_s *e) {
 }
 
 u64 *slot(ibfs_engine_s *e, int L) {
-  return e->ring + (size_t)(L % e->R) * (size_t)e->g->V * (size_t)e->W;
+  return e->ring + (size_t)(L % e->R) * (size_t)e->g->Vt * (size_t)e->W;
+}
+
+EmitSpace emit_space(const ibfs_graph_s *g) { return EmitSpace{g->d_cid, g->V, g->Vt}; }
+
+// Distinct sources without a status word (isolated vertices): they change
+// at their admission level like any source, but outside the status words.
+int64_t isolated_sources(const ibfs_graph_s *g, const int32_t *src, int k) {
+  std::vector<int32_t> iso;
+  for (int i = 0; i < k; ++i)
+    if (g->h_cid[src[i]] < 0) iso.push_back(src[i]);
+  std::sort(iso.begin(), iso.end());
+  return (int64_t)(std::unique(iso.begin(), iso.end()) - iso.begin());
 }
 
 // direction of level L (0 top-down, 1 bottom-up) from the counters of L-1
@@ -134,7 +148,7 @@ InspectArgs inspect_args(ibfs_engine_s *e, const u64 *prev, const u64 *cur, u64 
   a.ctl = ctl;
   a.ctl_zero = ctl_zero;
   a.deg = e->g->d_deg;
-  a.V = e->g->V;
+  a.V = e->g->Vt;
   a.q = e->q;
   a.sml = e->cfg.sml_shed;
   a.lrg = e->cfg.lrg_shed;
@@ -142,6 +156,8 @@ InspectArgs inspect_args(ibfs_engine_s *e, const u64 *prev, const u64 *cur, u64 
   a.hubs = e->g->d_hubs;
   a.H = e->g->H;
   a.hs = e->hs;
+  a.flags = e->flags;
+  a.vpt = e->cfg.inspect_vpt;
   return a;
 }
 
@@ -152,8 +168,8 @@ int emit_window(ibfs_engine_s *e, int b, int en, int k, ibfs_stats *stats) {
     e0 = next_event(e);
     HIP_TRY(hipEventRecord(e0, e->st));
   }
-  HIP_TRY(launch_emit(e->W, e->cfg.emit_mode, e->emit_vec4, e->ring, e->R, e->g->V, b, en,
-                      e->d_lanes, e->d_masks + 2 * e->W, e->d_masks + e->W, k, e->st));
+  HIP_TRY(launch_emit(e->W, e->cfg.emit_mode, e->emit_vec4, e->ring, e->R, emit_space(e->g), b,
+                      en, e->d_lanes, e->d_masks + 2 * e->W, e->d_masks + e->W, k, e->st));
   if (e->cfg.kernel_timing) {
     e1 = next_event(e);
     HIP_TRY(hipEventRecord(e1, e->st));
@@ -169,7 +185,7 @@ int emit_window(ibfs_engine_s *e, int b, int en, int k, ibfs_stats *stats) {
 // One group of k sources (lanes 0..k-1), depths into dout (device, k*V).
 int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_stats *stats) {
   ibfs_graph_s *g = e->g;
-  const int64_t V = g->V;
+  const int64_t V = g->V, Vt = g->Vt;
   const int W = e->W;
   e->ev_used = 0;
   e->log.clear();
@@ -178,10 +194,10 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
   std::memset(e->h_masks, 0, sizeof(u64) * masks_words(e));
   for (int i = 0; i < k; ++i) {
     e->h_masks[i >> 6] |= 1ull << (i & 63);
-    e->h_src[i] = src[i];
+    e->h_src[i] = g->h_cid[src[i]];
     e->h_lanes[i].out = dout + (size_t)i * (size_t)V;
     e->h_lanes[i].admit = 0;
-    e->h_lanes[i].pad = 0;
+    e->h_lanes[i].src = src[i];
   }
   e->emit_vec4 = (V % 4 == 0) && ((uintptr_t)dout % 16 == 0);
   for (int w = 0; w < W; ++w) e->h_masks[W + w] = e->h_masks[w];
@@ -197,9 +213,10 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
   HIP_TRY(hipEventRecord(t_begin, e->st));
 
   // level 0: S_0 = source bits; inspection builds the level-1 queues
-  HIP_TRY(launch_fill_u64(slot(e, 0), V * W, 0ull, e->st));
+  HIP_TRY(launch_fill_u64(slot(e, 0), Vt * W, 0ull, e->st));
+  HIP_TRY(launch_fill_u64(reinterpret_cast<u64 *>(e->flags), (Vt + 7) / 8, 0ull, e->st));
   HIP_TRY(launch_fill_u64(reinterpret_cast<u64 *>(e->d_ctl), (int64_t)e->R * 16, 0ull, e->st));
-  HIP_TRY(launch_init_sources(W, e->d_src, k, 0, slot(e, 0), e->st));
+  HIP_TRY(launch_init_sources(W, e->d_src, k, 0, slot(e, 0), e->flags, e->st));
   hipEvent_t ti0 = nullptr, ti1 = nullptr;
   if (timing) { ti0 = next_event(e); HIP_TRY(hipEventRecord(ti0, e->st)); }
   HIP_TRY(launch_inspect(W, inspect_args(e, nullptr, slot(e, 0), slot(e, 1), e->d_masks,
@@ -231,7 +248,11 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
     if (stats) stats->inspect_ms += insp0;
   }
   if (stats) stats->n_inspect_launches++;
-  record(0, -1, *e->h_ctl, 0.f, insp0);
+  {
+    LevelCtl c0 = *e->h_ctl;
+    c0.n_changed += (u64)isolated_sources(g, src, k);
+    record(0, -1, c0, 0.f, insp0);
+  }
 
   int base = 0, last = 0, dir_prev = -1;
   u64 nf_prev = ~0ull;
@@ -248,20 +269,22 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
     hipEvent_t x0 = nullptr, x1 = nullptr, y1 = nullptr;
     if (timing) { x0 = next_event(e); HIP_TRY(hipEventRecord(x0, e->st)); }
     if (dir == 0) {
-      HIP_TRY(launch_expand_td(W, g->d_beg, g->d_adj, slot(e, L - 1), slot(e, L), lamp, e->q,
-                               c.q[Q_TD_S], c.q[Q_TD_M], c.q[Q_TD_L], e->chunk_shift, e->st));
+      HIP_TRY(launch_expand_td(W, g->d_beg, g->d_adj, slot(e, L - 1), slot(e, L), e->flags, lamp,
+                               e->q, c.q[Q_TD_S], c.q[Q_TD_M], c.q[Q_TD_L], e->chunk_shift,
+                               e->st));
       if (stats) stats->n_td_launches++;
     } else {
       if (e->cfg.work_model) {  // instrumentation: exact early-termination prefixes
         HIP_TRY(hipMemsetAsync(e->d_count + 1, 0, 2 * sizeof(u64), e->st));
-        HIP_TRY(launch_bu_model(W, g->d_beg, g->d_tadj, slot(e, L - 1), e->hs, lamp, V,
+        HIP_TRY(launch_bu_model(W, g->d_beg, g->d_tadj, slot(e, L - 1), e->hs, lamp, Vt,
                                 e->d_count + 1, e->st));
         HIP_TRY(hipMemcpyAsync(e->h_count + 1, e->d_count + 1, 2 * sizeof(u64),
                                hipMemcpyDeviceToHost, e->st));
         if (timing) HIP_TRY(hipEventRecord(x0, e->st));
       }
-      HIP_TRY(launch_expand_bu(W, g->d_beg, g->d_tadj, slot(e, L - 1), e->hs, slot(e, L), lamp,
-                               e->q, c.q[Q_BU_S], c.q[Q_BU_M], c.q[Q_BU_L], e->st));
+      HIP_TRY(launch_expand_bu(W, g->d_beg, g->d_tadj, slot(e, L - 1), e->hs, slot(e, L),
+                               e->flags, lamp, e->q, c.q[Q_BU_S], c.q[Q_BU_M], c.q[Q_BU_L],
+                               e->st));
       if (stats) stats->n_bu_launches++;
     }
     if (timing) { x1 = next_event(e); HIP_TRY(hipEventRecord(x1, e->st)); }
@@ -317,7 +340,7 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
     stats->levels = std::max(stats->levels, last + 1);
     // TEPS numerator over the final snapshot (outside the timed span)
     HIP_TRY(hipMemsetAsync(e->d_count, 0, sizeof(u64), e->st));
-    HIP_TRY(launch_count_edges(W, slot(e, last), g->d_deg, V, e->d_masks, e->d_count, e->st));
+    HIP_TRY(launch_count_edges(W, slot(e, last), g->d_deg, Vt, e->d_masks, e->d_count, e->st));
     HIP_TRY(hipMemcpyAsync(e->h_count, e->d_count, sizeof(u64), hipMemcpyDeviceToHost, e->st));
     HIP_TRY(hipStreamSynchronize(e->st));
     stats->traversed_edges += (int64_t)*e->h_count;
@@ -392,6 +415,14 @@ int ibfs_memcpy_h2d(void *dst, const void *src, size_t bytes) {
   return IBFS_OK;
 }
 
+int ibfs_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+  if (bytes && (!dst || !src)) return fail(IBFS_ERR_ARG, "null device pointer");
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  // a device-to-device hipMemcpy may return before the copy has finished
+  HIP_TRY(hipDeviceSynchronize());
+  return IBFS_OK;
+}
+
 int ibfs_graph_opts_default(ibfs_graph_opts *o) {
   if (!o) return fail(IBFS_ERR_ARG, "null options");
   std::memset(o, 0, sizeof *o);
@@ -432,37 +463,63 @@ int ibfs_graph_create_ex(const int64_t *beg, int64_t V, const int32_t *adj, int6
       return fail(IBFS_ERR_FORMAT, "adjacency entry %lld = %d out of range", (long long)i, adj[i]);
   int rc = check_device(device);
   if (rc) return rc;
-  // Hubs: the H highest-degree vertices (ties by id).  Traversal adjacency:
-  // each list ordered by descending neighbour degree (ties by id), hub
-  // neighbours encoded as 0x80000000 | hub slot.
-  const int64_t H = std::min<int64_t>(opts.hub_count, V);
-  std::vector<int32_t> order((size_t)V);
-  for (int64_t v = 0; v < V; ++v) order[v] = (int32_t)v;
+  int nthreads = opts.nthreads > 0 ? opts.nthreads : (int)std::thread::hardware_concurrency();
+  nthreads = std::max(1, std::min(nthreads, 64));
+  // Traversal vertex space: vertices with an out- or in-edge, original order.
+  std::vector<uint8_t> has_in((size_t)V, 0);
+  struct MarkCtx {
+    const int32_t *adj;
+    uint8_t *has_in;
+    int64_t E;
+    int nt;
+  } mx{adj, has_in.data(), E, nthreads};
+  parallel_for_threads(nthreads, [](int t, void *p) {
+    MarkCtx &c = *(MarkCtx *)p;
+    const int64_t lo = c.E * t / c.nt, hi = c.E * (t + 1) / c.nt;
+    for (int64_t j = lo; j < hi; ++j) __atomic_store_n(&c.has_in[c.adj[j]], 1, __ATOMIC_RELAXED);
+  }, &mx);
+  std::vector<int32_t> cid((size_t)V, -1);
+  int64_t Vt = 0;
+  for (int64_t v = 0; v < V; ++v)
+    if (deg[v] > 0 || has_in[v]) cid[v] = (int32_t)Vt++;
+  std::vector<int64_t> tbeg((size_t)Vt + 1);
+  std::vector<int32_t> tdeg((size_t)std::max<int64_t>(Vt, 1));
+  for (int64_t v = 0; v < V; ++v)
+    if (cid[v] >= 0) {
+      tbeg[cid[v]] = beg[v];
+      tdeg[cid[v]] = deg[v];
+    }
+  tbeg[Vt] = E;
+  // Hubs: the H highest-degree traversal vertices (ties by id).  Traversal
+  // adjacency: each list ordered by descending neighbour degree (ties by
+  // id), hub neighbours encoded as 0x80000000 | hub slot.
+  const int64_t H = std::min<int64_t>(opts.hub_count, Vt);
+  std::vector<int32_t> order((size_t)Vt);
+  for (int64_t v = 0; v < Vt; ++v) order[v] = (int32_t)v;
   std::stable_sort(order.begin(), order.end(),
-                   [&](int32_t a, int32_t b) { return deg[a] > deg[b]; });
-  std::vector<int32_t> slot_of((size_t)V, -1);
+                   [&](int32_t a, int32_t b) { return tdeg[a] > tdeg[b]; });
+  std::vector<int32_t> slot_of((size_t)std::max<int64_t>(Vt, 1), -1);
   for (int64_t i = 0; i < H; ++i) slot_of[order[i]] = (int32_t)i;
-  std::vector<int32_t> tadj((size_t)std::max<int64_t>(E, 1));
+  std::vector<int32_t> xadj((size_t)std::max<int64_t>(E, 1)), tadj((size_t)std::max<int64_t>(E, 1));
   struct Ctx {
     const int64_t *beg;
-    const int32_t *adj, *deg, *slot_of;
-    int32_t *tadj;
-    int64_t V;
+    const int32_t *adj, *deg, *slot_of, *cid;
+    int32_t *xadj, *tadj;
+    int64_t Vt;
     int nt;
-  } cx{beg, adj, deg.data(), slot_of.data(), tadj.data(), V, 1};
-  {
-    int nt = opts.nthreads > 0 ? opts.nthreads : (int)std::thread::hardware_concurrency();
-    cx.nt = std::max(1, std::min(nt, 64));
-  }
+  } cx{tbeg.data(), adj, tdeg.data(), slot_of.data(), cid.data(), xadj.data(), tadj.data(), Vt,
+       nthreads};
   parallel_for_threads(cx.nt, [](int t, void *p) {
     Ctx &c = *(Ctx *)p;
-    for (int64_t v = t; v < c.V; v += c.nt) {
-      int32_t *a = c.tadj + c.beg[v], *e = c.tadj + c.beg[v + 1];
-      std::copy(c.adj + c.beg[v], c.adj + c.beg[v + 1], a);
-      std::sort(a, e, [&](int32_t x, int32_t y) {
+    for (int64_t v = t; v < c.Vt; v += c.nt) {
+      const int64_t s = c.beg[v], e = c.beg[v + 1];
+      for (int64_t j = s; j < e; ++j) c.xadj[j] = c.cid[c.adj[j]];
+      int32_t *a = c.tadj + s, *ae = c.tadj + e;
+      std::copy(c.xadj + s, c.xadj + e, a);
+      std::sort(a, ae, [&](int32_t x, int32_t y) {
         return c.deg[x] != c.deg[y] ? c.deg[x] > c.deg[y] : x < y;
       });
-      for (int32_t *x = a; x < e; ++x)
+      for (int32_t *x = a; x < ae; ++x)
         if (c.slot_of[*x] >= 0) *x = (int32_t)(0x80000000u | (uint32_t)c.slot_of[*x]);
     }
   }, &cx);
@@ -470,25 +527,31 @@ int ibfs_graph_create_ex(const int64_t *beg, int64_t V, const int32_t *adj, int6
   g->device = device;
   g->V = V;
   g->E = E;
+  g->Vt = Vt;
   g->H = H;
   g->max_deg = maxd;
   const size_t eb = sizeof(int32_t) * (size_t)std::max<int64_t>(E, 1);
-  hipError_t err = hipMalloc((void **)&g->d_beg, sizeof(int64_t) * (size_t)(V + 1));
+  const size_t vtb = sizeof(int32_t) * (size_t)std::max<int64_t>(Vt, 1);
+  hipError_t err = hipMalloc((void **)&g->d_beg, sizeof(int64_t) * (size_t)(Vt + 1));
   if (err == hipSuccess) err = hipMalloc((void **)&g->d_adj, eb);
   if (err == hipSuccess) err = hipMalloc((void **)&g->d_tadj, eb);
-  if (err == hipSuccess) err = hipMalloc((void **)&g->d_deg, sizeof(int32_t) * (size_t)V);
+  if (err == hipSuccess) err = hipMalloc((void **)&g->d_deg, vtb);
   if (err == hipSuccess) err = hipMalloc((void **)&g->d_hubs, sizeof(int32_t) * (size_t)std::max<int64_t>(H, 1));
-  if (err == hipSuccess) err = hipMemcpy(g->d_beg, beg, sizeof(int64_t) * (size_t)(V + 1), hipMemcpyHostToDevice);
-  if (err == hipSuccess && E > 0) err = hipMemcpy(g->d_adj, adj, sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMalloc((void **)&g->d_cid, sizeof(int32_t) * (size_t)V);
+  if (err == hipSuccess) err = hipMemcpy(g->d_beg, tbeg.data(), sizeof(int64_t) * (size_t)(Vt + 1), hipMemcpyHostToDevice);
+  if (err == hipSuccess && E > 0) err = hipMemcpy(g->d_adj, xadj.data(), sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice);
   if (err == hipSuccess && E > 0) err = hipMemcpy(g->d_tadj, tadj.data(), sizeof(int32_t) * (size_t)E, hipMemcpyHostToDevice);
-  if (err == hipSuccess) err = hipMemcpy(g->d_deg, deg.data(), sizeof(int32_t) * (size_t)V, hipMemcpyHostToDevice);
+  if (err == hipSuccess && Vt > 0) err = hipMemcpy(g->d_deg, tdeg.data(), sizeof(int32_t) * (size_t)Vt, hipMemcpyHostToDevice);
   if (err == hipSuccess && H > 0) err = hipMemcpy(g->d_hubs, order.data(), sizeof(int32_t) * (size_t)H, hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(g->d_cid, cid.data(), sizeof(int32_t) * (size_t)V, hipMemcpyHostToDevice);
+  if (err == hipSuccess) g->h_cid = std::move(cid);
   if (err != hipSuccess) {
     dfree(g->d_beg);
     dfree(g->d_adj);
     dfree(g->d_tadj);
     dfree(g->d_deg);
     dfree(g->d_hubs);
+    dfree(g->d_cid);
     delete g;
     return fail(err == hipErrorOutOfMemory ? IBFS_ERR_NOMEM : IBFS_ERR_HIP,
                 "graph upload failed: %s", hipGetErrorString(err));
@@ -519,6 +582,13 @@ int ibfs_graph_info(ibfs_graph_t g, int64_t *V, int64_t *E) {
   return IBFS_OK;
 }
 
+int ibfs_graph_layout(ibfs_graph_t g, int64_t *traversal_vertices, int64_t *hubs) {
+  if (!g) return fail(IBFS_ERR_ARG, "null graph");
+  if (traversal_vertices) *traversal_vertices = g->Vt;
+  if (hubs) *hubs = g->H;
+  return IBFS_OK;
+}
+
 int ibfs_graph_destroy(ibfs_graph_t g) {
   if (!g) return IBFS_OK;
   (void)hipSetDevice(g->device);
@@ -527,6 +597,7 @@ int ibfs_graph_destroy(ibfs_graph_t g) {
   dfree(g->d_tadj);
   dfree(g->d_deg);
   dfree(g->d_hubs);
+  dfree(g->d_cid);
   delete g;
   return IBFS_OK;
 }
@@ -543,7 +614,7 @@ int ibfs_groupby(ibfs_graph_t g, const int32_t *src, int64_t n, int32_t *out) {
   HIP_TRY(hipMalloc((void **)&d, sizeof(int32_t) * (size_t)n * 3));
   hipError_t err = hipMemcpy(d, src, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice);
   if (err == hipSuccess)
-    err = launch_groupby(g->d_beg, g->d_adj, g->d_deg, d, n, d + n, d + 2 * n, nullptr);
+    err = launch_groupby(g->d_beg, g->d_adj, g->d_deg, g->d_cid, d, n, d + n, d + 2 * n, nullptr);
   if (err == hipSuccess)
     err = hipMemcpy(out, d + 2 * n, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost);
   (void)hipFree(d);
@@ -570,6 +641,8 @@ int ibfs_engine_create(ibfs_graph_t g, const ibfs_config *cfg_in, ibfs_engine_t 
     return fail(IBFS_ERR_ARG, "emit_mode must be 0, 1 or 2 (got %d)", cfg.emit_mode);
   if (cfg.ring_levels < 4 || cfg.ring_levels > 255)
     return fail(IBFS_ERR_ARG, "ring_levels must be in [4, 255] (got %d)", cfg.ring_levels);
+  if (!valid_inspect_vpt(cfg.inspect_vpt))
+    return fail(IBFS_ERR_ARG, "inspect_vpt must be 0, 1, 4, 8 or 16 (got %d)", cfg.inspect_vpt);
   if (cfg.device != g->device)
     return fail(IBFS_ERR_ARG, "config device %d differs from the graph's device %d", cfg.device, g->device);
   int rc = check_device(g->device);
@@ -580,9 +653,9 @@ int ibfs_engine_create(ibfs_graph_t g, const ibfs_config *cfg_in, ibfs_engine_t 
   e->K = cfg.lane_bits;
   e->W = cfg.lane_bits / 64;
   e->chunk_shift = __builtin_ctz((unsigned)cfg.td_chunk);
-  const int64_t V = g->V;
+  const int64_t Vt = std::max<int64_t>(g->Vt, 1);
   // ring: at most 8 GiB of snapshots, at least 4 levels
-  const size_t slot_bytes = (size_t)V * (size_t)e->W * sizeof(u64);
+  const size_t slot_bytes = (size_t)Vt * (size_t)e->W * sizeof(u64);
   int R = cfg.ring_levels;
   const size_t budget = (size_t)8 << 30;
   while (R > 4 && (size_t)R * slot_bytes > budget) --R;
@@ -598,8 +671,8 @@ int ibfs_engine_create(ibfs_graph_t g, const ibfs_config *cfg_in, ibfs_engine_t 
   A((void **)&e->d_ctl, sizeof(LevelCtl) * (size_t)R);
   H((void **)&e->h_ctl, sizeof(LevelCtl));
   for (int i = 0; i < 6; ++i)
-    if (i != Q_TD_L) A((void **)&e->q.q[i], sizeof(int32_t) * (size_t)V);
-  const size_t nchunks = (size_t)V + (size_t)(g->E >> e->chunk_shift) + 2;
+    if (i != Q_TD_L) A((void **)&e->q.q[i], sizeof(int32_t) * (size_t)Vt);
+  const size_t nchunks = (size_t)Vt + (size_t)(g->E >> e->chunk_shift) + 2;
   A((void **)&e->q.chunks, sizeof(int2) * nchunks);
   A((void **)&e->d_masks, sizeof(u64) * masks_words(e));
   H((void **)&e->h_masks, sizeof(u64) * masks_words(e));
@@ -608,6 +681,7 @@ int ibfs_engine_create(ibfs_graph_t g, const ibfs_config *cfg_in, ibfs_engine_t 
   A((void **)&e->d_lanes, sizeof(LaneOut) * (size_t)e->K);
   H((void **)&e->h_lanes, sizeof(LaneOut) * (size_t)e->K);
   A((void **)&e->hs, sizeof(u64) * (size_t)e->W * (size_t)std::max<int64_t>(g->H, 1));
+  A((void **)&e->flags, sizeof(u64) * (size_t)((Vt + 7) / 8));
   A((void **)&e->d_count, 4 * sizeof(u64));
   H((void **)&e->h_count, 4 * sizeof(u64));
   if (err != hipSuccess) {

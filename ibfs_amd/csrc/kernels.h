@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <cstddef>
 #include <cstdint>
 
 namespace ibfs {
@@ -9,7 +10,8 @@ namespace ibfs {
 typedef unsigned long long u64;
 
 // Per-level control block, written by the inspector of level L with device
-// atomics and read by the expanders of level L+1 and by the host.
+// atomics and read by the expanders of level L+1, the direction decision and
+// the host.
 struct alignas(128) LevelCtl {
   u64 lam[4];          // lanes that gained a vertex at level L (OR of new bits)
   u64 n_changed;       // vertices with new bits at L
@@ -21,17 +23,20 @@ struct alignas(128) LevelCtl {
   unsigned int pad[2];
 };
 static_assert(sizeof(LevelCtl) == 128, "LevelCtl layout");
+// the inspector reserves queue ranges two classes at a time (q[2j], q[2j+1]
+// as one little-endian 64-bit counter)
+static_assert(offsetof(LevelCtl, q) % 8 == 0, "LevelCtl queue pairs");
 
 enum { Q_TD_S = 0, Q_TD_M = 1, Q_TD_L = 2, Q_BU_S = 3, Q_BU_M = 4, Q_BU_L = 5 };
 
-// Device-resident plan of a device-driven traversal: the inspector's last
-// workgroup decides the next level's direction here, the expander reads it.
+// Device-resident plan of a device-driven traversal: k_plan_decide writes the
+// next level's direction here after each inspection, the expander reads it.
 struct Plan {
   int level;            // level to expand next (== level being inspected)
   int dir;              // direction of `level`: 0 top-down, 1 bottom-up
   int done;             // 1 once a level produced no change
   int last;             // deepest level with a change (valid when done)
-  unsigned int blocks_done;  // inspector workgroups finished (tail election)
+  unsigned int blocks_done;  // unused (kept for the layout the host copies)
   int policy, sw_level, alpha, beta;
   int pad;
   u64 nf_prev;          // frontier size of the level before `level`
@@ -43,7 +48,16 @@ struct Plan {
 struct LaneOut {
   int32_t *out;   // depth array of V int32 for this lane (null: lane not emitted)
   int32_t admit;  // level at which the lane's source was set (depth 0)
-  int32_t pad;
+  int32_t src;    // the lane's source, original id (-1: none)
+};
+
+// Vertex spaces of an emission: depth arrays span V original ids; status
+// words span Vt traversal ids (slot stride Vt*W); cid maps the first to the
+// second (-1 = isolated: no status word, depth 0 only at the lane's source).
+struct EmitSpace {
+  const int32_t *cid;
+  int64_t V;
+  int64_t Vt;
 };
 
 struct Queues {
@@ -54,6 +68,8 @@ struct Queues {
 // Arguments of one inspector pass.  Host-driven: prev/cur/next/lam_prev/ctl
 // given explicitly and plan == nullptr.  Device-driven: plan != nullptr and
 // the level, snapshot slots and control blocks are derived from plan->level.
+// flags: one byte per traversal vertex, set by whatever changed the vertex's
+// word at this level (expanders, source admission), cleared by the pass.
 struct InspectArgs {
   const u64 *prev;
   const u64 *cur;
@@ -75,44 +91,48 @@ struct InspectArgs {
   int R;
   LevelCtl *ctl_ring;
   const u64 *lanes_all; // lam_prev of level 0
+  uint8_t *flags;
+  int vpt;              // vertices per lane and chunk (0: by size)
 };
 
 // Launchers (all asynchronous on `st`).  W = status words (u64) per vertex.
 hipError_t launch_fill_u64(u64 *p, int64_t n, u64 value, hipStream_t st);
 hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t value, hipStream_t st);
+// S0[src[i]] |= bit lane0+i, i < k (src in traversal ids; -1 entries skipped).
 hipError_t launch_init_sources(int W, const int32_t *src, int k, int lane0, u64 *S0,
-                               hipStream_t st);
-// S[src[i]] |= bit lane[i], i < n.
+                               uint8_t *flags, hipStream_t st);
+// S[src[i]] |= bit lane[i], i < n (traversal ids; -1 entries skipped).
 hipError_t launch_admit(int W, const int32_t *src, const int32_t *lane, int n, u64 *S,
-                        hipStream_t st);
+                        uint8_t *flags, hipStream_t st);
 hipError_t launch_inspect(int W, const InspectArgs &a, hipStream_t st);
 hipError_t launch_expand_td(int W, const int64_t *beg, const int32_t *adj, const u64 *prev,
-                            u64 *cur, const u64 *lam_prev, Queues q, int64_t ns, int64_t nm,
-                            int64_t nl, int chunk_shift, hipStream_t st);
+                            u64 *cur, uint8_t *flags, const u64 *lam_prev, Queues q, int64_t ns,
+                            int64_t nm, int64_t nl, int chunk_shift, hipStream_t st);
 // tadj: traversal adjacency (neighbours by descending degree, hub entries
 // encoded as 0x80000000 | hub slot); hs: hub cache mirroring prev.
 hipError_t launch_expand_bu(int W, const int64_t *beg, const int32_t *tadj, const u64 *prev,
-                            const u64 *hs, u64 *cur, const u64 *lam_prev, Queues q, int64_t ns,
-                            int64_t nm, int64_t nl, hipStream_t st);
+                            const u64 *hs, u64 *cur, uint8_t *flags, const u64 *lam_prev,
+                            Queues q, int64_t ns, int64_t nm, int64_t nl, hipStream_t st);
 // Device-driven expansion of plan->level (either direction), fixed grid.
 hipError_t launch_expand_dd(int W, const int64_t *beg, const int32_t *adj, const int32_t *tadj,
-                            u64 *ring, int R, int64_t V, const LevelCtl *ctl_ring,
+                            u64 *ring, uint8_t *flags, int R, int64_t V, const LevelCtl *ctl_ring,
                             const Plan *plan, const u64 *hs, Queues q, int chunk_shift,
                             int grid, hipStream_t st);
 // Device-driven direction decision for plan->level + 1 (one thread).
 hipError_t launch_plan_decide(Plan *plan, const LevelCtl *ctl_ring, int R, hipStream_t st);
 // Depth emission over the snapshot window [b, e] (ring slots L % R, each
-// V*W words).  level_masks[(L-b)*W + w] = lanes admitted at or before L.
+// Vt*W words).  level_masks[(L-b)*W + w] = lanes admitted at or before L.
 // mode 0: one vertex per thread, per-lane 4-byte stores; mode 1: LDS-tiled,
 // 16-byte stores when vec4 (all lane outputs 16-byte aligned, V % 4 == 0);
 // mode 2: as 0 with non-temporal stores.
-hipError_t launch_emit(int W, int mode, int vec4, const u64 *ring, int R, int64_t V, int b, int e,
-                       const LaneOut *lanes, const u64 *level_masks, const u64 *sel,
+hipError_t launch_emit(int W, int mode, int vec4, const u64 *ring, int R, EmitSpace sp, int b,
+                       int e, const LaneOut *lanes, const u64 *level_masks, const u64 *sel,
                        int k_lanes, hipStream_t st);
 // Closeness reduction of the window [b, e]: sum_out[k] += sum of depths and
 // reach_out[k] += vertices reached, over the bits of lane k that arrived in
-// the window (lane table and masks as for launch_emit).
-hipError_t launch_close(int W, const u64 *ring, int R, int64_t V, int b, int e,
+// the window (lane table and masks as for launch_emit).  Isolated sources
+// are not counted here (their only reached vertex is themselves, depth 0).
+hipError_t launch_close(int W, const u64 *ring, int R, EmitSpace sp, int b, int e,
                         const LaneOut *lanes, const u64 *level_masks, const u64 *sel, int k_lanes,
                         u64 *sum_out, u64 *reach_out, hipStream_t st);
 hipError_t launch_count_edges(int W, const u64 *S, const int32_t *deg, int64_t V,
@@ -123,9 +143,11 @@ hipError_t launch_bu_model(int W, const int64_t *beg, const int32_t *tadj, const
                            const u64 *hs, const u64 *lam_prev, int64_t V, u64 *out,
                            hipStream_t st);
 // GroupBy (groupby.cu:36-149): key[i] = first neighbour of degree > 128 of a
-// source of degree <= 128, else -1; then the stable placement.
+// source of degree <= 128, else -1; then the stable placement.  src holds
+// original ids (cid maps them to the traversal CSR); keys are traversal ids,
+// whose order is the original order.
 hipError_t launch_groupby(const int64_t *beg, const int32_t *adj, const int32_t *deg,
-                          const int32_t *src, int64_t n, int32_t *key, int32_t *out,
-                          hipStream_t st);
+                          const int32_t *cid, const int32_t *src, int64_t n, int32_t *key,
+                          int32_t *out, hipStream_t st);
 
 }  // namespace ibfs

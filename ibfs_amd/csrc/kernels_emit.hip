@@ -27,20 +27,20 @@ __device__ __forceinline__ void put_depth(int32_t *out, int64_t v, int32_t val, 
 }
 
 template <int W, int NS>
-__device__ __forceinline__ void emit_count(const u64 *__restrict__ ring, int R, int64_t V, int b,
-                                           int e, int64_t v, const u64 *__restrict__ level_masks,
+__device__ __forceinline__ void emit_count(const u64 *__restrict__ ring, int R, int64_t Vt, int b,
+                                           int e, int64_t i, const u64 *__restrict__ level_masks,
                                            u64 (&c)[NS][W], u64 (&fin)[W], u64 (&bs)[W]) {
-  const int64_t slot = V * W;
+  const int64_t slot = Vt * W;
 #pragma unroll
   for (int w = 0; w < W; ++w) {
 #pragma unroll
     for (int j = 0; j < NS; ++j) c[j][w] = 0;
   }
-  load_word<W>(ring + (int64_t)(e % R) * slot + v * W, fin);
-  load_word<W>(ring + (int64_t)(b % R) * slot + v * W, bs);
+  load_word<W>(ring + (int64_t)(e % R) * slot + i * W, fin);
+  load_word<W>(ring + (int64_t)(b % R) * slot + i * W, bs);
   for (int L = b; L <= e; ++L) {
     u64 s[W];
-    load_word<W>(ring + (int64_t)(L % R) * slot + v * W, s);
+    load_word<W>(ring + (int64_t)(L % R) * slot + i * W, s);
 #pragma unroll
     for (int w = 0; w < W; ++w) {
       u64 carry = ~s[w] & level_masks[(int64_t)(L - b) * W + w];
@@ -54,22 +54,56 @@ __device__ __forceinline__ void emit_count(const u64 *__restrict__ ring, int R, 
   }
 }
 
+// Counters of a vertex without a status word (isolated, or past the end).
 template <int W, int NS>
-__global__ __launch_bounds__(BS) void k_emit(const u64 *__restrict__ ring, int R, int64_t V, int b,
-                                             int e, const LaneOut *__restrict__ lanes,
+__device__ __forceinline__ void emit_zero(u64 (&c)[NS][W], u64 (&fin)[W], u64 (&bs)[W]) {
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    fin[w] = bs[w] = 0;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) c[j][w] = 0;
+  }
+}
+
+// Depth of lane k at vertex v, or SKIP when this window must not write it.
+// ci < 0: v is isolated; its only depth is 0 at the lane's own source.
+constexpr int32_t SKIP = (int32_t)0x80000000;
+
+template <int W, int NS>
+__device__ __forceinline__ int32_t lane_depth(const u64 (&c)[NS][W], const u64 (&fin)[W],
+                                              const u64 (&bs)[W], int w, int bit, int off,
+                                              bool iso, bool is_src) {
+  int cnt = 0;
+#pragma unroll
+  for (int j = 0; j < NS; ++j) cnt |= (int)((c[j][w] >> bit) & 1ull) << j;
+  const bool set = iso ? is_src : (bool)((fin[w] >> bit) & 1ull);
+  const bool at_base = iso ? is_src : (bool)((bs[w] >> bit) & 1ull);
+  if (off <= 0) return set ? cnt : -1;
+  return (set && !at_base) ? off + cnt : SKIP;
+}
+
+template <int W, int NS>
+__global__ __launch_bounds__(BS) void k_emit(const u64 *__restrict__ ring, int R, EmitSpace sp,
+                                             int b, int e, const LaneOut *__restrict__ lanes,
                                              const u64 *__restrict__ level_masks,
                                              const u64 *__restrict__ sel, int k_lanes, int nt) {
   __shared__ int32_t *s_out[4 * 64];
   __shared__ int32_t s_off[4 * 64];  // b - admit: <= 0 in the lane's first window
+  __shared__ int32_t s_src[4 * 64];
   for (int i = threadIdx.x; i < k_lanes; i += BS) {
     const bool on = (sel[i >> 6] >> (i & 63)) & 1ull;
     s_out[i] = on ? lanes[i].out : nullptr;
     s_off[i] = b - lanes[i].admit;
+    s_src[i] = lanes[i].src;
   }
   __syncthreads();
-  for (int64_t v = (int64_t)blockIdx.x * BS + threadIdx.x; v < V; v += (int64_t)gridDim.x * BS) {
+  for (int64_t v = (int64_t)blockIdx.x * BS + threadIdx.x; v < sp.V;
+       v += (int64_t)gridDim.x * BS) {
+    const int32_t ci = sp.cid[v];
+    const bool iso = ci < 0;
     u64 c[NS][W], fin[W], bs[W];
-    emit_count<W, NS>(ring, R, V, b, e, v, level_masks, c, fin, bs);
+    if (!iso) emit_count<W, NS>(ring, R, sp.Vt, b, e, ci, level_masks, c, fin, bs);
+    else emit_zero<W, NS>(c, fin, bs);
 #pragma unroll
     for (int w = 0; w < W; ++w) {
       for (int bit = 0; bit < 64; ++bit) {
@@ -77,13 +111,9 @@ __global__ __launch_bounds__(BS) void k_emit(const u64 *__restrict__ ring, int R
         if (k >= k_lanes) break;
         int32_t *out = s_out[k];
         if (!out) continue;
-        const int off = s_off[k];
-        int cnt = 0;
-#pragma unroll
-        for (int j = 0; j < NS; ++j) cnt |= (int)((c[j][w] >> bit) & 1ull) << j;
-        const bool set = (fin[w] >> bit) & 1ull;
-        if (off <= 0) put_depth(out, v, set ? cnt : -1, nt);
-        else if (set && !((bs[w] >> bit) & 1ull)) put_depth(out, v, off + cnt, nt);
+        const int32_t val =
+            lane_depth<W, NS>(c, fin, bs, w, bit, s_off[k], iso, s_src[k] == (int32_t)v);
+        if (val != SKIP) put_depth(out, v, val, nt);
       }
     }
   }
@@ -96,38 +126,34 @@ __global__ __launch_bounds__(BS) void k_emit(const u64 *__restrict__ ring, int R
 // the sentinel INT_MIN and are skipped.  vec4 = every lane's output is
 // 16-byte aligned and V % 4 == 0.
 template <int W, int NS>
-__global__ __launch_bounds__(BS) void k_emit_tiled(const u64 *__restrict__ ring, int R, int64_t V,
-                                                   int b, int e,
+__global__ __launch_bounds__(BS) void k_emit_tiled(const u64 *__restrict__ ring, int R,
+                                                   EmitSpace sp, int b, int e,
                                                    const LaneOut *__restrict__ lanes,
                                                    const u64 *__restrict__ level_masks,
                                                    const u64 *__restrict__ sel, int k_lanes,
                                                    int vec4) {
-  constexpr int32_t SKIP = (int32_t)0x80000000;
   __shared__ int4 tile4[32 * 64];  // [row][64 x int4] = 32 lanes x 256 vertices
   __shared__ int32_t *s_out[4 * 64];
   __shared__ int32_t s_off[4 * 64];
+  __shared__ int32_t s_src[4 * 64];
   int32_t *tile = reinterpret_cast<int32_t *>(tile4);
   const int t = threadIdx.x;
+  const int64_t V = sp.V;
   for (int i = t; i < k_lanes; i += BS) {
     const bool on = (sel[i >> 6] >> (i & 63)) & 1ull;
     s_out[i] = on ? lanes[i].out : nullptr;
     s_off[i] = b - lanes[i].admit;
+    s_src[i] = lanes[i].src;
   }
   __syncthreads();
   const int64_t nblk = (V + BS - 1) / BS;
   for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
     const int64_t v = blk * BS + t;
+    const int32_t ci = (v < V) ? sp.cid[v] : -1;
+    const bool iso = ci < 0;
     u64 c[NS][W], fin[W], bs[W];
-    if (v < V) {
-      emit_count<W, NS>(ring, R, V, b, e, v, level_masks, c, fin, bs);
-    } else {
-#pragma unroll
-      for (int w = 0; w < W; ++w) {
-        fin[w] = bs[w] = 0;
-#pragma unroll
-        for (int j = 0; j < NS; ++j) c[j][w] = 0;
-      }
-    }
+    if (!iso) emit_count<W, NS>(ring, R, sp.Vt, b, e, ci, level_masks, c, fin, bs);
+    else emit_zero<W, NS>(c, fin, bs);
 #pragma unroll
     for (int w = 0; w < W; ++w) {
 #pragma unroll
@@ -138,15 +164,8 @@ __global__ __launch_bounds__(BS) void k_emit_tiled(const u64 *__restrict__ ring,
           const int k = lane0 + r;
           const int bit = h * 32 + r;
           int32_t val = SKIP;
-          if (k < k_lanes && s_out[k]) {
-            const int off = s_off[k];
-            int cnt = 0;
-#pragma unroll
-            for (int j = 0; j < NS; ++j) cnt |= (int)((c[j][w] >> bit) & 1ull) << j;
-            const bool set = (fin[w] >> bit) & 1ull;
-            if (off <= 0) val = set ? cnt : -1;
-            else if (set && !((bs[w] >> bit) & 1ull)) val = off + cnt;
-          }
+          if (k < k_lanes && s_out[k] && v < V)
+            val = lane_depth<W, NS>(c, fin, bs, w, bit, s_off[k], iso, s_src[k] == (int32_t)v);
           tile[r * BS + t] = val;
         }
         __syncthreads();
@@ -190,8 +209,8 @@ __global__ __launch_bounds__(BS) void k_emit_tiled(const u64 *__restrict__ ring,
 // depth = off_k + count).  Wave partials meet in LDS, one atomic per lane
 // and workgroup.
 template <int W, int NS>
-__global__ __launch_bounds__(BS) void k_close(const u64 *__restrict__ ring, int R, int64_t V, int b,
-                                              int e, const LaneOut *__restrict__ lanes,
+__global__ __launch_bounds__(BS) void k_close(const u64 *__restrict__ ring, int R, EmitSpace sp,
+                                              int b, int e, const LaneOut *__restrict__ lanes,
                                               const u64 *__restrict__ level_masks,
                                               const u64 *__restrict__ sel, int k_lanes,
                                               unsigned long long *__restrict__ sum_out,
@@ -212,20 +231,13 @@ __global__ __launch_bounds__(BS) void k_close(const u64 *__restrict__ ring, int 
   u64 acc_sum[W], acc_reach[W];
 #pragma unroll
   for (int w = 0; w < W; ++w) acc_sum[w] = acc_reach[w] = 0;
-  const int64_t nblk = (V + BS - 1) / BS;
+  // reduced over traversal ids: an isolated vertex contributes nothing
+  const int64_t nblk = (sp.Vt + BS - 1) / BS;
   for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
     const int64_t v = blk * BS + t;
     u64 c[NS][W], fin[W], bs[W], M[W];
-    if (v < V) {
-      emit_count<W, NS>(ring, R, V, b, e, v, level_masks, c, fin, bs);
-    } else {
-#pragma unroll
-      for (int w = 0; w < W; ++w) {
-        fin[w] = bs[w] = 0;
-#pragma unroll
-        for (int j = 0; j < NS; ++j) c[j][w] = 0;
-      }
-    }
+    if (v < sp.Vt) emit_count<W, NS>(ring, R, sp.Vt, b, e, v, level_masks, c, fin, bs);
+    else emit_zero<W, NS>(c, fin, bs);
 #pragma unroll
     for (int w = 0; w < W; ++w) {
       M[w] = fin[w] & ~(bs[w] & cont[w]) & selw[w];
@@ -298,14 +310,15 @@ __global__ __launch_bounds__(BS) void k_count_edges(const u64 *__restrict__ S,
 __global__ __launch_bounds__(BS) void k_groupby_key(const int64_t *__restrict__ beg,
                                                     const int32_t *__restrict__ adj,
                                                     const int32_t *__restrict__ deg,
+                                                    const int32_t *__restrict__ cid,
                                                     const int32_t *__restrict__ src, int64_t n,
                                                     int32_t *__restrict__ key) {
   const int lane = threadIdx.x & 63;
   const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= n) return;
-  const int32_t s = src[i];
+  const int32_t s = cid[src[i]];  // < 0: isolated, no neighbour to key on
   int32_t h = -1;
-  if (deg[s] <= 128) {
+  if (s >= 0 && deg[s] <= 128) {
     const int64_t e = beg[s + 1];
     for (int64_t j0 = beg[s]; j0 < e; j0 += 64) {
       const int64_t j = j0 + lane;
@@ -354,40 +367,40 @@ __global__ __launch_bounds__(BS) void k_groupby_place(const int32_t *__restrict_
 }  // namespace
 
 template <int W, int NS>
-static void emit_ns(int mode, int vec4, const u64 *ring, int R, int64_t V, int b, int e,
+static void emit_ns(int mode, int vec4, const u64 *ring, int R, EmitSpace sp, int b, int e,
                     const LaneOut *lanes, const u64 *masks, const u64 *sel, int k,
                     hipStream_t st) {
   if (mode == 1) {
-    dim3 g(grid_for(V, BS, 4096)), blk(BS);
-    hipLaunchKernelGGL((k_emit_tiled<W, NS>), g, blk, 0, st, ring, R, V, b, e, lanes, masks, sel,
+    dim3 g(grid_for(sp.V, BS, 4096)), blk(BS);
+    hipLaunchKernelGGL((k_emit_tiled<W, NS>), g, blk, 0, st, ring, R, sp, b, e, lanes, masks, sel,
                        k, vec4);
   } else {
-    dim3 g(grid_for(V, BS, 8192)), blk(BS);
-    hipLaunchKernelGGL((k_emit<W, NS>), g, blk, 0, st, ring, R, V, b, e, lanes, masks, sel, k,
+    dim3 g(grid_for(sp.V, BS, 8192)), blk(BS);
+    hipLaunchKernelGGL((k_emit<W, NS>), g, blk, 0, st, ring, R, sp, b, e, lanes, masks, sel, k,
                        mode == 2 ? 1 : 0);
   }
 }
 
 template <int W>
-static hipError_t emit_w(int mode, int vec4, const u64 *ring, int R, int64_t V, int b, int e,
+static hipError_t emit_w(int mode, int vec4, const u64 *ring, int R, EmitSpace sp, int b, int e,
                          const LaneOut *lanes, const u64 *masks, const u64 *sel, int k,
                          hipStream_t st) {
   const int span = e - b + 1;
-  if (span < 16) emit_ns<W, 4>(mode, vec4, ring, R, V, b, e, lanes, masks, sel, k, st);
-  else if (span < 64) emit_ns<W, 6>(mode, vec4, ring, R, V, b, e, lanes, masks, sel, k, st);
-  else if (span < 256) emit_ns<W, 8>(mode, vec4, ring, R, V, b, e, lanes, masks, sel, k, st);
+  if (span < 16) emit_ns<W, 4>(mode, vec4, ring, R, sp, b, e, lanes, masks, sel, k, st);
+  else if (span < 64) emit_ns<W, 6>(mode, vec4, ring, R, sp, b, e, lanes, masks, sel, k, st);
+  else if (span < 256) emit_ns<W, 8>(mode, vec4, ring, R, sp, b, e, lanes, masks, sel, k, st);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
-hipError_t launch_emit(int W, int mode, int vec4, const u64 *ring, int R, int64_t V, int b, int e,
-                       const LaneOut *lanes, const u64 *level_masks, const u64 *sel, int k_lanes,
-                       hipStream_t st) {
-  if (V <= 0 || e < b) return hipSuccess;
+hipError_t launch_emit(int W, int mode, int vec4, const u64 *ring, int R, EmitSpace sp, int b,
+                       int e, const LaneOut *lanes, const u64 *level_masks, const u64 *sel,
+                       int k_lanes, hipStream_t st) {
+  if (sp.V <= 0 || e < b) return hipSuccess;
   switch (W) {
-    case 1: return emit_w<1>(mode, vec4, ring, R, V, b, e, lanes, level_masks, sel, k_lanes, st);
-    case 2: return emit_w<2>(mode, vec4, ring, R, V, b, e, lanes, level_masks, sel, k_lanes, st);
-    case 4: return emit_w<4>(mode, vec4, ring, R, V, b, e, lanes, level_masks, sel, k_lanes, st);
+    case 1: return emit_w<1>(mode, vec4, ring, R, sp, b, e, lanes, level_masks, sel, k_lanes, st);
+    case 2: return emit_w<2>(mode, vec4, ring, R, sp, b, e, lanes, level_masks, sel, k_lanes, st);
+    case 4: return emit_w<4>(mode, vec4, ring, R, sp, b, e, lanes, level_masks, sel, k_lanes, st);
     default: return hipErrorInvalidValue;
   }
 }
@@ -406,40 +419,40 @@ hipError_t launch_count_edges(int W, const u64 *S, const int32_t *deg, int64_t V
 }
 
 template <int W>
-static hipError_t close_w(const u64 *ring, int R, int64_t V, int b, int e, const LaneOut *lanes,
+static hipError_t close_w(const u64 *ring, int R, EmitSpace sp, int b, int e, const LaneOut *lanes,
                           const u64 *masks, const u64 *sel, int k, u64 *sum, u64 *reach,
                           hipStream_t st) {
   const int span = e - b + 1;
-  dim3 g(grid_for(V, BS * 8, 256)), blk(BS);
+  dim3 g(grid_for(sp.Vt, BS * 8, 256)), blk(BS);
   if (span < 16)
-    hipLaunchKernelGGL((k_close<W, 4>), g, blk, 0, st, ring, R, V, b, e, lanes, masks, sel, k, sum, reach);
+    hipLaunchKernelGGL((k_close<W, 4>), g, blk, 0, st, ring, R, sp, b, e, lanes, masks, sel, k, sum, reach);
   else if (span < 64)
-    hipLaunchKernelGGL((k_close<W, 6>), g, blk, 0, st, ring, R, V, b, e, lanes, masks, sel, k, sum, reach);
+    hipLaunchKernelGGL((k_close<W, 6>), g, blk, 0, st, ring, R, sp, b, e, lanes, masks, sel, k, sum, reach);
   else if (span < 256)
-    hipLaunchKernelGGL((k_close<W, 8>), g, blk, 0, st, ring, R, V, b, e, lanes, masks, sel, k, sum, reach);
+    hipLaunchKernelGGL((k_close<W, 8>), g, blk, 0, st, ring, R, sp, b, e, lanes, masks, sel, k, sum, reach);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
-hipError_t launch_close(int W, const u64 *ring, int R, int64_t V, int b, int e,
+hipError_t launch_close(int W, const u64 *ring, int R, EmitSpace sp, int b, int e,
                         const LaneOut *lanes, const u64 *level_masks, const u64 *sel, int k_lanes,
                         u64 *sum_out, u64 *reach_out, hipStream_t st) {
-  if (V <= 0 || e < b) return hipSuccess;
+  if (sp.Vt <= 0 || e < b) return hipSuccess;
   switch (W) {
-    case 1: return close_w<1>(ring, R, V, b, e, lanes, level_masks, sel, k_lanes, sum_out, reach_out, st);
-    case 2: return close_w<2>(ring, R, V, b, e, lanes, level_masks, sel, k_lanes, sum_out, reach_out, st);
-    case 4: return close_w<4>(ring, R, V, b, e, lanes, level_masks, sel, k_lanes, sum_out, reach_out, st);
+    case 1: return close_w<1>(ring, R, sp, b, e, lanes, level_masks, sel, k_lanes, sum_out, reach_out, st);
+    case 2: return close_w<2>(ring, R, sp, b, e, lanes, level_masks, sel, k_lanes, sum_out, reach_out, st);
+    case 4: return close_w<4>(ring, R, sp, b, e, lanes, level_masks, sel, k_lanes, sum_out, reach_out, st);
     default: return hipErrorInvalidValue;
   }
 }
 
 hipError_t launch_groupby(const int64_t *beg, const int32_t *adj, const int32_t *deg,
-                          const int32_t *src, int64_t n, int32_t *key, int32_t *out,
-                          hipStream_t st) {
+                          const int32_t *cid, const int32_t *src, int64_t n, int32_t *key,
+                          int32_t *out, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_groupby_key, dim3(grid_for(n, 4, 1 << 30)), dim3(BS), 0, st, beg, adj, deg,
-                     src, n, key);
+                     cid, src, n, key);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
   hipLaunchKernelGGL(k_groupby_place, dim3(grid_for(n, BS, 1 << 30)), dim3(BS), 0, st, key, src, n,

@@ -20,24 +20,27 @@ __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t value) {
     p[i] = value;
 }
 
-// Lane init: lane lane0+i gets bit of src[i].  OR, so duplicate sources
-// and zero-degree sources keep their bit (the reference stores and skips
-// zero-degree sources, expander.cu:63,70-71).
+// Lane init: lane lane0+i gets bit of src[i] (a traversal id).  OR, so
+// duplicate sources keep their bit.  src[i] < 0 is an isolated source: it
+// has no status word and its depth array is -1 except 0 at the source (the
+// reference stores and skips zero-degree sources, expander.cu:63,70-71).
 template <int W>
-__global__ void k_init_sources(const int32_t *src, int k, int lane0, u64 *S0) {
+__global__ void k_init_sources(const int32_t *src, int k, int lane0, u64 *S0, uint8_t *flags) {
   int i = blockIdx.x * BS + threadIdx.x;
-  if (i >= k) return;
+  if (i >= k || src[i] < 0) return;
   int lane = lane0 + i;
   atomicOr(&S0[(int64_t)src[i] * W + (lane >> 6)], 1ull << (lane & 63));
+  flags[src[i]] = 1;
 }
 
 // Admission: bit lane[i] set in S[src[i]] (lanes of any position).
 template <int W>
-__global__ void k_admit(const int32_t *src, const int32_t *lane, int n, u64 *S) {
+__global__ void k_admit(const int32_t *src, const int32_t *lane, int n, u64 *S, uint8_t *flags) {
   int i = blockIdx.x * BS + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || src[i] < 0) return;
   const int k = lane[i];
   atomicOr(&S[(int64_t)src[i] * W + (k >> 6)], 1ull << (k & 63));
+  flags[src[i]] = 1;
 }
 
 // Direction of level L+1 from the counters of level L (same rule as the
@@ -47,10 +50,11 @@ __device__ void plan_decide(Plan *p, const LevelCtl *c, int L) {
   const u64 nf = atomic_read(&c->n_front);
   const u64 mf = atomic_read(&c->m_front);
   const u64 mc = atomic_read(&c->m_cand);
-  p->blocks_done = 0;
   if (nch == 0) {
     p->done = 1;
-    p->last = L - 1;
+    // level 0 always ends a window: lanes whose source is isolated change
+    // nothing in the status words but still have depth 0 at the source
+    p->last = L > 0 ? L - 1 : 0;
     return;
   }
   const int Ln = L + 1;
@@ -72,24 +76,36 @@ __device__ void plan_decide(Plan *p, const LevelCtl *c, int L) {
 }
 
 // ---------------------------------------------------------------------------
-// Inspector of level L: one pass over all vertices.  Diffs cur (S_L) against
-// prev (S_{L-1}), copies S_L into next (S_{L+1}, the slot the expanders of
-// L+1 update), refreshes the hub cache from S_L, accumulates the lanes that
-// advanced (lam of L), and compacts, per degree class, both the top-down
-// frontier (changed, degree > 0) and the bottom-up candidates (degree > 0 and
-// missing a lane of lam_prev) of level L+1.  Replaces td/sw_frontier_count +
-// insp_scan + td/sw_frontier_gather + extract_depth of inspector.cu:37-500
-// and scan.cuh:455-904 with a single kernel: per-tile flags in LDS, a
-// workgroup scan and one atomic per class and tile.  Queue order is
-// ascending vertex order within a tile.  Device-driven mode: the level and
-// its slots come from the plan; k_plan_decide then picks the next direction.
-template <int W, int VPT>
-__global__ __launch_bounds__(BS) void k_inspect(InspectArgs a) {
-  constexpr int TILE = BS * VPT;
-  __shared__ uint8_t fl[TILE];
-  __shared__ unsigned wsum[4][6];
-  __shared__ unsigned qbase[6];
-  __shared__ u64 red[4][5 + W];
+// Inspector of level L: one pass over the traversal vertices.  Copies S_L
+// (cur) into S_{L+1} (next, the slot the expanders of L+1 update), refreshes
+// the hub cache from S_L, and for the vertices the expanders (or the source
+// admission) flagged as changed at L, clears the flag, diffs S_L against
+// S_{L-1} (prev is read only for them) and accumulates the lanes that
+// advanced (lam of L).  It compacts, per degree class, both the top-down
+// frontier of L+1 (changed, degree > 0) and the bottom-up candidates
+// (degree > 0 and missing a lane of lam_prev).  Replaces
+// td/sw_frontier_count + insp_scan + td/sw_frontier_gather + extract_depth
+// of inspector.cu:37-500 and scan.cuh:455-904 with a single kernel.
+//
+// A workgroup of NW waves takes tiles of NW*64*VPT consecutive vertices;
+// lane l of wave w handles vertex (w*VPT + i)*64 + l in round i.  A wave
+// issues all loads of its rounds before any store, classifies in registers,
+// and the waves' class totals meet in LDS, so each tile reserves its queue
+// ranges with three 64-bit atomics (two classes each): same-address device
+// atomics serialize, so their number per level is what this layout keeps
+// small.  Entries go to ballot prefix positions, ascending vertex order
+// within a tile.  Device-driven mode: the level and its slots come from the
+// plan; k_plan_decide then picks the next direction.
+__device__ __forceinline__ unsigned lanes_below(u64 m) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+template <int W, int VPT, int NW>
+__global__ __launch_bounds__(NW * 64) void k_inspect(InspectArgs a) {
+  constexpr int IBS = NW * 64;
+  __shared__ unsigned wsum[2][NW][6];   // per-wave class totals, double-buffered by tile parity
+  __shared__ unsigned wbase[2][NW][6];  // per-wave queue bases
+  __shared__ u64 red[NW][5 + W];
 
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const u64 *prev = a.prev, *cur = a.cur;
@@ -109,7 +125,7 @@ __global__ __launch_bounds__(BS) void k_inspect(InspectArgs a) {
   }
   if (blockIdx.x == 0 && ctl_zero && t < (int)(sizeof(LevelCtl) / 8))
     reinterpret_cast<u64 *>(ctl_zero)[t] = 0;
-  for (int64_t i = (int64_t)blockIdx.x * BS + t; i < a.H; i += (int64_t)gridDim.x * BS) {
+  for (int64_t i = (int64_t)blockIdx.x * IBS + t; i < a.H; i += (int64_t)gridDim.x * IBS) {
     u64 x[W];
     load_word<W>(cur + (int64_t)a.hubs[i] * W, x);
     store_word<W>(a.hs + i * W, x);
@@ -126,105 +142,144 @@ __global__ __launch_bounds__(BS) void k_inspect(InspectArgs a) {
   u64 n_changed = 0, n_front = 0, m_front = 0, n_cand = 0, m_cand = 0;
   const int64_t V = a.V;
   const int sml = a.sml, lrg = a.lrg, cs = a.chunk_shift;
+  uint8_t *flags = a.flags;
+  const u64 *__restrict__ curp = cur;
+  const u64 *__restrict__ prevp = prev;
+  const int32_t *__restrict__ degp = a.deg;
+  // queue lengths as three 64-bit pairs (q[2j] low half, q[2j+1] high half)
+  unsigned long long *qpair = reinterpret_cast<unsigned long long *>(ctl->q);
 
+  constexpr int TILE = IBS * VPT;
   const int64_t ntiles = (V + TILE - 1) / TILE;
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int64_t t0 = tile * TILE;
+  int par = 0;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x, par ^= 1) {
+    const int64_t c0 = tile * TILE + (int64_t)wv * 64 * VPT;
+    // every load of the wave's rounds first, so that none waits behind a store
+    uint8_t fl[VPT];
+    int dgv[VPT];
+    u64 cw[VPT][W], pw[VPT][W];
 #pragma unroll
     for (int i = 0; i < VPT; ++i) {
-      const int64_t v = t0 + i * BS + t;
-      uint8_t f = 0;
-      if (v < V) {
-        u64 c[W], p[W], n[W];
-        load_word<W>(cur + v * W, c);
-        if (prev) {
-          load_word<W>(prev + v * W, p);
-        } else {
+      const int64_t v = c0 + i * 64 + lane;
+      fl[i] = 0;
+      dgv[i] = 0;
 #pragma unroll
-          for (int w = 0; w < W; ++w) p[w] = 0;
-        }
+      for (int w = 0; w < W; ++w) cw[i][w] = pw[i][w] = 0;
+      if (v < V) {
+        fl[i] = flags[v];
+        dgv[i] = degp[v];
+        load_word<W>(curp + v * W, cw[i]);
+      }
+    }
+    if (prevp) {
+#pragma unroll
+      for (int i = 0; i < VPT; ++i)
+        if (fl[i]) load_word<W>(prevp + (c0 + i * 64 + lane) * W, pw[i]);
+    }
+    uint8_t code[VPT];  // bits 0-1 top-down class, bits 2-3 bottom-up class
+    unsigned tot[6] = {0, 0, 0, 0, 0, 0};  // wave totals (uniform)
+#pragma unroll
+    for (int i = 0; i < VPT; ++i) {
+      const int64_t v = c0 + i * 64 + lane;
+      int tc = 0, bc = 0, nck = 0;
+      if (v < V) {
         bool ch = false, cover = true;
+        u64 n[W];
+        if (fl[i]) {
+          flags[v] = 0;
+#pragma unroll
+          for (int w = 0; w < W; ++w) {
+            const u64 d = cw[i][w] & ~(pw[i][w] & keep[w]);
+            lamacc[w] |= d;
+            ch |= (d != 0);
+          }
+        }
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-          const u64 d = c[w] & ~(p[w] & keep[w]);
-          lamacc[w] |= d;
-          ch |= (d != 0);
-          n[w] = c[w] & nand[w];
-          cover &= ((c[w] & lp[w]) == lp[w]);
+          n[w] = cw[i][w] & nand[w];
+          cover &= ((cw[i][w] & lp[w]) == lp[w]);
         }
         store_word<W>(next + v * W, n);
-        const int dg = a.deg[v];
+        const int dg = dgv[i];
         if (ch) {
           ++n_changed;
           if (dg > 0) {
             ++n_front;
             m_front += (u64)dg;
-            f |= (uint8_t)degree_class(dg, sml, lrg);
+            tc = degree_class(dg, sml, lrg);
+            if (tc == 3) nck = (dg + (1 << cs) - 1) >> cs;
           }
         }
         if (dg > 0 && !cover) {
           ++n_cand;
           m_cand += (u64)dg;
-          f |= (uint8_t)(degree_class(dg, sml, lrg) << 2);
+          bc = degree_class(dg, sml, lrg);
         }
       }
-      fl[i * BS + t] = f;
+      code[i] = (uint8_t)(tc | (bc << 2));
+      tot[Q_TD_S] += (unsigned)__popcll(__ballot(tc == 1));
+      tot[Q_TD_M] += (unsigned)__popcll(__ballot(tc == 2));
+      if (__ballot(tc == 3)) tot[Q_TD_L] += (unsigned)wave_sum((u64)nck);
+      tot[Q_BU_S] += (unsigned)__popcll(__ballot(bc == 1));
+      tot[Q_BU_M] += (unsigned)__popcll(__ballot(bc == 2));
+      tot[Q_BU_L] += (unsigned)__popcll(__ballot(bc == 3));
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) wsum[par][wv][k] = tot[k];
     }
     __syncthreads();
-
-    unsigned cnt[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int j = 0; j < VPT; ++j) {
-      const uint8_t f = fl[t * VPT + j];
-      const int tc = f & 3, bc = (f >> 2) & 3;
-      if (tc == 1) {
-        ++cnt[Q_TD_S];
-      } else if (tc == 2) {
-        ++cnt[Q_TD_M];
-      } else if (tc == 3) {
-        const int dg = a.deg[t0 + t * VPT + j];
-        cnt[Q_TD_L] += (unsigned)((dg + (1 << cs) - 1) >> cs);
+    // reservation for the tile: thread j < 3 adds classes 2j, 2j+1 in one
+    // 64-bit atomic and lays out the waves' bases in wave order
+    if (t < 3) {
+      unsigned lo = 0, hi = 0;
+      for (int w = 0; w < NW; ++w) {
+        lo += wsum[par][w][2 * t];
+        hi += wsum[par][w][2 * t + 1];
       }
-      if (bc) ++cnt[Q_BU_S + bc - 1];
-    }
-    unsigned ex[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      const unsigned inc = wave_incl_scan(cnt[k], lane);
-      ex[k] = inc - cnt[k];
-      if (lane == 63) wsum[wv][k] = inc;
-    }
-    __syncthreads();
-    if (t < 6) {
-      const unsigned tot = wsum[0][t] + wsum[1][t] + wsum[2][t] + wsum[3][t];
-      qbase[t] = tot ? atomicAdd(&ctl->q[t], tot) : 0u;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      unsigned add = qbase[k];
-      for (int w = 0; w < wv; ++w) add += wsum[w][k];
-      ex[k] += add;
-    }
-#pragma unroll
-    for (int j = 0; j < VPT; ++j) {
-      const uint8_t f = fl[t * VPT + j];
-      if (!f) continue;
-      const int32_t v = (int32_t)(t0 + t * VPT + j);
-      const int tc = f & 3, bc = (f >> 2) & 3;
-      if (tc == 1) {
-        a.q.q[Q_TD_S][ex[Q_TD_S]++] = v;
-      } else if (tc == 2) {
-        a.q.q[Q_TD_M][ex[Q_TD_M]++] = v;
-      } else if (tc == 3) {
-        const int nch = (a.deg[v] + (1 << cs) - 1) >> cs;
-        for (int c = 0; c < nch; ++c) a.q.chunks[ex[Q_TD_L]++] = make_int2(v, c);
+      const unsigned long long add = (unsigned long long)lo | ((unsigned long long)hi << 32);
+      const unsigned long long got = add ? atomicAdd(&qpair[t], add) : 0ull;
+      unsigned blo = (unsigned)got, bhi = (unsigned)(got >> 32);
+      for (int w = 0; w < NW; ++w) {
+        wbase[par][w][2 * t] = blo;
+        wbase[par][w][2 * t + 1] = bhi;
+        blo += wsum[par][w][2 * t];
+        bhi += wsum[par][w][2 * t + 1];
       }
-      if (bc == 1) a.q.q[Q_BU_S][ex[Q_BU_S]++] = v;
-      else if (bc == 2) a.q.q[Q_BU_M][ex[Q_BU_M]++] = v;
-      else if (bc == 3) a.q.q[Q_BU_L][ex[Q_BU_L]++] = v;
     }
     __syncthreads();
+    unsigned base[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) base[k] = wbase[par][wv][k];
+#pragma unroll
+    for (int i = 0; i < VPT; ++i) {
+      const int tc = code[i] & 3, bc = code[i] >> 2;
+      const int32_t v = (int32_t)(c0 + i * 64 + lane);
+      u64 m = __ballot(tc == 1);
+      if (tc == 1) a.q.q[Q_TD_S][base[Q_TD_S] + lanes_below(m)] = v;
+      base[Q_TD_S] += (unsigned)__popcll(m);
+      m = __ballot(tc == 2);
+      if (tc == 2) a.q.q[Q_TD_M][base[Q_TD_M] + lanes_below(m)] = v;
+      base[Q_TD_M] += (unsigned)__popcll(m);
+      if (__ballot(tc == 3)) {  // hub frontier vertices: (vertex, chunk) pairs
+        const int nck = (tc == 3) ? (dgv[i] + (1 << cs) - 1) >> cs : 0;
+        const unsigned inc = wave_incl_scan((unsigned)nck, lane);
+        if (tc == 3) {
+          const unsigned p0 = base[Q_TD_L] + inc - (unsigned)nck;
+          for (int c = 0; c < nck; ++c) a.q.chunks[p0 + c] = make_int2(v, c);
+        }
+        base[Q_TD_L] += (unsigned)__shfl((int)inc, 63, 64);
+      }
+      m = __ballot(bc == 1);
+      if (bc == 1) a.q.q[Q_BU_S][base[Q_BU_S] + lanes_below(m)] = v;
+      base[Q_BU_S] += (unsigned)__popcll(m);
+      m = __ballot(bc == 2);
+      if (bc == 2) a.q.q[Q_BU_M][base[Q_BU_M] + lanes_below(m)] = v;
+      base[Q_BU_M] += (unsigned)__popcll(m);
+      m = __ballot(bc == 3);
+      if (bc == 3) a.q.q[Q_BU_L][base[Q_BU_L] + lanes_below(m)] = v;
+      base[Q_BU_L] += (unsigned)__popcll(m);
+    }
   }
 
   // workgroup reduction of the level counters, one atomic each
@@ -241,10 +296,12 @@ __global__ __launch_bounds__(BS) void k_inspect(InspectArgs a) {
   }
   __syncthreads();
   if (t < 5) {
-    const u64 s = red[0][t] + red[1][t] + red[2][t] + red[3][t];
+    u64 s = 0;
+    for (int w = 0; w < NW; ++w) s += red[w][t];
     if (s) atomicAdd(&(&ctl->n_changed)[t], s);
   } else if (t < 5 + W) {
-    const u64 s = red[0][t] | red[1][t] | red[2][t] | red[3][t];
+    u64 s = 0;
+    for (int w = 0; w < NW; ++w) s |= red[w][t];
     if (s) atomicOr(&ctl->lam[t - 5], s);
   }
 }
@@ -263,10 +320,12 @@ __global__ void k_plan_decide(Plan *plan, const LevelCtl *ctl_ring, int R) {
 // Top-down: a frontier vertex u pushes S_{L-1}[u] & lam_{L-1} into S_L of its
 // neighbours with 64-bit atomic ORs, skipping neighbours that already hold
 // the bits (expander.cu:100-252).
+// A neighbour whose word misses a pushed bit when read changes at this level
+// (by this push or a concurrent one), so it is flagged for the inspector.
 template <int W>
 __device__ __forceinline__ void td_push(const int64_t *__restrict__ beg,
                                         const int32_t *__restrict__ adj,
-                                        const u64 *__restrict__ prev, u64 *cur,
+                                        const u64 *__restrict__ prev, u64 *cur, uint8_t *flags,
                                         const u64 (&lam)[W], int32_t u, int64_t j0, int64_t j1,
                                         int stride) {
   u64 p[W];
@@ -277,9 +336,14 @@ __device__ __forceinline__ void td_push(const int64_t *__restrict__ beg,
     const int64_t v = adj[j];
     u64 c[W];
     load_word<W>(cur + v * W, c);
+    bool hit = false;
 #pragma unroll
     for (int w = 0; w < W; ++w)
-      if (p[w] & ~c[w]) atomicOr(&cur[v * W + w], p[w]);
+      if (p[w] & ~c[w]) {
+        atomicOr(&cur[v * W + w], p[w]);
+        hit = true;
+      }
+    if (hit) flags[v] = 1;
   }
 }
 
@@ -300,7 +364,7 @@ template <int W>
 __device__ __forceinline__ void bu_thread(const int64_t *__restrict__ beg,
                                           const int32_t *__restrict__ tadj,
                                           const u64 *__restrict__ prev,
-                                          const u64 *__restrict__ hs, u64 *__restrict__ cur,
+                                          const u64 *__restrict__ hs, u64 *__restrict__ cur, uint8_t *flags,
                                           const u64 (&lam)[W], int64_t v) {
   u64 x[W], x0[W];
   load_word<W>(cur + v * W, x);
@@ -324,7 +388,10 @@ __device__ __forceinline__ void bu_thread(const int64_t *__restrict__ beg,
     j += 4;
     if (covers<W>(x, lam)) break;
   }
-  if (differs<W>(x, x0)) store_word<W>(cur + v * W, x);
+  if (differs<W>(x, x0)) {
+    store_word<W>(cur + v * W, x);
+    flags[v] = 1;
+  }
 }
 
 // One candidate per wave: 64 neighbours per step, wave OR-reduction.
@@ -332,7 +399,7 @@ template <int W>
 __device__ __forceinline__ void bu_wave(const int64_t *__restrict__ beg,
                                         const int32_t *__restrict__ tadj,
                                         const u64 *__restrict__ prev, const u64 *__restrict__ hs,
-                                        u64 *__restrict__ cur, const u64 (&lam)[W], int64_t v,
+                                        u64 *__restrict__ cur, uint8_t *flags, const u64 (&lam)[W], int64_t v,
                                         int lane) {
   u64 x[W], x0[W];
   load_word<W>(cur + v * W, x);
@@ -353,7 +420,10 @@ __device__ __forceinline__ void bu_wave(const int64_t *__restrict__ beg,
     for (int w = 0; w < W; ++w) x[w] |= wave_or(y[w] & lam[w]);
     if (covers<W>(x, lam)) break;
   }
-  if (lane == 0 && differs<W>(x, x0)) store_word<W>(cur + v * W, x);
+  if (lane == 0 && differs<W>(x, x0)) {
+    store_word<W>(cur + v * W, x);
+    flags[v] = 1;
+  }
 }
 
 // One candidate per workgroup: 256 neighbours per step, the four wave
@@ -362,7 +432,7 @@ template <int W>
 __device__ __forceinline__ void bu_block(const int64_t *__restrict__ beg,
                                          const int32_t *__restrict__ tadj,
                                          const u64 *__restrict__ prev, const u64 *__restrict__ hs,
-                                         u64 *__restrict__ cur, const u64 (&lam)[W], int64_t v,
+                                         u64 *__restrict__ cur, uint8_t *flags, const u64 (&lam)[W], int64_t v,
                                          u64 (*part)[W]) {
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   u64 x[W], x0[W];
@@ -392,7 +462,10 @@ __device__ __forceinline__ void bu_block(const int64_t *__restrict__ beg,
     __syncthreads();
     if (covers<W>(x, lam)) break;
   }
-  if (t == 0 && differs<W>(x, x0)) store_word<W>(cur + v * W, x);
+  if (t == 0 && differs<W>(x, x0)) {
+    store_word<W>(cur + v * W, x);
+    flags[v] = 1;
+  }
 }
 
 // Host-driven top-down: workgroups [0, nbs) take small vertices one per
@@ -402,9 +475,9 @@ template <int W>
 __global__ __launch_bounds__(BS) void k_expand_td(const int64_t *__restrict__ beg,
                                                   const int32_t *__restrict__ adj,
                                                   const u64 *__restrict__ prev, u64 *cur,
-                                                  const u64 *__restrict__ lam_prev, Queues q,
-                                                  int64_t ns, int64_t nm, int64_t nl, int64_t nbs,
-                                                  int64_t nbm, int cs) {
+                                                  uint8_t *flags, const u64 *__restrict__ lam_prev,
+                                                  Queues q, int64_t ns, int64_t nm, int64_t nl,
+                                                  int64_t nbs, int64_t nbm, int cs) {
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int64_t b = blockIdx.x;
   u64 lam[W];
@@ -414,19 +487,19 @@ __global__ __launch_bounds__(BS) void k_expand_td(const int64_t *__restrict__ be
     const int64_t i = b * BS + t;
     if (i >= ns) return;
     const int32_t u = q.q[Q_TD_S][i];
-    td_push<W>(beg, adj, prev, cur, lam, u, beg[u], beg[u + 1], 1);
+    td_push<W>(beg, adj, prev, cur, flags, lam, u, beg[u], beg[u + 1], 1);
   } else if (b < nbs + nbm) {
     const int64_t i = (b - nbs) * 4 + wv;
     if (i >= nm) return;
     const int32_t u = q.q[Q_TD_M][i];
-    td_push<W>(beg, adj, prev, cur, lam, u, beg[u] + lane, beg[u + 1], 64);
+    td_push<W>(beg, adj, prev, cur, flags, lam, u, beg[u] + lane, beg[u + 1], 64);
   } else {
     const int64_t i = b - nbs - nbm;
     if (i >= nl) return;
     const int2 e = q.chunks[i];
     int64_t lo, hi;
     td_chunk_range(beg, e, cs, lo, hi);
-    td_push<W>(beg, adj, prev, cur, lam, e.x, lo + t, hi, BS);
+    td_push<W>(beg, adj, prev, cur, flags, lam, e.x, lo + t, hi, BS);
   }
 }
 
@@ -436,7 +509,7 @@ __global__ __launch_bounds__(BS) void k_expand_bu(const int64_t *__restrict__ be
                                                   const int32_t *__restrict__ tadj,
                                                   const u64 *__restrict__ prev,
                                                   const u64 *__restrict__ hs,
-                                                  u64 *__restrict__ cur,
+                                                  u64 *__restrict__ cur, uint8_t *flags,
                                                   const u64 *__restrict__ lam_prev, Queues q,
                                                   int64_t ns, int64_t nm, int64_t nl, int64_t nbs,
                                                   int64_t nbm) {
@@ -448,13 +521,13 @@ __global__ __launch_bounds__(BS) void k_expand_bu(const int64_t *__restrict__ be
   for (int w = 0; w < W; ++w) lam[w] = lam_prev[w];
   if (b < nbs) {
     const int64_t i = b * BS + t;
-    if (i < ns) bu_thread<W>(beg, tadj, prev, hs, cur, lam, q.q[Q_BU_S][i]);
+    if (i < ns) bu_thread<W>(beg, tadj, prev, hs, cur, flags, lam, q.q[Q_BU_S][i]);
   } else if (b < nbs + nbm) {
     const int64_t i = (b - nbs) * 4 + wv;
-    if (i < nm) bu_wave<W>(beg, tadj, prev, hs, cur, lam, q.q[Q_BU_M][i], lane);
+    if (i < nm) bu_wave<W>(beg, tadj, prev, hs, cur, flags, lam, q.q[Q_BU_M][i], lane);
   } else {
     const int64_t i = b - nbs - nbm;
-    if (i < nl) bu_block<W>(beg, tadj, prev, hs, cur, lam, q.q[Q_BU_L][i], part);
+    if (i < nl) bu_block<W>(beg, tadj, prev, hs, cur, flags, lam, q.q[Q_BU_L][i], part);
   }
 }
 
@@ -464,7 +537,7 @@ template <int W>
 __global__ __launch_bounds__(BS) void k_expand_dd(const int64_t *__restrict__ beg,
                                                   const int32_t *__restrict__ adj,
                                                   const int32_t *__restrict__ tadj, u64 *ring,
-                                                  int R, int64_t V,
+                                                  uint8_t *flags, int R, int64_t V,
                                                   const LevelCtl *__restrict__ ctl_ring,
                                                   const Plan *__restrict__ plan,
                                                   const u64 *__restrict__ hs, Queues q, int cs) {
@@ -484,26 +557,26 @@ __global__ __launch_bounds__(BS) void k_expand_dd(const int64_t *__restrict__ be
     const int64_t ns = c->q[Q_TD_S], nm = c->q[Q_TD_M], nl = c->q[Q_TD_L];
     for (int64_t i = (int64_t)blockIdx.x * BS + t; i < ns; i += G * BS) {
       const int32_t u = q.q[Q_TD_S][i];
-      td_push<W>(beg, adj, prev, cur, lam, u, beg[u], beg[u + 1], 1);
+      td_push<W>(beg, adj, prev, cur, flags, lam, u, beg[u], beg[u + 1], 1);
     }
     for (int64_t i = (int64_t)blockIdx.x * 4 + wv; i < nm; i += G * 4) {
       const int32_t u = q.q[Q_TD_M][i];
-      td_push<W>(beg, adj, prev, cur, lam, u, beg[u] + lane, beg[u + 1], 64);
+      td_push<W>(beg, adj, prev, cur, flags, lam, u, beg[u] + lane, beg[u + 1], 64);
     }
     for (int64_t i = blockIdx.x; i < nl; i += G) {
       const int2 e = q.chunks[i];
       int64_t lo, hi;
       td_chunk_range(beg, e, cs, lo, hi);
-      td_push<W>(beg, adj, prev, cur, lam, e.x, lo + t, hi, BS);
+      td_push<W>(beg, adj, prev, cur, flags, lam, e.x, lo + t, hi, BS);
     }
   } else {
     const int64_t ns = c->q[Q_BU_S], nm = c->q[Q_BU_M], nl = c->q[Q_BU_L];
     for (int64_t i = (int64_t)blockIdx.x * BS + t; i < ns; i += G * BS)
-      bu_thread<W>(beg, tadj, prev, hs, cur, lam, q.q[Q_BU_S][i]);
+      bu_thread<W>(beg, tadj, prev, hs, cur, flags, lam, q.q[Q_BU_S][i]);
     for (int64_t i = (int64_t)blockIdx.x * 4 + wv; i < nm; i += G * 4)
-      bu_wave<W>(beg, tadj, prev, hs, cur, lam, q.q[Q_BU_M][i], lane);
+      bu_wave<W>(beg, tadj, prev, hs, cur, flags, lam, q.q[Q_BU_M][i], lane);
     for (int64_t i = blockIdx.x; i < nl; i += G)
-      bu_block<W>(beg, tadj, prev, hs, cur, lam, q.q[Q_BU_L][i], part);
+      bu_block<W>(beg, tadj, prev, hs, cur, flags, lam, q.q[Q_BU_L][i], part);
   }
 }
 
@@ -577,31 +650,49 @@ hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t value, hipStream_t st)
 }
 
 hipError_t launch_init_sources(int W, const int32_t *src, int k, int lane0, u64 *S0,
-                               hipStream_t st) {
+                               uint8_t *flags, hipStream_t st) {
   if (k <= 0) return hipSuccess;
   dim3 g(grid_for(k, BS, 1 << 20)), blk(BS);
-  IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_init_sources<WW>, g, blk, 0, st, src, k, lane0, S0));
+  IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_init_sources<WW>, g, blk, 0, st, src, k, lane0, S0,
+                                        flags));
   return hipGetLastError();
 }
 
 hipError_t launch_admit(int W, const int32_t *src, const int32_t *lane, int n, u64 *S,
-                        hipStream_t st) {
+                        uint8_t *flags, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   dim3 g(grid_for(n, BS, 1 << 20)), blk(BS);
-  IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_admit<WW>, g, blk, 0, st, src, lane, n, S));
+  IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_admit<WW>, g, blk, 0, st, src, lane, n, S, flags));
   return hipGetLastError();
+}
+
+// Vertices per lane and round set: 8 (4 for 256-lane words, whose
+// registers would otherwise limit occupancy) when that still gives 1024
+// tiles of 512 threads, else 4, else 1; a.vpt overrides.
+int inspect_vpt(const InspectArgs &a, int W) {
+  if (a.vpt == 16) return W == 1 ? 16 : 8;  // wider words would spill registers
+  if (a.vpt == 1 || a.vpt == 4 || a.vpt == 8) return a.vpt;
+  for (int v : {8, 4})
+    if (v * W <= 16 && a.V >= (int64_t)512 * v * 1024) return v;
+  return 1;
+}
+
+template <int W, int VPT>
+static void inspect_launch(const InspectArgs &a, hipStream_t st) {
+  constexpr int NW = 8;
+  const int64_t tiles = (a.V + 64 * NW * VPT - 1) / (64 * NW * VPT);
+  hipLaunchKernelGGL((k_inspect<W, VPT, NW>), dim3(grid_for(tiles, 1, 8192)), dim3(64 * NW), 0,
+                     st, a);
 }
 
 template <int W>
 static void inspect_w(const InspectArgs &a, hipStream_t st) {
-  // vertices per thread: 16 when that still gives >= 1024 tiles, else 4, else 1
-  const int64_t V = a.V;
-  if (V >= (int64_t)BS * 16 * 1024)
-    hipLaunchKernelGGL((k_inspect<W, 16>), dim3(grid_for(V, BS * 16, 4096)), dim3(BS), 0, st, a);
-  else if (V >= (int64_t)BS * 4 * 512)
-    hipLaunchKernelGGL((k_inspect<W, 4>), dim3(grid_for(V, BS * 4, 4096)), dim3(BS), 0, st, a);
-  else
-    hipLaunchKernelGGL((k_inspect<W, 1>), dim3(grid_for(V, BS, 4096)), dim3(BS), 0, st, a);
+  switch (inspect_vpt(a, W)) {
+    case 16: inspect_launch<W, 16>(a, st); break;
+    case 8: inspect_launch<W, 8>(a, st); break;
+    case 4: inspect_launch<W, 4>(a, st); break;
+    default: inspect_launch<W, 1>(a, st); break;
+  }
 }
 
 hipError_t launch_inspect(int W, const InspectArgs &a, hipStream_t st) {
@@ -611,38 +702,38 @@ hipError_t launch_inspect(int W, const InspectArgs &a, hipStream_t st) {
 }
 
 hipError_t launch_expand_td(int W, const int64_t *beg, const int32_t *adj, const u64 *prev,
-                            u64 *cur, const u64 *lam_prev, Queues q, int64_t ns, int64_t nm,
-                            int64_t nl, int chunk_shift, hipStream_t st) {
+                            u64 *cur, uint8_t *flags, const u64 *lam_prev, Queues q, int64_t ns,
+                            int64_t nm, int64_t nl, int chunk_shift, hipStream_t st) {
   const int64_t nbs = (ns + BS - 1) / BS, nbm = (nm + 3) / 4;
   const int64_t nb = nbs + nbm + nl;
   if (nb == 0) return hipSuccess;
   if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
   dim3 g((unsigned)nb), blk(BS);
   IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_expand_td<WW>, g, blk, 0, st, beg, adj, prev, cur,
-                                        lam_prev, q, ns, nm, nl, nbs, nbm, chunk_shift));
+                                        flags, lam_prev, q, ns, nm, nl, nbs, nbm, chunk_shift));
   return hipGetLastError();
 }
 
 hipError_t launch_expand_bu(int W, const int64_t *beg, const int32_t *tadj, const u64 *prev,
-                            const u64 *hs, u64 *cur, const u64 *lam_prev, Queues q, int64_t ns,
-                            int64_t nm, int64_t nl, hipStream_t st) {
+                            const u64 *hs, u64 *cur, uint8_t *flags, const u64 *lam_prev,
+                            Queues q, int64_t ns, int64_t nm, int64_t nl, hipStream_t st) {
   const int64_t nbs = (ns + BS - 1) / BS, nbm = (nm + 3) / 4;
   const int64_t nb = nbs + nbm + nl;
   if (nb == 0) return hipSuccess;
   if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
   dim3 g((unsigned)nb), blk(BS);
   IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_expand_bu<WW>, g, blk, 0, st, beg, tadj, prev, hs, cur,
-                                        lam_prev, q, ns, nm, nl, nbs, nbm));
+                                        flags, lam_prev, q, ns, nm, nl, nbs, nbm));
   return hipGetLastError();
 }
 
 hipError_t launch_expand_dd(int W, const int64_t *beg, const int32_t *adj, const int32_t *tadj,
-                            u64 *ring, int R, int64_t V, const LevelCtl *ctl_ring,
+                            u64 *ring, uint8_t *flags, int R, int64_t V, const LevelCtl *ctl_ring,
                             const Plan *plan, const u64 *hs, Queues q, int chunk_shift,
                             int grid, hipStream_t st) {
   dim3 g(grid > 0 ? grid : 1), blk(BS);
-  IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_expand_dd<WW>, g, blk, 0, st, beg, adj, tadj, ring, R,
-                                        V, ctl_ring, plan, hs, q, chunk_shift));
+  IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_expand_dd<WW>, g, blk, 0, st, beg, adj, tadj, ring,
+                                        flags, R, V, ctl_ring, plan, hs, q, chunk_shift));
   return hipGetLastError();
 }
 

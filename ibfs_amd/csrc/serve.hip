@@ -75,6 +75,7 @@ struct SCtx {
   LevelCtl *h_ctl = nullptr;
   Queues q{};
   u64 *hs = nullptr;
+  uint8_t *flags = nullptr;  // changed-at-this-level byte per traversal vertex
   u64 *d_mask = nullptr, *h_mask = nullptr;
   int32_t *d_adm = nullptr, *h_adm = nullptr;  // [0,K) sources, [K,2K) lanes
   LaneOut *d_lanes = nullptr, *h_lanes = nullptr;
@@ -124,7 +125,7 @@ size_t emit_words(const ibfs_server_s *s) { return (size_t)(s->R + 2) * s->W; }
 size_t mask_words(const ibfs_server_s *s) { return 2 * emit_words(s) + 3 * (size_t)s->W; }
 
 u64 *slot(const ibfs_server_s *s, const SCtx &c, int L) {
-  return c.ring + (size_t)(L % s->R) * (size_t)s->g->V * (size_t)s->W;
+  return c.ring + (size_t)(L % s->R) * (size_t)s->g->Vt * (size_t)s->W;
 }
 
 void release_ctx(SCtx &c) {
@@ -135,6 +136,7 @@ void release_ctx(SCtx &c) {
   for (int i = 0; i < 6; ++i) dfree(c.q.q[i]);
   dfree(c.q.chunks);
   dfree(c.hs);
+  dfree(c.flags);
   dfree(c.d_mask);
   hfree(c.h_mask);
   dfree(c.d_adm);
@@ -148,7 +150,7 @@ void release_ctx(SCtx &c) {
 }
 
 int alloc_ctx(ibfs_server_s *s, SCtx &c) {
-  const int64_t V = s->g->V;
+  const int64_t V = std::max<int64_t>(s->g->Vt, 1);  // status words and queues: traversal ids
   hipError_t err = hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking);
   auto A = [&](void **p, size_t bytes) {
     if (err == hipSuccess) err = hipMalloc(p, bytes ? bytes : 1);
@@ -163,6 +165,7 @@ int alloc_ctx(ibfs_server_s *s, SCtx &c) {
     if (i != Q_TD_L) A((void **)&c.q.q[i], sizeof(int32_t) * (size_t)V);
   A((void **)&c.q.chunks, sizeof(int2) * ((size_t)V + (size_t)(s->g->E >> s->chunk_shift) + 2));
   A((void **)&c.hs, sizeof(u64) * (size_t)s->W * (size_t)std::max<int64_t>(s->g->H, 1));
+  A((void **)&c.flags, sizeof(u64) * (size_t)((V + 7) / 8));
   A((void **)&c.d_mask, sizeof(u64) * mask_words(s));
   H((void **)&c.h_mask, sizeof(u64) * mask_words(s));
   A((void **)&c.d_adm, sizeof(int32_t) * 2 * (size_t)s->K);
@@ -193,8 +196,9 @@ int emit_lanes(ibfs_server_s *s, SCtx &c, const std::vector<int> &lanes, int b, 
     for (int k : lanes)
       if (c.lanes[k].admit <= L) set_bit(h + W + (size_t)(L - b) * W, k);
   HIP_CHK(hipMemcpyAsync(d, h, sizeof(u64) * emit_words(s), hipMemcpyHostToDevice, c.st));
-  HIP_CHK(launch_emit(W, s->cfg.emit_mode, 0, c.ring, s->R, s->g->V, b, e, c.d_lanes, d + W, d,
-                      s->K, c.st));
+  const EmitSpace sp{s->g->d_cid, s->g->V, s->g->Vt};
+  HIP_CHK(launch_emit(W, s->cfg.emit_mode, 0, c.ring, s->R, sp, b, e, c.d_lanes, d + W, d, s->K,
+                      c.st));
   return IBFS_OK;
 }
 
@@ -210,10 +214,11 @@ int admit(ibfs_server_s *s, SCtx &c, int L, u64 *A, int *n_adm) {
     c.lanes[k].state = ACTIVE;
     c.lanes[k].qid = p.qid;
     c.lanes[k].admit = L;
-    c.h_adm[n] = q.src[p.idx];
+    c.h_adm[n] = s->g->h_cid[q.src[p.idx]];
     c.h_adm[s->K + n] = k;
     c.h_lanes[k].out = q.out + (size_t)p.idx * (size_t)s->g->V;
     c.h_lanes[k].admit = L;
+    c.h_lanes[k].src = q.src[p.idx];
     if (q.t_admit < 0) q.t_admit = now_ms();
     set_bit(A, k);
     ++n;
@@ -225,7 +230,7 @@ int admit(ibfs_server_s *s, SCtx &c, int L, u64 *A, int *n_adm) {
                          hipMemcpyHostToDevice, c.st));
   HIP_CHK(hipMemcpyAsync(c.d_lanes, c.h_lanes, sizeof(LaneOut) * (size_t)s->K,
                          hipMemcpyHostToDevice, c.st));
-  HIP_CHK(launch_admit(s->W, c.d_adm, c.d_adm + s->K, n, slot(s, c, L), c.st));
+  HIP_CHK(launch_admit(s->W, c.d_adm, c.d_adm + s->K, n, slot(s, c, L), c.flags, c.st));
   return IBFS_OK;
 }
 
@@ -251,7 +256,7 @@ int inspect_level(ibfs_server_s *s, SCtx &c, int L, const u64 *filter, const u64
   a.ctl = c.ctl + L % s->R;
   a.ctl_zero = c.ctl + (L + 1) % s->R;
   a.deg = s->g->d_deg;
-  a.V = s->g->V;
+  a.V = s->g->Vt;
   a.q = c.q;
   a.sml = s->cfg.sml_shed;
   a.lrg = s->cfg.lrg_shed;
@@ -259,6 +264,8 @@ int inspect_level(ibfs_server_s *s, SCtx &c, int L, const u64 *filter, const u64
   a.hubs = s->g->d_hubs;
   a.H = s->g->H;
   a.hs = c.hs;
+  a.flags = c.flags;
+  a.vpt = s->cfg.inspect_vpt;
   HIP_CHK(launch_inspect(W, a, c.st));
   HIP_CHK(hipMemcpyAsync(c.h_ctl, c.ctl + L % s->R, sizeof(LevelCtl), hipMemcpyDeviceToHost, c.st));
   HIP_CHK(hipEventRecord(c.ev, c.st));
@@ -273,7 +280,8 @@ int start_ctx(ibfs_server_s *s, SCtx &c) {
   c.base = 0;
   c.dir_prev = -1;
   c.nf_prev = ~0ull;
-  HIP_CHK(launch_fill_u64(slot(s, c, 0), s->g->V * W, 0ull, c.st));
+  HIP_CHK(launch_fill_u64(slot(s, c, 0), s->g->Vt * W, 0ull, c.st));
+  HIP_CHK(launch_fill_u64(reinterpret_cast<u64 *>(c.flags), (s->g->Vt + 7) / 8, 0ull, c.st));
   HIP_CHK(launch_fill_u64(reinterpret_cast<u64 *>(c.ctl), (int64_t)s->R * 16, 0ull, c.st));
   std::vector<u64> A(W, 0), keep(W, ~0ull), nand(W, ~0ull);
   int n = 0;
@@ -356,11 +364,12 @@ int process_level(ibfs_server_s *s, SCtx &c) {
   const int dir = decide(s, Ln, lc, c.dir_prev, c.nf_prev);
   const u64 *lamp = reinterpret_cast<const u64 *>(c.ctl + L % s->R);
   if (dir == 0)
-    HIP_CHK(launch_expand_td(W, s->g->d_beg, s->g->d_adj, slot(s, c, L), slot(s, c, Ln), lamp,
-                             c.q, lc.q[Q_TD_S], lc.q[Q_TD_M], lc.q[Q_TD_L], s->chunk_shift, c.st));
+    HIP_CHK(launch_expand_td(W, s->g->d_beg, s->g->d_adj, slot(s, c, L), slot(s, c, Ln), c.flags,
+                             lamp, c.q, lc.q[Q_TD_S], lc.q[Q_TD_M], lc.q[Q_TD_L], s->chunk_shift,
+                             c.st));
   else
     HIP_CHK(launch_expand_bu(W, s->g->d_beg, s->g->d_tadj, slot(s, c, L), c.hs, slot(s, c, Ln),
-                             lamp, c.q, lc.q[Q_BU_S], lc.q[Q_BU_M], lc.q[Q_BU_L], c.st));
+                             c.flags, lamp, c.q, lc.q[Q_BU_S], lc.q[Q_BU_M], lc.q[Q_BU_L], c.st));
   c.dir_prev = dir;
   c.nf_prev = lc.n_front;
   // 3. the next copy (slot Ln+1) would overwrite the window base: flush all
@@ -453,14 +462,16 @@ int ibfs_server_create(ibfs_graph_t g, const ibfs_config *cfg_in, int64_t status
   if (cfg.td_chunk < 256 || (cfg.td_chunk & (cfg.td_chunk - 1)))
     return fail(IBFS_ERR_ARG, "td_chunk must be a power of two >= 256");
   if (cfg.emit_mode < 0 || cfg.emit_mode > 2) return fail(IBFS_ERR_ARG, "bad emit_mode");
+  if (!valid_inspect_vpt(cfg.inspect_vpt)) return fail(IBFS_ERR_ARG, "bad inspect_vpt");
   if (cfg.device != g->device) return fail(IBFS_ERR_ARG, "config device differs from the graph's");
   int nd = 0;
   if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0)
     return fail(IBFS_ERR_NODEVICE, "no HIP device available");
   HIP_CHK(hipSetDevice(g->device));
   // admission capacity from the status-array budget: one context holds
-  // lane_bits lanes in ring_levels snapshots of V * lane_bits / 8 bytes
-  const size_t per_ctx = (size_t)cfg.ring_levels * (size_t)g->V * (size_t)(cfg.lane_bits / 8);
+  // lane_bits lanes in ring_levels snapshots of Vt * lane_bits / 8 bytes
+  const size_t per_ctx =
+      (size_t)cfg.ring_levels * (size_t)std::max<int64_t>(g->Vt, 1) * (size_t)(cfg.lane_bits / 8);
   int C = status_bytes > 0 ? (int)((size_t)status_bytes / per_ctx) : 1;
   if (C < 1)
     return fail(IBFS_ERR_NOMEM, "status budget %lld below one context (%zu bytes)",

@@ -68,7 +68,8 @@ typedef struct ibfs_config {
   int32_t emit_mode;   /* depth emission: 0 per-thread stores, 1 LDS-tiled,
                           2 per-thread non-temporal stores                   */
   int32_t dd_grid;     /* workgroups of the device-driven expander (0: auto)  */
-  int32_t reserved[1];
+  int32_t inspect_vpt; /* inspector vertices per lane and tile: 1, 4, 8 or 16
+                          (16 only for 64 lanes, else 8; 0: by vertex count) */
 } ibfs_config;
 
 /* Per-level record of the last traversal (the build's monitoring log, in place
@@ -111,6 +112,9 @@ int ibfs_device_alloc(int device, size_t bytes, void **dptr);
 int ibfs_device_free(void *dptr);
 int ibfs_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int ibfs_memcpy_h2d(void *dst, const void *src, size_t bytes);
+/* Device-to-device copy (synchronous): moving depth arrays between device
+ * buffers, and the plain copy of known size that scales HBM counters. */
+int ibfs_memcpy_d2d(void *dst, const void *src, size_t bytes);
 
 /* ---- host CSR: reference file format, generators, converter -------------- */
 /* Reads <beg_pos.bin> (V+1 little-endian 8-byte integers) and <csr.bin>
@@ -165,8 +169,10 @@ typedef struct ibfs_graph_opts {
 int ibfs_graph_opts_default(ibfs_graph_opts *opts);
 /* Uploads a CSR to the device (int64 beg_pos, int32 adjacency, int32 degree,
  * plus a traversal copy of the adjacency ordered by descending neighbour
- * degree).  Replaces the graph upload in the gpu_ibfs constructor
- * (gpu_ibfs.cu:54-116). */
+ * degree).  The device copy keeps only the vertices with an edge (in or
+ * out), renumbered in their original order; isolated vertices get no status
+ * words and their depths come from the source list alone.  Replaces the
+ * graph upload in the gpu_ibfs constructor (gpu_ibfs.cu:54-116). */
 int ibfs_graph_create(const int64_t *beg_pos, int64_t vert_count,
                       const int32_t *adj, int64_t edge_count, int device,
                       ibfs_graph_t *out);
@@ -177,6 +183,9 @@ int ibfs_graph_from_host(ibfs_hostcsr_t h, int device, ibfs_graph_t *out);
 int ibfs_graph_load_files(const char *beg_path, const char *csr_path,
                           int device, ibfs_graph_t *out);
 int ibfs_graph_info(ibfs_graph_t g, int64_t *vert_count, int64_t *edge_count);
+/* Device layout sizes: vertices with status words (an edge in or out) and
+ * hub-cache entries.  No reference counterpart (layout introspection). */
+int ibfs_graph_layout(ibfs_graph_t g, int64_t *traversal_vertices, int64_t *hubs);
 int ibfs_graph_destroy(ibfs_graph_t g);
 
 /* GroupBy on the device: a permutation of src_list in which sources of degree

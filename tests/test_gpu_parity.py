@@ -80,6 +80,65 @@ def test_partial_group_duplicates_and_isolated_sources():
         np.testing.assert_array_equal(got, z["depth"][src])
 
 
+def _directed_with_sinks(V=400, m=1500, seed=4):
+    """Random directed CSR where 1/4 of the vertices have no out-edges (some
+    of them reachable sinks) and vertices V-10..V-1 have no edges at all."""
+    rng = np.random.default_rng(seed)
+    a = rng.integers(0, V - 10, size=m)
+    b = rng.integers(0, V - 10, size=m)
+    keep = (a % 4 != 0) & (a != b)
+    a, b = a[keep], b[keep]
+    key = np.unique(a.astype(np.int64) * V + b)
+    a, b = (key // V).astype(np.int64), (key % V).astype(np.int32)
+    beg = np.zeros(V + 1, dtype=np.int64)
+    np.add.at(beg, a + 1, 1)
+    return np.cumsum(beg), b
+
+
+def test_directed_sinks_and_isolated_top_down():
+    """Top-down only (the direction that is defined on a directed CSR):
+    sinks (out-degree 0, reachable) and isolated vertices keep exact depths
+    in every runner."""
+    say("[gpu] directed sinks / isolated")
+    beg, adj = _directed_with_sinks()
+    V = len(beg) - 1
+    deg = np.diff(beg)
+    indeg = np.bincount(adj, minlength=V)
+    assert ((deg == 0) & (indeg > 0)).any() and ((deg == 0) & (indeg == 0)).any()
+    g = ibfs_amd.DeviceGraph(ibfs_amd.HostCsr.from_arrays(beg, adj))
+    assert g.traversal_vertices == int(((deg > 0) | (indeg > 0)).sum())
+    src = np.arange(V, dtype=np.int32)
+    want = oracle.bfs_many(beg, adj, src, threads=8)
+    td = POLICIES["td"]
+    np.testing.assert_array_equal(ibfs_amd.Engine(g, **td).run_groups(src), want)
+    b = ibfs_amd.Batch(g, contexts=2, burst=3, ring_levels=5, **td)
+    np.testing.assert_array_equal(b.run(src), want)
+    dsum, reach = b.closeness(src)
+    want_sum, want_reach = _close_ref(want)
+    np.testing.assert_array_equal(reach, want_reach)
+    np.testing.assert_array_equal(dsum, want_sum)
+    srv = ibfs_amd.Server(g, lane_bits=64, ring_levels=8, **td)
+    queries = [src[i:i + n] for i, n in ((390, 10), (0, 64), (100, 3), (395, 5), (200, 150))]
+    outs, _ = _serve(srv, queries, V, gap_s=0.001)
+    for q, o in zip(queries, outs):
+        np.testing.assert_array_equal(o, want[q])
+    srv.close()
+
+
+def test_server_isolated_sources():
+    say("[gpu] server isolated sources")
+    z = load_golden("bfs_twocomp.npz")
+    g = ibfs_amd.DeviceGraph(ibfs_amd.HostCsr.from_arrays(z["beg_pos"], z["adj"]))
+    V = len(z["beg_pos"]) - 1
+    assert g.traversal_vertices == V - 1  # vertex 7 has no edge
+    queries = [np.array(q, dtype=np.int32) for q in ([7], [7, 7, 0], [3, 7, 6, 1], [7] * 70)]
+    srv = ibfs_amd.Server(g, lane_bits=64, ring_levels=8)
+    outs, _ = _serve(srv, queries, V, gap_s=0.001)
+    for q, o in zip(queries, outs):
+        np.testing.assert_array_equal(o, z["depth"][q])
+    srv.close()
+
+
 def test_lane_position_invariance():
     say("[gpu] lane position")
     z = load_golden("bfs_grid6x7.npz")
@@ -120,6 +179,20 @@ def test_hub_cache_and_emission_variants(hubs, emit_mode):
                 src = np.arange(0, V, max(1, V // 200), dtype=np.int32)
                 np.testing.assert_array_equal(eng.run_groups(src), z["depth"][src],
                                               err_msg=f"{name} {pol} ring={ring}")
+
+
+@pytest.mark.parametrize("vpt", [1, 4, 8, 16])
+def test_inspector_chunking(s16, vpt):
+    say(f"[gpu] inspect_vpt={vpt}")
+    for lb in (64, 256):
+        eng = ibfs_amd.Engine(s16["g"], lane_bits=lb, inspect_vpt=vpt)
+        np.testing.assert_array_equal(eng.run_groups(s16["src"]), s16["want"])
+    z = load_golden("bfs_classes1500.npz")
+    g = ibfs_amd.DeviceGraph(ibfs_amd.HostCsr.from_arrays(z["beg_pos"], z["adj"]))
+    V = len(z["beg_pos"]) - 1
+    src = np.arange(V, dtype=np.int32)
+    b = ibfs_amd.Batch(g, contexts=2, burst=4, inspect_vpt=vpt)
+    np.testing.assert_array_equal(b.run(src), z["depth"])
 
 
 @pytest.mark.parametrize("hubs", [0, 64, None])
