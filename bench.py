@@ -147,11 +147,12 @@ def edges_of(g, sources, chunk=256):
     return total
 
 
-def served_leg(graphs, rates, n_queries, seed, K):
+def served_leg(graphs, rates, n_queries, seed, K, admit_window=None):
     """Poisson arrivals at each rate; every query names one graph and carries
     a log-uniform 1..256 source set.  Latency = done - submit."""
     rng = np.random.default_rng(seed)
-    servers = [ibfs_amd.Server(gr["g"], lane_bits=K, status_bytes=gr["status_bytes"]) for gr in graphs]
+    servers = [ibfs_amd.Server(gr["g"], lane_bits=K, status_bytes=gr["status_bytes"],
+                               admit_window=admit_window) for gr in graphs]
     results = []
     for lam in rates:
         gi = rng.integers(0, len(graphs), size=n_queries)
@@ -206,6 +207,7 @@ def served_leg(graphs, rates, n_queries, seed, K):
         log(f"served lambda={lam}/s: p50 {results[-1]['p50_ms']} ms p99 {results[-1]['p99_ms']} ms "
             f"{results[-1]['gteps']} GTEPS over {results[-1]['span_ms']} ms")
     info = {"graphs": [gr["name"] for gr in graphs], "lane_bits": K,
+            "admit_window": 0 if admit_window is None else admit_window,
             "contexts": [s.contexts for s in servers], "rates": results}
     for s in servers:
         s.close()
@@ -233,7 +235,8 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-queries", type=int, default=120)
-    ap.add_argument("--serve-rates", default="200,800")
+    ap.add_argument("--serve-rates", default="200,800,2000")
+    ap.add_argument("--admit-window", type=int, default=None)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device 4 times (HBM counter yardstick)")
     args = ap.parse_args()
@@ -339,7 +342,8 @@ def main() -> int:
              "status_bytes": 2 * 32 * hc.vert_count * 32},
         ]
         rates = [float(x) for x in args.serve_rates.split(",") if x]
-        served = served_leg(graphs, rates, args.serve_queries, seed=3, K=256)
+        served = served_leg(graphs, rates, args.serve_queries, seed=3, K=256,
+                            admit_window=args.admit_window)
 
     res = {
         "metric": METRIC,

@@ -390,13 +390,15 @@ class Server:
     are written to the given device buffer ([n][V] int32)."""
 
     def __init__(self, g: DeviceGraph, status_bytes: int = 0, cfg: Optional[Config] = None,
-                 **overrides):
+                 admit_window: Optional[int] = None, **overrides):
         if cfg is None:
             overrides.setdefault("lane_bits", 256)
             cfg = default_config(device=g.device, **overrides)
         h = C.c_void_p()
         check(lib().ibfs_server_create(g._h, C.byref(cfg), status_bytes, C.byref(h)))
         self._h = h
+        if admit_window is not None:
+            check(lib().ibfs_server_set_admission(h, admit_window))
         self.graph = g
         self.cfg = cfg
         c, l = C.c_int32(), C.c_int32()

@@ -130,6 +130,7 @@ SIGNATURES = {
     "ibfs_batch_closeness": (C.c_int, [P, I32P, C.c_int64, I64P, I64P, C.POINTER(Stats)]),
     "ibfs_server_create": (C.c_int, [P, C.POINTER(Config), C.c_int64, C.POINTER(P)]),
     "ibfs_server_contexts": (C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "ibfs_server_set_admission": (C.c_int, [P, C.c_int32]),
     "ibfs_server_submit": (C.c_int, [P, I32P, C.c_int32, P, C.POINTER(C.c_int64)]),
     "ibfs_server_wait": (C.c_int, [P, C.c_int64, C.c_double, C.POINTER(QueryInfo)]),
     "ibfs_server_release": (C.c_int, [P, C.c_int64]),

@@ -667,13 +667,13 @@ hipError_t launch_admit(int W, const int32_t *src, const int32_t *lane, int n, u
 }
 
 // Vertices per lane and round set: 8 (4 for 256-lane words, whose
-// registers would otherwise limit occupancy) when that still gives 1024
+// registers would otherwise limit occupancy) when that still gives 512
 // tiles of 512 threads, else 4, else 1; a.vpt overrides.
 int inspect_vpt(const InspectArgs &a, int W) {
   if (a.vpt == 16) return W == 1 ? 16 : 8;  // wider words would spill registers
   if (a.vpt == 1 || a.vpt == 4 || a.vpt == 8) return a.vpt;
   for (int v : {8, 4})
-    if (v * W <= 16 && a.V >= (int64_t)512 * v * 1024) return v;
+    if (v * W <= 16 && a.V >= (int64_t)512 * v * 512) return v;
   return 1;
 }
 

@@ -10,6 +10,12 @@
 // the next snapshot.  Lanes are independent bits, so a query's depths do not
 // depend on what shares the context with it.  Contexts are multiplexed with
 // events, so one context's level boundary overlaps another's kernels.
+//
+// Admission is by cohort: a context takes waiting sources when all its lanes
+// have finished (a new cohort) and, within admit_window levels of that
+// start, into its remaining clean lanes.  A level's direction and cost are
+// shared by all lanes of a context, so lanes far apart in BFS phase would
+// make every level as expensive as the densest phase among them.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -83,6 +89,7 @@ struct SCtx {
   std::vector<Lane> lanes;
   bool running = false;
   int L = 0, base = 0, dir_prev = -1;
+  int cohort = 0;  // level at which the current cohort's first sources were admitted
   u64 nf_prev = ~0ull;
   std::vector<int64_t> done_pending;
 };
@@ -116,6 +123,9 @@ struct ibfs_server_s {
   std::deque<PendingSrc> incoming;  // submitted, not yet seen by the worker
   std::deque<PendingSrc> pending;   // worker-owned FIFO of sources to admit
   int64_t levels_issued = 0, lanes_admitted = 0;
+  // sources join a running context only within this many levels of its
+  // cohort's start, so that the lanes of a context stay close in phase
+  int admit_window = 0;
 };
 
 namespace {
@@ -278,6 +288,7 @@ int start_ctx(ibfs_server_s *s, SCtx &c) {
   const int W = s->W;
   for (auto &l : c.lanes) l = Lane{};
   c.base = 0;
+  c.cohort = 0;
   c.dir_prev = -1;
   c.nf_prev = ~0ull;
   HIP_CHK(launch_fill_u64(slot(s, c, 0), s->g->Vt * W, 0ull, c.st));
@@ -325,17 +336,20 @@ int process_level(ibfs_server_s *s, SCtx &c) {
     std::lock_guard<std::mutex> lk(s->mu);
     mark_done(s, c);  // emissions issued before this level's event are complete
   }
-  // 1. lanes whose BFS ended: emit [base, L-1], free them
+  // 1. lanes whose BFS ended: emit them from the earliest admission among
+  //    them (the window base if one is older), free them
   std::vector<int> fin;
   std::vector<u64> finmask(W, 0);
+  int fin_from = L;
   for (int k = 0; k < s->K; ++k) {
     Lane &l = c.lanes[k];
     if (l.state == ACTIVE && l.admit < L && !has_bit(lc.lam, k)) {
       fin.push_back(k);
       set_bit(finmask.data(), k);
+      fin_from = std::min(fin_from, l.admit);
     }
   }
-  int rc = emit_lanes(s, c, fin, c.base, L - 1, 0);
+  int rc = emit_lanes(s, c, fin, std::max(c.base, fin_from), L - 1, 0);
   if (rc) return rc;
   bool any_active = false;
   {
@@ -372,8 +386,16 @@ int process_level(ibfs_server_s *s, SCtx &c) {
                              c.flags, lamp, c.q, lc.q[Q_BU_S], lc.q[Q_BU_M], lc.q[Q_BU_L], c.st));
   c.dir_prev = dir;
   c.nf_prev = lc.n_front;
-  // 3. the next copy (slot Ln+1) would overwrite the window base: flush all
-  //    active lanes over [base, Ln]
+  // 3. snapshots older than every active lane's admission are no longer
+  //    needed: the window base follows the oldest active lane.  If the next
+  //    copy (slot Ln+1) would still overwrite the base, flush all active
+  //    lanes over [base, Ln].
+  {
+    int oldest = Ln;
+    for (int k = 0; k < s->K; ++k)
+      if (c.lanes[k].state == ACTIVE) oldest = std::min(oldest, c.lanes[k].admit);
+    c.base = std::max(c.base, oldest);
+  }
   if ((Ln + 1) - c.base >= s->R) {
     std::vector<int> act;
     for (int k = 0; k < s->K; ++k)
@@ -382,12 +404,17 @@ int process_level(ibfs_server_s *s, SCtx &c) {
     if (rc) return rc;
     c.base = Ln;
   }
-  // 4. admission at Ln
+  // 4. admission at Ln: into a context whose lanes have all finished (a new
+  //    cohort starts), or while the current cohort is young enough
   std::vector<u64> A(W, 0), filter(W), keep(W), nand(W);
   int n_adm = 0;
   {
     std::lock_guard<std::mutex> lk(s->mu);
-    rc = admit(s, c, Ln, A.data(), &n_adm);
+    const bool fresh = !any_active;
+    if (fresh || Ln - c.cohort <= s->admit_window) {
+      rc = admit(s, c, Ln, A.data(), &n_adm);
+      if (fresh && n_adm > 0) c.cohort = Ln;
+    }
   }
   if (rc) return rc;
   for (int w = 0; w < W; ++w) {
@@ -496,6 +523,14 @@ int ibfs_server_create(ibfs_graph_t g, const ibfs_config *cfg_in, int64_t status
   }
   s->worker = std::thread(worker_main, s);
   *out = s;
+  return IBFS_OK;
+}
+
+int ibfs_server_set_admission(ibfs_server_t s, int32_t window_levels) {
+  if (!s) return fail(IBFS_ERR_ARG, "null server");
+  if (window_levels < 0) return fail(IBFS_ERR_ARG, "window_levels must be >= 0");
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->admit_window = window_levels;
   return IBFS_OK;
 }
 

@@ -256,6 +256,14 @@ typedef struct ibfs_query_info {
 } ibfs_query_info;
 int ibfs_server_create(ibfs_graph_t g, const ibfs_config *cfg, int64_t status_bytes,
                        ibfs_server_t *out);
+/* Admission policy: a source joins a running context only within
+ * window_levels levels of the start of that context's current cohort (a
+ * context whose lanes have all finished starts a new cohort with whatever is
+ * waiting).  Lanes of one context then stay close in BFS phase, so a level's
+ * direction and cost suit all of them.  0 admits only at cohort starts; a
+ * large value admits at every level boundary.  Default 0.  No reference
+ * counterpart (serving). */
+int ibfs_server_set_admission(ibfs_server_t s, int32_t window_levels);
 int ibfs_server_contexts(ibfs_server_t s, int32_t *contexts, int32_t *lanes);
 int ibfs_server_submit(ibfs_server_t s, const int32_t *src, int32_t n,
                        int32_t *depth_out, int64_t *id);

@@ -305,15 +305,16 @@ def test_server_mixed_equals_alone_s16(s16, lane_bits):
     alone.close()
 
 
-@pytest.mark.parametrize("ring", [8, 9, 32])
-def test_server_ring_flush_and_recycling(ring):
-    say(f"[gpu] server ring={ring}")
+@pytest.mark.parametrize("ring,window", [(8, 1), (9, 0), (32, 1), (8, 1 << 20), (32, 3)])
+def test_server_ring_flush_and_recycling(ring, window):
+    say(f"[gpu] server ring={ring} admit_window={window}")
     z = load_golden("bfs_classes1500.npz")
     g = ibfs_amd.DeviceGraph(ibfs_amd.HostCsr.from_arrays(z["beg_pos"], z["adj"]))
     V = len(z["beg_pos"]) - 1
     rng = np.random.default_rng(ring)
     queries = [rng.integers(0, V, size=int(n)).astype(np.int32) for n in (1, 7, 64, 3, 100, 2, 40)]
-    srv = ibfs_amd.Server(g, lane_bits=64, ring_levels=ring)  # 64 lanes < 217 sources
+    # 64 lanes < 217 sources
+    srv = ibfs_amd.Server(g, lane_bits=64, ring_levels=ring, admit_window=window)
     outs, infos = _serve(srv, queries, V, gap_s=0.001)
     for q, o in zip(queries, outs):
         np.testing.assert_array_equal(o, z["depth"][q])

@@ -186,6 +186,7 @@ def test_c_abi_argument_checks():
     assert l.ibfs_groupby(None, None, 0, None) == _lib.IBFS_ERR_ARG
     assert l.ibfs_graph_info(None, None, None) == _lib.IBFS_ERR_ARG
     assert l.ibfs_graph_layout(None, None, None) == _lib.IBFS_ERR_ARG
+    assert l.ibfs_server_set_admission(None, 1) == _lib.IBFS_ERR_ARG
     assert "null" in _lib.last_error()
     assert l.ibfs_graph_destroy(None) == 0 and l.ibfs_engine_destroy(None) == 0
 
