@@ -122,6 +122,28 @@ def roofline_leg(g, src, K, V, tune, kern_ms, launches, steps):
     return roof, kernels, levels
 
 
+KERNEL_NAMES = {"expand_bu": "k_expand_bu", "expand_td": "k_expand_td", "inspect": "k_inspect",
+                "emit": "k_emit"}
+
+
+def measured_traffic(workload, K, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (scaled by the 1 GiB copy yardstick
+    recorded there), or None when no such summary exists."""
+    path = ROOT / "profiles" / f"hbm_traffic_rmat_{workload}_plain{K}.md"
+    if not path.exists():
+        return None, None
+    want = f"{KERNEL_NAMES[kernel]}<{K // 64}"
+    for line in path.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 7 and cells[0].startswith(want):
+            try:
+                return float(cells[4]), path.name
+            except ValueError:
+                return None, None
+    return None, None
+
+
 def batch_leg(g, src, K, contexts, burst, steps, warmup, tune, emit_mode, out):
     b = ibfs_amd.Batch(g, contexts=contexts, burst=burst, lane_bits=K, emit_mode=emit_mode, **tune)
     for _ in range(warmup):
@@ -292,6 +314,10 @@ def main() -> int:
     log(f"plain: {plain['ms_per_step']:.2f} ms/step, {plain['gteps']:.1f} GTEPS")
     roof, kernels, levels = roofline_leg(g, src, K, V, tune, plain["kern_ms"], plain["launches"],
                                          args.steps)
+    roof["traffic"], roof["traffic_source"] = measured_traffic(args.workload, K, roof["kernel"])
+    if roof["traffic"]:
+        roof["alg_bytes_per_launch"] = round(kernels[roof["kernel"]]["alg_GB_per_step"] * 1e9
+                                             / max(1, kernels[roof["kernel"]]["launches_per_step"]))
     log(f"dominant kernel {roof['kernel']}: {roof['achieved']:.0f} GB/s algorithmic")
 
     batched = None
