@@ -84,12 +84,18 @@ __device__ __forceinline__ int degree_class(int d, int sml, int lrg) {
   return d < sml ? 1 : (d > lrg ? 3 : 2);
 }
 
+// Sources of a bottom-up gather at level L+1: S_L of non-hub neighbours and
+// the hub cache (the hubs' words of S_L).
+struct Gather {
+  const u64 *__restrict__ prev;
+  const u64 *__restrict__ hs;
+};
+
 // Word of traversal-adjacency entry a: a >= 0 is a vertex id (read from the
 // snapshot), a < 0 is hub slot a & 0x7fffffff (read from the hub cache).
 template <int W>
-__device__ __forceinline__ void load_nbr(const u64 *__restrict__ prev, const u64 *__restrict__ hs,
-                                         int32_t a, u64 (&y)[W]) {
-  const u64 *p = (a < 0) ? hs + (int64_t)(a & 0x7fffffff) * W : prev + (int64_t)a * W;
+__device__ __forceinline__ void load_nbr(const Gather &gs, int32_t a, u64 (&y)[W]) {
+  const u64 *p = (a < 0) ? gs.hs + (int64_t)(a & 0x7fffffff) * W : gs.prev + (int64_t)a * W;
   load_word<W>(p, y);
 }
 

@@ -139,8 +139,9 @@ int decide(const ibfs_engine_s *e, int L, const LevelCtl &c, int dir_prev, u64 n
 // Host-driven inspector arguments (the pass also refreshes the hub cache
 // from cur for the next bottom-up level).
 InspectArgs inspect_args(ibfs_engine_s *e, const u64 *prev, const u64 *cur, u64 *next,
-                         const u64 *lam_prev, LevelCtl *ctl, LevelCtl *ctl_zero) {
+                         const u64 *lam_prev, LevelCtl *ctl, LevelCtl *ctl_zero, int full_diff) {
   InspectArgs a{};
+  a.full_diff = full_diff;
   a.prev = prev;
   a.cur = cur;
   a.next = next;
@@ -220,7 +221,7 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
   hipEvent_t ti0 = nullptr, ti1 = nullptr;
   if (timing) { ti0 = next_event(e); HIP_TRY(hipEventRecord(ti0, e->st)); }
   HIP_TRY(launch_inspect(W, inspect_args(e, nullptr, slot(e, 0), slot(e, 1), e->d_masks,
-                                         &e->d_ctl[0], &e->d_ctl[1 % e->R]),
+                                         &e->d_ctl[0], &e->d_ctl[1 % e->R], 1),
                          e->st));
   if (timing) { ti1 = next_event(e); HIP_TRY(hipEventRecord(ti1, e->st)); }
   HIP_TRY(hipMemcpyAsync(e->h_ctl, &e->d_ctl[0], sizeof(LevelCtl), hipMemcpyDeviceToHost, e->st));
@@ -269,9 +270,8 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
     hipEvent_t x0 = nullptr, x1 = nullptr, y1 = nullptr;
     if (timing) { x0 = next_event(e); HIP_TRY(hipEventRecord(x0, e->st)); }
     if (dir == 0) {
-      HIP_TRY(launch_expand_td(W, g->d_beg, g->d_adj, slot(e, L - 1), slot(e, L), e->flags, lamp,
-                               e->q, c.q[Q_TD_S], c.q[Q_TD_M], c.q[Q_TD_L], e->chunk_shift,
-                               e->st));
+      HIP_TRY(launch_expand_td(W, g->d_beg, g->d_adj, slot(e, L - 1), slot(e, L), lamp, e->q,
+                               c.q[Q_TD_S], c.q[Q_TD_M], c.q[Q_TD_L], e->chunk_shift, e->st));
       if (stats) stats->n_td_launches++;
     } else {
       if (e->cfg.work_model) {  // instrumentation: exact early-termination prefixes
@@ -297,7 +297,8 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
     hipEvent_t y0 = nullptr;
     if (timing) { y0 = next_event(e); HIP_TRY(hipEventRecord(y0, e->st)); }
     HIP_TRY(launch_inspect(W, inspect_args(e, slot(e, L - 1), slot(e, L), slot(e, L + 1), lamp,
-                                           &e->d_ctl[L % e->R], &e->d_ctl[(L + 1) % e->R]),
+                                           &e->d_ctl[L % e->R], &e->d_ctl[(L + 1) % e->R],
+                                           dir == 0 ? 1 : 0),
                            e->st));
     if (timing) { y1 = next_event(e); HIP_TRY(hipEventRecord(y1, e->st)); }
     HIP_TRY(hipMemcpyAsync(e->h_ctl, &e->d_ctl[L % e->R], sizeof(LevelCtl),

@@ -68,8 +68,10 @@ struct Queues {
 // Arguments of one inspector pass.  Host-driven: prev/cur/next/lam_prev/ctl
 // given explicitly and plan == nullptr.  Device-driven: plan != nullptr and
 // the level, snapshot slots and control blocks are derived from plan->level.
-// flags: one byte per traversal vertex, set by whatever changed the vertex's
-// word at this level (expanders, source admission), cleared by the pass.
+// flags: one byte per traversal vertex, set by the bottom-up expander and
+// the source admission for the vertices they change, cleared by the pass.
+// full_diff (host-driven): diff every vertex (after a top-down level and at
+// level 0) instead of only the flagged ones; plan mode derives it.
 struct InspectArgs {
   const u64 *prev;
   const u64 *cur;
@@ -93,6 +95,7 @@ struct InspectArgs {
   const u64 *lanes_all; // lam_prev of level 0
   uint8_t *flags;
   int vpt;              // vertices per lane and chunk (0: by size)
+  int full_diff;
 };
 
 // Launchers (all asynchronous on `st`).  W = status words (u64) per vertex.
@@ -106,8 +109,8 @@ hipError_t launch_admit(int W, const int32_t *src, const int32_t *lane, int n, u
                         uint8_t *flags, hipStream_t st);
 hipError_t launch_inspect(int W, const InspectArgs &a, hipStream_t st);
 hipError_t launch_expand_td(int W, const int64_t *beg, const int32_t *adj, const u64 *prev,
-                            u64 *cur, uint8_t *flags, const u64 *lam_prev, Queues q, int64_t ns,
-                            int64_t nm, int64_t nl, int chunk_shift, hipStream_t st);
+                            u64 *cur, const u64 *lam_prev, Queues q, int64_t ns, int64_t nm,
+                            int64_t nl, int chunk_shift, hipStream_t st);
 // tadj: traversal adjacency (neighbours by descending degree, hub entries
 // encoded as 0x80000000 | hub slot); hs: hub cache mirroring prev.
 hipError_t launch_expand_bu(int W, const int64_t *beg, const int32_t *tadj, const u64 *prev,

@@ -78,10 +78,11 @@ __device__ void plan_decide(Plan *p, const LevelCtl *c, int L) {
 // ---------------------------------------------------------------------------
 // Inspector of level L: one pass over the traversal vertices.  Copies S_L
 // (cur) into S_{L+1} (next, the slot the expanders of L+1 update), refreshes
-// the hub cache from S_L, and for the vertices the expanders (or the source
-// admission) flagged as changed at L, clears the flag, diffs S_L against
-// S_{L-1} (prev is read only for them) and accumulates the lanes that
-// advanced (lam of L).  It compacts, per degree class, both the top-down
+// the hub cache from S_L, diffs S_L against S_{L-1} and accumulates the lanes
+// that advanced (lam of L).  After a bottom-up level only the vertices the
+// expander (or the source admission) flagged are diffed (prev is read only
+// for them); after a top-down level and at level 0 every vertex is.  Flags
+// are cleared as they are read.  It compacts, per degree class, both the top-down
 // frontier of L+1 (changed, degree > 0) and the bottom-up candidates
 // (degree > 0 and missing a lane of lam_prev).  Replaces
 // td/sw_frontier_count + insp_scan + td/sw_frontier_gather + extract_depth
@@ -112,9 +113,11 @@ __global__ __launch_bounds__(NW * 64) void k_inspect(InspectArgs a) {
   u64 *next = a.next;
   const u64 *lam_prev = a.lam_prev;
   LevelCtl *ctl = a.ctl, *ctl_zero = a.ctl_zero;
+  int full = a.full_diff;
   if (a.plan) {
     if (a.plan->done) return;
     const int L = a.plan->level;
+    full = (L == 0 || a.plan->dir != 1);
     const int64_t slot = a.V * W;
     prev = L > 0 ? a.ring + (int64_t)((L - 1) % a.R) * slot : nullptr;
     cur = a.ring + (int64_t)(L % a.R) * slot;
@@ -125,11 +128,6 @@ __global__ __launch_bounds__(NW * 64) void k_inspect(InspectArgs a) {
   }
   if (blockIdx.x == 0 && ctl_zero && t < (int)(sizeof(LevelCtl) / 8))
     reinterpret_cast<u64 *>(ctl_zero)[t] = 0;
-  for (int64_t i = (int64_t)blockIdx.x * IBS + t; i < a.H; i += (int64_t)gridDim.x * IBS) {
-    u64 x[W];
-    load_word<W>(cur + (int64_t)a.hubs[i] * W, x);
-    store_word<W>(a.hs + i * W, x);
-  }
 
   u64 lp[W], keep[W], nand[W], lamacc[W];
 #pragma unroll
@@ -138,6 +136,11 @@ __global__ __launch_bounds__(NW * 64) void k_inspect(InspectArgs a) {
     keep[w] = a.prev_keep ? a.prev_keep[w] : ~0ull;
     nand[w] = a.next_and ? a.next_and[w] : ~0ull;
     lamacc[w] = 0;
+  }
+  for (int64_t i = (int64_t)blockIdx.x * IBS + t; i < a.H; i += (int64_t)gridDim.x * IBS) {
+    u64 x[W];
+    load_word<W>(cur + (int64_t)a.hubs[i] * W, x);
+    store_word<W>(a.hs + i * W, x);
   }
   u64 n_changed = 0, n_front = 0, m_front = 0, n_cand = 0, m_cand = 0;
   const int64_t V = a.V;
@@ -174,7 +177,8 @@ __global__ __launch_bounds__(NW * 64) void k_inspect(InspectArgs a) {
     if (prevp) {
 #pragma unroll
       for (int i = 0; i < VPT; ++i)
-        if (fl[i]) load_word<W>(prevp + (c0 + i * 64 + lane) * W, pw[i]);
+        if ((full || fl[i]) && c0 + i * 64 + lane < V)
+          load_word<W>(prevp + (c0 + i * 64 + lane) * W, pw[i]);
     }
     uint8_t code[VPT];  // bits 0-1 top-down class, bits 2-3 bottom-up class
     unsigned tot[6] = {0, 0, 0, 0, 0, 0};  // wave totals (uniform)
@@ -182,11 +186,12 @@ __global__ __launch_bounds__(NW * 64) void k_inspect(InspectArgs a) {
     for (int i = 0; i < VPT; ++i) {
       const int64_t v = c0 + i * 64 + lane;
       int tc = 0, bc = 0, nck = 0;
+      bool ch = false;
       if (v < V) {
-        bool ch = false, cover = true;
+        bool cover = true;
         u64 n[W];
-        if (fl[i]) {
-          flags[v] = 0;
+        if (fl[i]) flags[v] = 0;
+        if (full || fl[i]) {
 #pragma unroll
           for (int w = 0; w < W; ++w) {
             const u64 d = cw[i][w] & ~(pw[i][w] & keep[w]);
@@ -320,12 +325,13 @@ __global__ void k_plan_decide(Plan *plan, const LevelCtl *ctl_ring, int R) {
 // Top-down: a frontier vertex u pushes S_{L-1}[u] & lam_{L-1} into S_L of its
 // neighbours with 64-bit atomic ORs, skipping neighbours that already hold
 // the bits (expander.cu:100-252).
-// A neighbour whose word misses a pushed bit when read changes at this level
-// (by this push or a concurrent one), so it is flagged for the inspector.
+// Top-down levels flag nothing: their changes are scattered, and a byte store
+// per push costs a partial-line write once the line is evicted; the inspector
+// diffs every vertex after a top-down level instead.
 template <int W>
 __device__ __forceinline__ void td_push(const int64_t *__restrict__ beg,
                                         const int32_t *__restrict__ adj,
-                                        const u64 *__restrict__ prev, u64 *cur, uint8_t *flags,
+                                        const u64 *__restrict__ prev, u64 *cur,
                                         const u64 (&lam)[W], int32_t u, int64_t j0, int64_t j1,
                                         int stride) {
   u64 p[W];
@@ -336,14 +342,46 @@ __device__ __forceinline__ void td_push(const int64_t *__restrict__ beg,
     const int64_t v = adj[j];
     u64 c[W];
     load_word<W>(cur + v * W, c);
-    bool hit = false;
 #pragma unroll
     for (int w = 0; w < W; ++w)
-      if (p[w] & ~c[w]) {
-        atomicOr(&cur[v * W + w], p[w]);
-        hit = true;
-      }
-    if (hit) flags[v] = 1;
+      if (p[w] & ~c[w]) atomicOr(&cur[v * W + w], p[w]);
+  }
+}
+
+// One thread per small frontier vertex: four neighbours per step, their
+// adjacency entries and words loaded before any atomic, so four dependent
+// load chains are in flight instead of one.
+template <int W>
+__device__ __forceinline__ void td_push_small(const int64_t *__restrict__ beg,
+                                              const int32_t *__restrict__ adj,
+                                              const u64 *__restrict__ prev, u64 *cur,
+                                              const u64 (&lam)[W], int32_t u) {
+  u64 p[W];
+  load_word<W>(prev + (int64_t)u * W, p);
+#pragma unroll
+  for (int w = 0; w < W; ++w) p[w] &= lam[w];
+  int64_t j = beg[u];
+  const int64_t e = beg[u + 1];
+  for (; j + 4 <= e; j += 4) {
+    int64_t v[4];
+    u64 c[4][W];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[r] = adj[j + r];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) load_word<W>(cur + v[r] * W, c[r]);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int w = 0; w < W; ++w)
+        if (p[w] & ~c[r][w]) atomicOr(&cur[v[r] * W + w], p[w]);
+  }
+  for (; j < e; ++j) {
+    const int64_t v = adj[j];
+    u64 c[W];
+    load_word<W>(cur + v * W, c);
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+      if (p[w] & ~c[w]) atomicOr(&cur[v * W + w], p[w]);
   }
 }
 
@@ -362,9 +400,8 @@ __device__ __forceinline__ void td_chunk_range(const int64_t *__restrict__ beg, 
 // One candidate per thread, four neighbours per step.
 template <int W>
 __device__ __forceinline__ void bu_thread(const int64_t *__restrict__ beg,
-                                          const int32_t *__restrict__ tadj,
-                                          const u64 *__restrict__ prev,
-                                          const u64 *__restrict__ hs, u64 *__restrict__ cur, uint8_t *flags,
+                                          const int32_t *__restrict__ tadj, const Gather &gs,
+                                          u64 *__restrict__ cur, uint8_t *flags,
                                           const u64 (&lam)[W], int64_t v) {
   u64 x[W], x0[W];
   load_word<W>(cur + v * W, x);
@@ -379,10 +416,10 @@ __device__ __forceinline__ void bu_thread(const int64_t *__restrict__ beg,
     const int32_t a2 = (j + 2 < e) ? tadj[j + 2] : a0;
     const int32_t a3 = (j + 3 < e) ? tadj[j + 3] : a0;
     u64 s0[W], s1[W], s2[W], s3[W];
-    load_nbr<W>(prev, hs, a0, s0);
-    load_nbr<W>(prev, hs, a1, s1);
-    load_nbr<W>(prev, hs, a2, s2);
-    load_nbr<W>(prev, hs, a3, s3);
+    load_nbr<W>(gs, a0, s0);
+    load_nbr<W>(gs, a1, s1);
+    load_nbr<W>(gs, a2, s2);
+    load_nbr<W>(gs, a3, s3);
 #pragma unroll
     for (int w = 0; w < W; ++w) x[w] |= (s0[w] | s1[w] | s2[w] | s3[w]) & lam[w];
     j += 4;
@@ -397,10 +434,9 @@ __device__ __forceinline__ void bu_thread(const int64_t *__restrict__ beg,
 // One candidate per wave: 64 neighbours per step, wave OR-reduction.
 template <int W>
 __device__ __forceinline__ void bu_wave(const int64_t *__restrict__ beg,
-                                        const int32_t *__restrict__ tadj,
-                                        const u64 *__restrict__ prev, const u64 *__restrict__ hs,
-                                        u64 *__restrict__ cur, uint8_t *flags, const u64 (&lam)[W], int64_t v,
-                                        int lane) {
+                                        const int32_t *__restrict__ tadj, const Gather &gs,
+                                        u64 *__restrict__ cur, uint8_t *flags,
+                                        const u64 (&lam)[W], int64_t v, int lane) {
   u64 x[W], x0[W];
   load_word<W>(cur + v * W, x);
 #pragma unroll
@@ -411,7 +447,7 @@ __device__ __forceinline__ void bu_wave(const int64_t *__restrict__ beg,
     const int64_t j = r + lane;
     u64 y[W];
     if (j < e) {
-      load_nbr<W>(prev, hs, tadj[j], y);
+      load_nbr<W>(gs, tadj[j], y);
     } else {
 #pragma unroll
       for (int w = 0; w < W; ++w) y[w] = 0;
@@ -430,10 +466,9 @@ __device__ __forceinline__ void bu_wave(const int64_t *__restrict__ beg,
 // partials combined through LDS.  Must be called by the whole workgroup.
 template <int W>
 __device__ __forceinline__ void bu_block(const int64_t *__restrict__ beg,
-                                         const int32_t *__restrict__ tadj,
-                                         const u64 *__restrict__ prev, const u64 *__restrict__ hs,
-                                         u64 *__restrict__ cur, uint8_t *flags, const u64 (&lam)[W], int64_t v,
-                                         u64 (*part)[W]) {
+                                         const int32_t *__restrict__ tadj, const Gather &gs,
+                                         u64 *__restrict__ cur, uint8_t *flags,
+                                         const u64 (&lam)[W], int64_t v, u64 (*part)[W]) {
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   u64 x[W], x0[W];
   load_word<W>(cur + v * W, x);
@@ -445,7 +480,7 @@ __device__ __forceinline__ void bu_block(const int64_t *__restrict__ beg,
     const int64_t j = r + t;
     u64 y[W];
     if (j < e) {
-      load_nbr<W>(prev, hs, tadj[j], y);
+      load_nbr<W>(gs, tadj[j], y);
     } else {
 #pragma unroll
       for (int w = 0; w < W; ++w) y[w] = 0;
@@ -475,9 +510,9 @@ template <int W>
 __global__ __launch_bounds__(BS) void k_expand_td(const int64_t *__restrict__ beg,
                                                   const int32_t *__restrict__ adj,
                                                   const u64 *__restrict__ prev, u64 *cur,
-                                                  uint8_t *flags, const u64 *__restrict__ lam_prev,
-                                                  Queues q, int64_t ns, int64_t nm, int64_t nl,
-                                                  int64_t nbs, int64_t nbm, int cs) {
+                                                  const u64 *__restrict__ lam_prev, Queues q,
+                                                  int64_t ns, int64_t nm, int64_t nl, int64_t nbs,
+                                                  int64_t nbm, int cs) {
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int64_t b = blockIdx.x;
   u64 lam[W];
@@ -486,29 +521,26 @@ __global__ __launch_bounds__(BS) void k_expand_td(const int64_t *__restrict__ be
   if (b < nbs) {
     const int64_t i = b * BS + t;
     if (i >= ns) return;
-    const int32_t u = q.q[Q_TD_S][i];
-    td_push<W>(beg, adj, prev, cur, flags, lam, u, beg[u], beg[u + 1], 1);
+    td_push_small<W>(beg, adj, prev, cur, lam, q.q[Q_TD_S][i]);
   } else if (b < nbs + nbm) {
     const int64_t i = (b - nbs) * 4 + wv;
     if (i >= nm) return;
     const int32_t u = q.q[Q_TD_M][i];
-    td_push<W>(beg, adj, prev, cur, flags, lam, u, beg[u] + lane, beg[u + 1], 64);
+    td_push<W>(beg, adj, prev, cur, lam, u, beg[u] + lane, beg[u + 1], 64);
   } else {
     const int64_t i = b - nbs - nbm;
     if (i >= nl) return;
     const int2 e = q.chunks[i];
     int64_t lo, hi;
     td_chunk_range(beg, e, cs, lo, hi);
-    td_push<W>(beg, adj, prev, cur, flags, lam, e.x, lo + t, hi, BS);
+    td_push<W>(beg, adj, prev, cur, lam, e.x, lo + t, hi, BS);
   }
 }
 
 // Host-driven bottom-up, same workgroup ranges as k_expand_td.
 template <int W>
 __global__ __launch_bounds__(BS) void k_expand_bu(const int64_t *__restrict__ beg,
-                                                  const int32_t *__restrict__ tadj,
-                                                  const u64 *__restrict__ prev,
-                                                  const u64 *__restrict__ hs,
+                                                  const int32_t *__restrict__ tadj, Gather gs,
                                                   u64 *__restrict__ cur, uint8_t *flags,
                                                   const u64 *__restrict__ lam_prev, Queues q,
                                                   int64_t ns, int64_t nm, int64_t nl, int64_t nbs,
@@ -521,13 +553,13 @@ __global__ __launch_bounds__(BS) void k_expand_bu(const int64_t *__restrict__ be
   for (int w = 0; w < W; ++w) lam[w] = lam_prev[w];
   if (b < nbs) {
     const int64_t i = b * BS + t;
-    if (i < ns) bu_thread<W>(beg, tadj, prev, hs, cur, flags, lam, q.q[Q_BU_S][i]);
+    if (i < ns) bu_thread<W>(beg, tadj, gs, cur, flags, lam, q.q[Q_BU_S][i]);
   } else if (b < nbs + nbm) {
     const int64_t i = (b - nbs) * 4 + wv;
-    if (i < nm) bu_wave<W>(beg, tadj, prev, hs, cur, flags, lam, q.q[Q_BU_M][i], lane);
+    if (i < nm) bu_wave<W>(beg, tadj, gs, cur, flags, lam, q.q[Q_BU_M][i], lane);
   } else {
     const int64_t i = b - nbs - nbm;
-    if (i < nl) bu_block<W>(beg, tadj, prev, hs, cur, flags, lam, q.q[Q_BU_L][i], part);
+    if (i < nl) bu_block<W>(beg, tadj, gs, cur, flags, lam, q.q[Q_BU_L][i], part);
   }
 }
 
@@ -555,28 +587,27 @@ __global__ __launch_bounds__(BS) void k_expand_dd(const int64_t *__restrict__ be
   for (int w = 0; w < W; ++w) lam[w] = c->lam[w];
   if (dir == 0) {
     const int64_t ns = c->q[Q_TD_S], nm = c->q[Q_TD_M], nl = c->q[Q_TD_L];
-    for (int64_t i = (int64_t)blockIdx.x * BS + t; i < ns; i += G * BS) {
-      const int32_t u = q.q[Q_TD_S][i];
-      td_push<W>(beg, adj, prev, cur, flags, lam, u, beg[u], beg[u + 1], 1);
-    }
+    for (int64_t i = (int64_t)blockIdx.x * BS + t; i < ns; i += G * BS)
+      td_push_small<W>(beg, adj, prev, cur, lam, q.q[Q_TD_S][i]);
     for (int64_t i = (int64_t)blockIdx.x * 4 + wv; i < nm; i += G * 4) {
       const int32_t u = q.q[Q_TD_M][i];
-      td_push<W>(beg, adj, prev, cur, flags, lam, u, beg[u] + lane, beg[u + 1], 64);
+      td_push<W>(beg, adj, prev, cur, lam, u, beg[u] + lane, beg[u + 1], 64);
     }
     for (int64_t i = blockIdx.x; i < nl; i += G) {
       const int2 e = q.chunks[i];
       int64_t lo, hi;
       td_chunk_range(beg, e, cs, lo, hi);
-      td_push<W>(beg, adj, prev, cur, flags, lam, e.x, lo + t, hi, BS);
+      td_push<W>(beg, adj, prev, cur, lam, e.x, lo + t, hi, BS);
     }
   } else {
     const int64_t ns = c->q[Q_BU_S], nm = c->q[Q_BU_M], nl = c->q[Q_BU_L];
+    const Gather gs{prev, hs};
     for (int64_t i = (int64_t)blockIdx.x * BS + t; i < ns; i += G * BS)
-      bu_thread<W>(beg, tadj, prev, hs, cur, flags, lam, q.q[Q_BU_S][i]);
+      bu_thread<W>(beg, tadj, gs, cur, flags, lam, q.q[Q_BU_S][i]);
     for (int64_t i = (int64_t)blockIdx.x * 4 + wv; i < nm; i += G * 4)
-      bu_wave<W>(beg, tadj, prev, hs, cur, flags, lam, q.q[Q_BU_M][i], lane);
+      bu_wave<W>(beg, tadj, gs, cur, flags, lam, q.q[Q_BU_M][i], lane);
     for (int64_t i = blockIdx.x; i < nl; i += G)
-      bu_block<W>(beg, tadj, prev, hs, cur, flags, lam, q.q[Q_BU_L][i], part);
+      bu_block<W>(beg, tadj, gs, cur, flags, lam, q.q[Q_BU_L][i], part);
   }
 }
 
@@ -595,6 +626,7 @@ __global__ __launch_bounds__(BS) void k_bu_model(const int64_t *__restrict__ beg
   u64 lam[W];
 #pragma unroll
   for (int w = 0; w < W; ++w) lam[w] = lam_prev[w];
+  const Gather gs{prev, hs};
   u64 nv = 0, pref = 0;
   for (int64_t v = (int64_t)blockIdx.x * BS + threadIdx.x; v < V; v += (int64_t)gridDim.x * BS) {
     const int64_t s = beg[v], e = beg[v + 1];
@@ -605,7 +637,7 @@ __global__ __launch_bounds__(BS) void k_bu_model(const int64_t *__restrict__ beg
     ++nv;
     for (int64_t j = s; j < e; ++j) {
       u64 y[W];
-      load_nbr<W>(prev, hs, tadj[j], y);
+      load_nbr<W>(gs, tadj[j], y);
 #pragma unroll
       for (int w = 0; w < W; ++w) x[w] |= y[w] & lam[w];
       ++pref;
@@ -702,27 +734,28 @@ hipError_t launch_inspect(int W, const InspectArgs &a, hipStream_t st) {
 }
 
 hipError_t launch_expand_td(int W, const int64_t *beg, const int32_t *adj, const u64 *prev,
-                            u64 *cur, uint8_t *flags, const u64 *lam_prev, Queues q, int64_t ns,
-                            int64_t nm, int64_t nl, int chunk_shift, hipStream_t st) {
+                            u64 *cur, const u64 *lam_prev, Queues q, int64_t ns, int64_t nm,
+                            int64_t nl, int chunk_shift, hipStream_t st) {
   const int64_t nbs = (ns + BS - 1) / BS, nbm = (nm + 3) / 4;
   const int64_t nb = nbs + nbm + nl;
   if (nb == 0) return hipSuccess;
   if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
   dim3 g((unsigned)nb), blk(BS);
   IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_expand_td<WW>, g, blk, 0, st, beg, adj, prev, cur,
-                                        flags, lam_prev, q, ns, nm, nl, nbs, nbm, chunk_shift));
+                                        lam_prev, q, ns, nm, nl, nbs, nbm, chunk_shift));
   return hipGetLastError();
 }
 
 hipError_t launch_expand_bu(int W, const int64_t *beg, const int32_t *tadj, const u64 *prev,
                             const u64 *hs, u64 *cur, uint8_t *flags, const u64 *lam_prev,
                             Queues q, int64_t ns, int64_t nm, int64_t nl, hipStream_t st) {
+  const Gather gs{prev, hs};
   const int64_t nbs = (ns + BS - 1) / BS, nbm = (nm + 3) / 4;
   const int64_t nb = nbs + nbm + nl;
   if (nb == 0) return hipSuccess;
   if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
   dim3 g((unsigned)nb), blk(BS);
-  IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_expand_bu<WW>, g, blk, 0, st, beg, tadj, prev, hs, cur,
+  IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_expand_bu<WW>, g, blk, 0, st, beg, tadj, gs, cur,
                                         flags, lam_prev, q, ns, nm, nl, nbs, nbm));
   return hipGetLastError();
 }

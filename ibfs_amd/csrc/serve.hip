@@ -246,7 +246,7 @@ int admit(ibfs_server_s *s, SCtx &c, int L, u64 *A, int *n_adm) {
 
 // Inspection of level L (masks in staging slot I) + control-block copy + event.
 int inspect_level(ibfs_server_s *s, SCtx &c, int L, const u64 *filter, const u64 *keep,
-                  const u64 *nand) {
+                  const u64 *nand, int full_diff) {
   const int W = s->W;
   u64 *h = c.h_mask + 2 * emit_words(s);
   u64 *d = c.d_mask + 2 * emit_words(s);
@@ -276,6 +276,7 @@ int inspect_level(ibfs_server_s *s, SCtx &c, int L, const u64 *filter, const u64
   a.hs = c.hs;
   a.flags = c.flags;
   a.vpt = s->cfg.inspect_vpt;
+  a.full_diff = full_diff;
   HIP_CHK(launch_inspect(W, a, c.st));
   HIP_CHK(hipMemcpyAsync(c.h_ctl, c.ctl + L % s->R, sizeof(LevelCtl), hipMemcpyDeviceToHost, c.st));
   HIP_CHK(hipEventRecord(c.ev, c.st));
@@ -299,7 +300,7 @@ int start_ctx(ibfs_server_s *s, SCtx &c) {
   int rc = admit(s, c, 0, A.data(), &n);
   if (rc) return rc;
   c.running = true;
-  return inspect_level(s, c, 0, A.data(), keep.data(), nand.data());
+  return inspect_level(s, c, 0, A.data(), keep.data(), nand.data(), 1);
 }
 
 int decide(const ibfs_server_s *s, int Ln, const LevelCtl &lc, int dir_prev, u64 nf_prev) {
@@ -378,9 +379,8 @@ int process_level(ibfs_server_s *s, SCtx &c) {
   const int dir = decide(s, Ln, lc, c.dir_prev, c.nf_prev);
   const u64 *lamp = reinterpret_cast<const u64 *>(c.ctl + L % s->R);
   if (dir == 0)
-    HIP_CHK(launch_expand_td(W, s->g->d_beg, s->g->d_adj, slot(s, c, L), slot(s, c, Ln), c.flags,
-                             lamp, c.q, lc.q[Q_TD_S], lc.q[Q_TD_M], lc.q[Q_TD_L], s->chunk_shift,
-                             c.st));
+    HIP_CHK(launch_expand_td(W, s->g->d_beg, s->g->d_adj, slot(s, c, L), slot(s, c, Ln), lamp,
+                             c.q, lc.q[Q_TD_S], lc.q[Q_TD_M], lc.q[Q_TD_L], s->chunk_shift, c.st));
   else
     HIP_CHK(launch_expand_bu(W, s->g->d_beg, s->g->d_tadj, slot(s, c, L), c.hs, slot(s, c, Ln),
                              c.flags, lamp, c.q, lc.q[Q_BU_S], lc.q[Q_BU_M], lc.q[Q_BU_L], c.st));
@@ -423,7 +423,7 @@ int process_level(ibfs_server_s *s, SCtx &c) {
     nand[w] = ~finmask[w];
   }
   // 5. inspection of Ln (clears the lanes freed at L in slot Ln+1)
-  return inspect_level(s, c, Ln, filter.data(), keep.data(), nand.data());
+  return inspect_level(s, c, Ln, filter.data(), keep.data(), nand.data(), dir == 0 ? 1 : 0);
 }
 
 void worker_main(ibfs_server_s *s) {
