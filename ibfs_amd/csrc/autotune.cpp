@@ -82,7 +82,7 @@ int ibfs_autotune(ibfs_graph_t g, const int32_t *src, int64_t n, const ibfs_conf
   rc = ibfs_device_alloc(cur.device, sizeof(int32_t) * (size_t)n * (size_t)V, (void **)&dout);
   if (rc) return rc;
   const std::vector<Param> params = {
-      {"lane_bits", &ibfs_config::lane_bits, {64, 128, 256}},
+      {"lane_bits", &ibfs_config::lane_bits, {64, 128, 256, 512}},
       {"alpha", &ibfs_config::alpha, {1, 2, 3, 4, 6, 8, 14}},
       {"beta", &ibfs_config::beta, {4, 8, 16, 24, 48, 96}},
       {"sml_shed", &ibfs_config::sml_shed, {8, 16, 32, 64}},

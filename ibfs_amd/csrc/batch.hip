@@ -227,7 +227,7 @@ int start_group(ibfs_batch_s *b, Ctx &c, const int32_t *src, int64_t g0, int k, 
   HIP_TRY(hipMemcpyAsync(c.d_plan, c.h_plan, sizeof(Plan), hipMemcpyHostToDevice, c.st));
   HIP_TRY(launch_fill_u64(slot(b, c, 0), b->g->Vt * W, 0ull, c.st));
   HIP_TRY(launch_fill_u64(reinterpret_cast<u64 *>(c.flags), (b->g->Vt + 7) / 8, 0ull, c.st));
-  HIP_TRY(launch_fill_u64(reinterpret_cast<u64 *>(c.ctl), (int64_t)b->R * 16, 0ull, c.st));
+  HIP_TRY(launch_fill_u64(reinterpret_cast<u64 *>(c.ctl), (int64_t)b->R * CTL_WORDS, 0ull, c.st));
   HIP_TRY(launch_init_sources(W, c.d_src, k, 0, slot(b, c, 0), c.flags, c.st));
   HIP_TRY(launch_inspect(W, dd_inspect_args(b, c), c.st));
   HIP_TRY(launch_plan_decide(c.d_plan, c.ctl, b->R, c.st));
@@ -284,8 +284,8 @@ int ibfs_batch_create(ibfs_graph_t g, const ibfs_config *cfg_in, int32_t context
   ibfs_config cfg;
   if (cfg_in) cfg = *cfg_in;
   else ibfs_config_default(&cfg);
-  if (cfg.lane_bits != 64 && cfg.lane_bits != 128 && cfg.lane_bits != 256)
-    return fail(IBFS_ERR_ARG, "lane_bits must be 64, 128 or 256 (got %d)", cfg.lane_bits);
+  if (cfg.lane_bits != 64 && cfg.lane_bits != 128 && cfg.lane_bits != 256 && cfg.lane_bits != 512)
+    return fail(IBFS_ERR_ARG, "lane_bits must be 64, 128, 256 or 512 (got %d)", cfg.lane_bits);
   if (cfg.sml_shed < 1 || cfg.lrg_shed < cfg.sml_shed)
     return fail(IBFS_ERR_ARG, "need 1 <= sml_shed <= lrg_shed");
   if (cfg.policy < IBFS_POLICY_FIXED || cfg.policy > IBFS_POLICY_BU_ONLY)

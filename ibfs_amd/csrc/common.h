@@ -26,7 +26,9 @@ struct HostCsr {
 };
 
 // inspector chunking override of ibfs_config (0 = by vertex count)
-inline bool valid_inspect_vpt(int v) { return v == 0 || v == 1 || v == 4 || v == 8 || v == 16; }
+inline bool valid_inspect_vpt(int v) {
+  return v == 0 || v == 1 || v == 2 || v == 4 || v == 8 || v == 16;
+}
 
 // Parallel helper: runs fn(t) on nthreads std::threads.
 void parallel_for_threads(int nthreads, void (*fn)(int, void *), void *ctx);

@@ -216,7 +216,7 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
   // level 0: S_0 = source bits; inspection builds the level-1 queues
   HIP_TRY(launch_fill_u64(slot(e, 0), Vt * W, 0ull, e->st));
   HIP_TRY(launch_fill_u64(reinterpret_cast<u64 *>(e->flags), (Vt + 7) / 8, 0ull, e->st));
-  HIP_TRY(launch_fill_u64(reinterpret_cast<u64 *>(e->d_ctl), (int64_t)e->R * 16, 0ull, e->st));
+  HIP_TRY(launch_fill_u64(reinterpret_cast<u64 *>(e->d_ctl), (int64_t)e->R * CTL_WORDS, 0ull, e->st));
   HIP_TRY(launch_init_sources(W, e->d_src, k, 0, slot(e, 0), e->flags, e->st));
   hipEvent_t ti0 = nullptr, ti1 = nullptr;
   if (timing) { ti0 = next_event(e); HIP_TRY(hipEventRecord(ti0, e->st)); }
@@ -628,8 +628,8 @@ int ibfs_engine_create(ibfs_graph_t g, const ibfs_config *cfg_in, ibfs_engine_t 
   ibfs_config cfg;
   if (cfg_in) cfg = *cfg_in;
   else ibfs_config_default(&cfg);
-  if (cfg.lane_bits != 64 && cfg.lane_bits != 128 && cfg.lane_bits != 256)
-    return fail(IBFS_ERR_ARG, "lane_bits must be 64, 128 or 256 (got %d)", cfg.lane_bits);
+  if (cfg.lane_bits != 64 && cfg.lane_bits != 128 && cfg.lane_bits != 256 && cfg.lane_bits != 512)
+    return fail(IBFS_ERR_ARG, "lane_bits must be 64, 128, 256 or 512 (got %d)", cfg.lane_bits);
   if (cfg.sml_shed < 1 || cfg.lrg_shed < cfg.sml_shed)
     return fail(IBFS_ERR_ARG, "need 1 <= sml_shed <= lrg_shed (got %d, %d)", cfg.sml_shed, cfg.lrg_shed);
   if (cfg.policy < IBFS_POLICY_FIXED || cfg.policy > IBFS_POLICY_BU_ONLY)

@@ -9,20 +9,24 @@ namespace ibfs {
 
 typedef unsigned long long u64;
 
+constexpr int MAX_W = 8;               // status words per vertex: up to 512 lanes
+constexpr int MAX_LANES = 64 * MAX_W;
+
 // Per-level control block, written by the inspector of level L with device
 // atomics and read by the expanders of level L+1, the direction decision and
 // the host.
-struct alignas(128) LevelCtl {
-  u64 lam[4];          // lanes that gained a vertex at level L (OR of new bits)
+struct alignas(64) LevelCtl {
+  u64 lam[MAX_W];      // lanes that gained a vertex at level L (OR of new bits)
   u64 n_changed;       // vertices with new bits at L
   u64 n_front;         // ... of degree > 0 (top-down frontier of L+1)
   u64 m_front;         // sum of their degrees
   u64 n_cand;          // bottom-up candidates for L+1
   u64 m_cand;          // sum of their degrees
   unsigned int q[6];   // queue lengths: td sml, td mid, td lrg chunks, bu sml, bu mid, bu lrg
-  unsigned int pad[2];
+  unsigned int pad[14];
 };
-static_assert(sizeof(LevelCtl) == 128, "LevelCtl layout");
+static_assert(sizeof(LevelCtl) == 192, "LevelCtl layout");
+constexpr int64_t CTL_WORDS = sizeof(LevelCtl) / 8;
 // the inspector reserves queue ranges two classes at a time (q[2j], q[2j+1]
 // as one little-endian 64-bit counter)
 static_assert(offsetof(LevelCtl, q) % 8 == 0, "LevelCtl queue pairs");

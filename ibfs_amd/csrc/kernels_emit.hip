@@ -87,9 +87,9 @@ __global__ __launch_bounds__(BS) void k_emit(const u64 *__restrict__ ring, int R
                                              int b, int e, const LaneOut *__restrict__ lanes,
                                              const u64 *__restrict__ level_masks,
                                              const u64 *__restrict__ sel, int k_lanes, int nt) {
-  __shared__ int32_t *s_out[4 * 64];
-  __shared__ int32_t s_off[4 * 64];  // b - admit: <= 0 in the lane's first window
-  __shared__ int32_t s_src[4 * 64];
+  __shared__ int32_t *s_out[MAX_LANES];
+  __shared__ int32_t s_off[MAX_LANES];  // b - admit: <= 0 in the lane's first window
+  __shared__ int32_t s_src[MAX_LANES];
   for (int i = threadIdx.x; i < k_lanes; i += BS) {
     const bool on = (sel[i >> 6] >> (i & 63)) & 1ull;
     s_out[i] = on ? lanes[i].out : nullptr;
@@ -133,9 +133,9 @@ __global__ __launch_bounds__(BS) void k_emit_tiled(const u64 *__restrict__ ring,
                                                    const u64 *__restrict__ sel, int k_lanes,
                                                    int vec4) {
   __shared__ int4 tile4[32 * 64];  // [row][64 x int4] = 32 lanes x 256 vertices
-  __shared__ int32_t *s_out[4 * 64];
-  __shared__ int32_t s_off[4 * 64];
-  __shared__ int32_t s_src[4 * 64];
+  __shared__ int32_t *s_out[MAX_LANES];
+  __shared__ int32_t s_off[MAX_LANES];
+  __shared__ int32_t s_src[MAX_LANES];
   int32_t *tile = reinterpret_cast<int32_t *>(tile4);
   const int t = threadIdx.x;
   const int64_t V = sp.V;
@@ -215,10 +215,10 @@ __global__ __launch_bounds__(BS) void k_close(const u64 *__restrict__ ring, int 
                                               const u64 *__restrict__ sel, int k_lanes,
                                               unsigned long long *__restrict__ sum_out,
                                               unsigned long long *__restrict__ reach_out) {
-  __shared__ int32_t s_off[4 * 64];
-  __shared__ u64 red[2][4][4 * 64];
+  __shared__ int32_t s_off[MAX_LANES];
+  __shared__ u64 red[2][4][MAX_LANES];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  for (int i = t; i < 4 * 64; i += BS) s_off[i] = (i < k_lanes) ? b - lanes[i].admit : 0;
+  for (int i = t; i < MAX_LANES; i += BS) s_off[i] = (i < k_lanes) ? b - lanes[i].admit : 0;
   __syncthreads();
   u64 selw[W], cont[W];
 #pragma unroll
@@ -401,6 +401,7 @@ hipError_t launch_emit(int W, int mode, int vec4, const u64 *ring, int R, EmitSp
     case 1: return emit_w<1>(mode, vec4, ring, R, sp, b, e, lanes, level_masks, sel, k_lanes, st);
     case 2: return emit_w<2>(mode, vec4, ring, R, sp, b, e, lanes, level_masks, sel, k_lanes, st);
     case 4: return emit_w<4>(mode, vec4, ring, R, sp, b, e, lanes, level_masks, sel, k_lanes, st);
+    case 8: return emit_w<8>(mode, vec4, ring, R, sp, b, e, lanes, level_masks, sel, k_lanes, st);
     default: return hipErrorInvalidValue;
   }
 }
@@ -413,6 +414,7 @@ hipError_t launch_count_edges(int W, const u64 *S, const int32_t *deg, int64_t V
     case 1: hipLaunchKernelGGL(k_count_edges<1>, g, blk, 0, st, S, deg, V, lanes, out); break;
     case 2: hipLaunchKernelGGL(k_count_edges<2>, g, blk, 0, st, S, deg, V, lanes, out); break;
     case 4: hipLaunchKernelGGL(k_count_edges<4>, g, blk, 0, st, S, deg, V, lanes, out); break;
+    case 8: hipLaunchKernelGGL(k_count_edges<8>, g, blk, 0, st, S, deg, V, lanes, out); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -443,6 +445,7 @@ hipError_t launch_close(int W, const u64 *ring, int R, EmitSpace sp, int b, int 
     case 1: return close_w<1>(ring, R, sp, b, e, lanes, level_masks, sel, k_lanes, sum_out, reach_out, st);
     case 2: return close_w<2>(ring, R, sp, b, e, lanes, level_masks, sel, k_lanes, sum_out, reach_out, st);
     case 4: return close_w<4>(ring, R, sp, b, e, lanes, level_masks, sel, k_lanes, sum_out, reach_out, st);
+    case 8: return close_w<8>(ring, R, sp, b, e, lanes, level_masks, sel, k_lanes, sum_out, reach_out, st);
     default: return hipErrorInvalidValue;
   }
 }

@@ -663,6 +663,7 @@ __global__ __launch_bounds__(BS) void k_bu_model(const int64_t *__restrict__ beg
     case 1: { constexpr int WW = 1; STMT; break; } \
     case 2: { constexpr int WW = 2; STMT; break; } \
     case 4: { constexpr int WW = 4; STMT; break; } \
+    case 8: { constexpr int WW = 8; STMT; break; } \
     default: return hipErrorInvalidValue;    \
   }
 
@@ -698,13 +699,17 @@ hipError_t launch_admit(int W, const int32_t *src, const int32_t *lane, int n, u
   return hipGetLastError();
 }
 
-// Vertices per lane and round set: 8 (4 for 256-lane words, whose
-// registers would otherwise limit occupancy) when that still gives 512
-// tiles of 512 threads, else 4, else 1; a.vpt overrides.
+// Vertices per lane and round set: the largest of 8, 4, 2 whose chunk
+// registers stay within 16 words per lane (8 for 128-lane words, 4 for 256,
+// 2 for 512) and that still gives 512 tiles of 512 threads, else 1; a.vpt
+// overrides (capped for wide words, which would spill registers).
 int inspect_vpt(const InspectArgs &a, int W) {
-  if (a.vpt == 16) return W == 1 ? 16 : 8;  // wider words would spill registers
-  if (a.vpt == 1 || a.vpt == 4 || a.vpt == 8) return a.vpt;
-  for (int v : {8, 4})
+  if (a.vpt == 1 || a.vpt == 2 || a.vpt == 4 || a.vpt == 8 || a.vpt == 16) {
+    int v = a.vpt;
+    while (v > 2 && v * W > 16) v /= 2;
+    return v;
+  }
+  for (int v : {8, 4, 2})
     if (v * W <= 16 && a.V >= (int64_t)512 * v * 512) return v;
   return 1;
 }
@@ -723,6 +728,7 @@ static void inspect_w(const InspectArgs &a, hipStream_t st) {
     case 16: inspect_launch<W, 16>(a, st); break;
     case 8: inspect_launch<W, 8>(a, st); break;
     case 4: inspect_launch<W, 4>(a, st); break;
+    case 2: inspect_launch<W, 2>(a, st); break;
     default: inspect_launch<W, 1>(a, st); break;
   }
 }

@@ -294,7 +294,7 @@ int start_ctx(ibfs_server_s *s, SCtx &c) {
   c.nf_prev = ~0ull;
   HIP_CHK(launch_fill_u64(slot(s, c, 0), s->g->Vt * W, 0ull, c.st));
   HIP_CHK(launch_fill_u64(reinterpret_cast<u64 *>(c.flags), (s->g->Vt + 7) / 8, 0ull, c.st));
-  HIP_CHK(launch_fill_u64(reinterpret_cast<u64 *>(c.ctl), (int64_t)s->R * 16, 0ull, c.st));
+  HIP_CHK(launch_fill_u64(reinterpret_cast<u64 *>(c.ctl), (int64_t)s->R * CTL_WORDS, 0ull, c.st));
   std::vector<u64> A(W, 0), keep(W, ~0ull), nand(W, ~0ull);
   int n = 0;
   int rc = admit(s, c, 0, A.data(), &n);
@@ -482,8 +482,8 @@ int ibfs_server_create(ibfs_graph_t g, const ibfs_config *cfg_in, int64_t status
   ibfs_config cfg;
   if (cfg_in) cfg = *cfg_in;
   else ibfs_config_default(&cfg);
-  if (cfg.lane_bits != 64 && cfg.lane_bits != 128 && cfg.lane_bits != 256)
-    return fail(IBFS_ERR_ARG, "lane_bits must be 64, 128 or 256");
+  if (cfg.lane_bits != 64 && cfg.lane_bits != 128 && cfg.lane_bits != 256 && cfg.lane_bits != 512)
+    return fail(IBFS_ERR_ARG, "lane_bits must be 64, 128, 256 or 512");
   if (cfg.ring_levels < 8 || cfg.ring_levels > 255)
     return fail(IBFS_ERR_ARG, "serving needs ring_levels in [8, 255]");
   if (cfg.td_chunk < 256 || (cfg.td_chunk & (cfg.td_chunk - 1)))

@@ -51,7 +51,7 @@ typedef struct ibfs_server_s *ibfs_server_t;
  * 128 lanes (concurrent-count, readme.md:27), sml_shed=32 and lrg_shed=512
  * (main.cu:56-57). */
 typedef struct ibfs_config {
-  int32_t lane_bits;   /* 64, 128 or 256 source lanes per status word          */
+  int32_t lane_bits;   /* 64, 128, 256 or 512 source lanes per status word     */
   int32_t sml_shed;    /* degree < sml_shed        -> thread class             */
   int32_t lrg_shed;    /* degree > lrg_shed        -> workgroup class          */
   int32_t policy;      /* IBFS_POLICY_*                                        */
@@ -68,8 +68,8 @@ typedef struct ibfs_config {
   int32_t emit_mode;   /* depth emission: 0 per-thread stores, 1 LDS-tiled,
                           2 per-thread non-temporal stores                   */
   int32_t dd_grid;     /* workgroups of the device-driven expander (0: auto)  */
-  int32_t inspect_vpt; /* inspector vertices per lane and tile: 1, 4, 8 or 16
-                          (16 only for 64 lanes, else 8; 0: by vertex count) */
+  int32_t inspect_vpt; /* inspector vertices per lane and tile: 1, 2, 4, 8 or
+                          16, capped at 16 words per lane (0: by size)       */
 } ibfs_config;
 
 /* Per-level record of the last traversal (the build's monitoring log, in place

@@ -48,7 +48,7 @@ def test_golden_all_sources(name, pol):
     np.testing.assert_array_equal(got, z["depth"])
 
 
-@pytest.mark.parametrize("lane_bits", [64, 128, 256])
+@pytest.mark.parametrize("lane_bits", [64, 128, 256, 512])
 @pytest.mark.parametrize("pol", ["ab", "bu", "td"])
 def test_s16_128_sources(s16, lane_bits, pol):
     say(f"[gpu] s16 lane_bits={lane_bits} policy={pol}")
@@ -143,8 +143,8 @@ def test_lane_position_invariance():
     say("[gpu] lane position")
     z = load_golden("bfs_grid6x7.npz")
     g = ibfs_amd.DeviceGraph(ibfs_amd.HostCsr.from_arrays(z["beg_pos"], z["adj"]))
-    eng = ibfs_amd.Engine(g, lane_bits=256)
-    for lane in (0, 63, 64, 127, 128, 255):
+    eng = ibfs_amd.Engine(g, lane_bits=512)
+    for lane in (0, 63, 64, 127, 128, 255, 256, 383, 448, 511):
         src = np.full(lane + 1, 5, dtype=np.int32)
         src[lane] = 17
         got = eng.traversal(src)
@@ -204,7 +204,7 @@ def test_s16_hub_cache(s16, hubs):
         np.testing.assert_array_equal(eng.run_groups(s16["src"]), s16["want"])
 
 
-@pytest.mark.parametrize("lane_bits", [64, 128, 256])
+@pytest.mark.parametrize("lane_bits", [64, 128, 256, 512])
 @pytest.mark.parametrize("contexts,burst", [(1, 1), (2, 3), (4, 8)])
 def test_batch_s16(s16, lane_bits, contexts, burst):
     say(f"[gpu] batch s16 lanes={lane_bits} ctx={contexts} burst={burst}")
@@ -239,7 +239,7 @@ def _close_ref(depth):
     return np.where(reached, d, 0).sum(axis=1), reached.sum(axis=1)
 
 
-@pytest.mark.parametrize("lane_bits", [64, 128, 256])
+@pytest.mark.parametrize("lane_bits", [64, 128, 256, 512])
 def test_closeness_matches_depth_arrays_s16(s16, lane_bits):
     say(f"[gpu] closeness s16 lanes={lane_bits}")
     b = ibfs_amd.Batch(s16["g"], contexts=2, burst=4, lane_bits=lane_bits)
@@ -284,7 +284,7 @@ def _serve(server, queries, V, gap_s=0.0):
     return outs, infos
 
 
-@pytest.mark.parametrize("lane_bits", [64, 256])
+@pytest.mark.parametrize("lane_bits", [64, 256, 512])
 def test_server_mixed_equals_alone_s16(s16, lane_bits):
     say(f"[gpu] server s16 lanes={lane_bits}")
     rng = np.random.default_rng(3)
