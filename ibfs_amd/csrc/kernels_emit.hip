@@ -366,6 +366,12 @@ __global__ __launch_bounds__(BS) void k_groupby_place(const int32_t *__restrict_
 
 }  // namespace
 
+// One vertex per thread up to 16M vertices: fewer, longer-lived threads
+// leave the write stream short of stores in flight.
+#ifndef IBFS_EMIT_GRID_CAP
+#define IBFS_EMIT_GRID_CAP 65536
+#endif
+
 template <int W, int NS>
 static void emit_ns(int mode, int vec4, const u64 *ring, int R, EmitSpace sp, int b, int e,
                     const LaneOut *lanes, const u64 *masks, const u64 *sel, int k,
@@ -375,7 +381,7 @@ static void emit_ns(int mode, int vec4, const u64 *ring, int R, EmitSpace sp, in
     hipLaunchKernelGGL((k_emit_tiled<W, NS>), g, blk, 0, st, ring, R, sp, b, e, lanes, masks, sel,
                        k, vec4);
   } else {
-    dim3 g(grid_for(sp.V, BS, 8192)), blk(BS);
+    dim3 g(grid_for(sp.V, BS, IBFS_EMIT_GRID_CAP)), blk(BS);
     hipLaunchKernelGGL((k_emit<W, NS>), g, blk, 0, st, ring, R, sp, b, e, lanes, masks, sel, k,
                        mode == 2 ? 1 : 0);
   }

@@ -699,27 +699,34 @@ hipError_t launch_admit(int W, const int32_t *src, const int32_t *lane, int n, u
   return hipGetLastError();
 }
 
-// Vertices per lane and round set: the largest of 8, 4, 2 whose chunk
-// registers stay within 16 words per lane (8 for 128-lane words, 4 for 256,
-// 2 for 512) and that still gives 512 tiles of 512 threads, else 1; a.vpt
-// overrides (capped for wide words, which would spill registers).
+// Vertices per lane and round set: 8/W, at least 2 (8 for 64-lane words, 4
+// for 128, 2 for 256 and 512: chunk registers within 8 words per lane, and
+// tiles large enough that the per-tile reservation atomics stay rare), or
+// less while the graph has fewer than 512 such tiles; a.vpt overrides
+// (capped for wide words, which would spill registers).
 int inspect_vpt(const InspectArgs &a, int W) {
   if (a.vpt == 1 || a.vpt == 2 || a.vpt == 4 || a.vpt == 8 || a.vpt == 16) {
     int v = a.vpt;
     while (v > 2 && v * W > 16) v /= 2;
     return v;
   }
-  for (int v : {8, 4, 2})
-    if (v * W <= 16 && a.V >= (int64_t)512 * v * 512) return v;
-  return 1;
+  int v = 8 / W < 2 ? 2 : 8 / W;
+  while (v > 1 && a.V < (int64_t)512 * v * 512) v /= 2;
+  return v;
 }
+
+// A grid of 512 workgroups (two per CU) strides over the tiles: with one
+// workgroup per tile the last round of tiles runs on a half-empty part.
+#ifndef IBFS_INSPECT_GRID_CAP
+#define IBFS_INSPECT_GRID_CAP 512
+#endif
 
 template <int W, int VPT>
 static void inspect_launch(const InspectArgs &a, hipStream_t st) {
   constexpr int NW = 8;
   const int64_t tiles = (a.V + 64 * NW * VPT - 1) / (64 * NW * VPT);
-  hipLaunchKernelGGL((k_inspect<W, VPT, NW>), dim3(grid_for(tiles, 1, 8192)), dim3(64 * NW), 0,
-                     st, a);
+  hipLaunchKernelGGL((k_inspect<W, VPT, NW>), dim3(grid_for(tiles, 1, IBFS_INSPECT_GRID_CAP)),
+                     dim3(64 * NW), 0, st, a);
 }
 
 template <int W>
