@@ -318,7 +318,9 @@ int ibfs_batch_create(ibfs_graph_t g, const ibfs_config *cfg_in, int32_t context
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device) != hipSuccess ||
       cus <= 0)
     cus = 256;
-  b->grid = cfg.dd_grid > 0 ? cfg.dd_grid : cus * 8;
+  // 32 workgroups per CU: a fixed grid strides over the queues, and fewer
+  // workgroups leave too few gathers in flight (measured on R-MAT s22)
+  b->grid = cfg.dd_grid > 0 ? cfg.dd_grid : cus * 32;
   b->ctx.resize(contexts);
   hipError_t err = hipMalloc((void **)&b->d_edges, sizeof(u64));
   if (err == hipSuccess) err = hipHostMalloc((void **)&b->h_edges, sizeof(u64), hipHostMallocDefault);

@@ -376,7 +376,7 @@ int ibfs_config_default(ibfs_config *cfg) {
   cfg->alpha = 2;
   cfg->beta = 24;
   cfg->wg_size = 256;
-  cfg->td_chunk = 4096;
+  cfg->td_chunk = 2048;
   cfg->ring_levels = 32;
   cfg->kernel_timing = 0;
   cfg->device = 0;

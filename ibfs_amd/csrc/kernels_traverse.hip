@@ -393,11 +393,16 @@ __device__ __forceinline__ void td_chunk_range(const int64_t *__restrict__ beg, 
   hi = (lo + (1ll << cs)) < end ? (lo + (1ll << cs)) : end;
 }
 
+// neighbours gathered per step by one thread's bottom-up candidate
+#ifndef IBFS_BU_STEP
+#define IBFS_BU_STEP 4
+#endif
+
 // Bottom-up: a candidate v ORs S_{L-1} of its neighbours (restricted to
 // lam_{L-1}, the only lanes that can still be new) in traversal-adjacency
 // order (descending degree, hubs first) and stops as soon as its word
 // covers lam_{L-1} (bitwise early termination, expander.cu:258-359).
-// One candidate per thread, four neighbours per step.
+// One candidate per thread, IBFS_BU_STEP neighbours per step.
 template <int W>
 __device__ __forceinline__ void bu_thread(const int64_t *__restrict__ beg,
                                           const int32_t *__restrict__ tadj, const Gather &gs,
@@ -411,18 +416,18 @@ __device__ __forceinline__ void bu_thread(const int64_t *__restrict__ beg,
   int64_t j = beg[v];
   const int64_t e = beg[v + 1];
   while (j < e) {
-    const int32_t a0 = tadj[j];
-    const int32_t a1 = (j + 1 < e) ? tadj[j + 1] : a0;
-    const int32_t a2 = (j + 2 < e) ? tadj[j + 2] : a0;
-    const int32_t a3 = (j + 3 < e) ? tadj[j + 3] : a0;
-    u64 s0[W], s1[W], s2[W], s3[W];
-    load_nbr<W>(gs, a0, s0);
-    load_nbr<W>(gs, a1, s1);
-    load_nbr<W>(gs, a2, s2);
-    load_nbr<W>(gs, a3, s3);
+    int32_t an[IBFS_BU_STEP];
+    an[0] = tadj[j];
 #pragma unroll
-    for (int w = 0; w < W; ++w) x[w] |= (s0[w] | s1[w] | s2[w] | s3[w]) & lam[w];
-    j += 4;
+    for (int r = 1; r < IBFS_BU_STEP; ++r) an[r] = (j + r < e) ? tadj[j + r] : an[0];
+    u64 sn[IBFS_BU_STEP][W];
+#pragma unroll
+    for (int r = 0; r < IBFS_BU_STEP; ++r) load_nbr<W>(gs, an[r], sn[r]);
+#pragma unroll
+    for (int r = 0; r < IBFS_BU_STEP; ++r)
+#pragma unroll
+      for (int w = 0; w < W; ++w) x[w] |= sn[r][w] & lam[w];
+    j += IBFS_BU_STEP;
     if (covers<W>(x, lam)) break;
   }
   if (differs<W>(x, x0)) {
