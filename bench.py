@@ -144,8 +144,15 @@ def measured_traffic(workload, K, kernel):
     return None, None
 
 
+# batched-leg settings found by ibfs_autotune_batch on this workload (R-MAT s22,
+# the 1,024 GroupBy-ordered sources): 256 lanes, 2 contexts, bursts of 4,
+# lrg_shed 4096, beta 96
+BATCH_TUNED = {"lrg_shed": 4096, "beta": 96}
+
+
 def batch_leg(g, src, K, contexts, burst, steps, warmup, tune, emit_mode, out):
-    b = ibfs_amd.Batch(g, contexts=contexts, burst=burst, lane_bits=K, emit_mode=emit_mode, **tune)
+    opts = dict(BATCH_TUNED, **tune)
+    b = ibfs_amd.Batch(g, contexts=contexts, burst=burst, lane_bits=K, emit_mode=emit_mode, **opts)
     for _ in range(warmup):
         b.run(src, out=out)
     edges = 0
@@ -155,7 +162,7 @@ def batch_leg(g, src, K, contexts, burst, steps, warmup, tune, emit_mode, out):
         edges += b.last_stats.traversed_edges
     wall = time.perf_counter() - t0
     return dict(gteps=edges / wall / 1e9, ms_per_step=wall * 1e3 / steps, lane_bits=K,
-                contexts=contexts, burst=burst)
+                contexts=contexts, burst=burst, config=opts)
 
 
 def edges_of(g, sources, chunk=256):
@@ -253,7 +260,7 @@ def main() -> int:
     ap.add_argument("--inspect-vpt", type=int, default=None)
     ap.add_argument("--batch-contexts", type=int, default=2)
     ap.add_argument("--batch-lanes", type=int, default=256)
-    ap.add_argument("--burst", type=int, default=8)
+    ap.add_argument("--burst", type=int, default=4)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-queries", type=int, default=120)

@@ -320,9 +320,31 @@ def autotune(g: DeviceGraph, sample_sources, passes: int = 2, reps: int = 2,
     return best, report
 
 
+def autotune_batch(g: DeviceGraph, sample_sources, contexts: int = 2, burst: int = 8,
+                   passes: int = 2, reps: int = 2, start: Optional[Config] = None,
+                   save_to: Optional[str] = None) -> tuple[Config, int, int, dict]:
+    """Tunes the batched launcher for graph g on the sample sources; returns
+    (config, contexts, burst, report)."""
+    import json
+    s = np.ascontiguousarray(sample_sources, dtype=np.int32)
+    best = Config()
+    bc, bb = C.c_int32(), C.c_int32()
+    buf = C.create_string_buffer(1 << 20)
+    st = start if start is not None else default_config(device=g.device, lane_bits=256)
+    check(lib().ibfs_autotune_batch(g._h, _i32p(s), len(s), C.byref(st), contexts, burst, passes,
+                                    reps, C.byref(best), C.byref(bc), C.byref(bb), buf, len(buf)))
+    report = json.loads(buf.value.decode())
+    if save_to:
+        with open(save_to, "w") as f:
+            json.dump(report, f, indent=1)
+    return best, bc.value, bb.value, report
+
+
 def config_from_dict(d: dict, device: int = 0) -> Config:
-    """Config from the "best" record of an autotune report."""
-    return default_config(device=device, **{k: int(v) for k, v in d.items()})
+    """Config from the "best" record of an autotune report (the batched
+    launcher's contexts and burst, which are not config fields, are skipped)."""
+    return default_config(device=device, **{k: int(v) for k, v in d.items()
+                                            if k not in ("contexts", "burst")})
 
 
 class Batch:

@@ -137,6 +137,9 @@ SIGNATURES = {
     "ibfs_server_destroy": (C.c_int, [P]),
     "ibfs_autotune": (C.c_int, [P, I32P, C.c_int64, C.POINTER(Config), C.c_int32, C.c_int32,
                                 C.POINTER(Config), C.c_char_p, C.c_size_t]),
+    "ibfs_autotune_batch": (C.c_int, [P, I32P, C.c_int64, C.POINTER(Config), C.c_int32, C.c_int32,
+                                      C.c_int32, C.c_int32, C.POINTER(Config), I32P, I32P,
+                                      C.c_char_p, C.c_size_t]),
 }
 
 _lib = None

@@ -151,6 +151,7 @@ InspectArgs dd_inspect_args(const ibfs_batch_s *b, const Ctx &c) {
   a.lanes_all = c.d_masks;
   a.flags = c.flags;
   a.vpt = b->cfg.inspect_vpt;
+  a.nw = b->cfg.wg_size / 64;
   return a;
 }
 
@@ -296,6 +297,7 @@ int ibfs_batch_create(ibfs_graph_t g, const ibfs_config *cfg_in, int32_t context
     return fail(IBFS_ERR_ARG, "ring_levels must be in [4, 255]");
   if (cfg.emit_mode < 0 || cfg.emit_mode > 2) return fail(IBFS_ERR_ARG, "bad emit_mode");
   if (!valid_inspect_vpt(cfg.inspect_vpt)) return fail(IBFS_ERR_ARG, "bad inspect_vpt");
+  if (!valid_wg_size(cfg.wg_size)) return fail(IBFS_ERR_ARG, "wg_size must be 256, 512 or 1024");
   if (cfg.device != g->device) return fail(IBFS_ERR_ARG, "config device differs from the graph's");
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)

@@ -29,6 +29,8 @@ struct HostCsr {
 inline bool valid_inspect_vpt(int v) {
   return v == 0 || v == 1 || v == 2 || v == 4 || v == 8 || v == 16;
 }
+// inspector workgroup size of ibfs_config
+inline bool valid_wg_size(int v) { return v == 256 || v == 512 || v == 1024; }
 
 // Parallel helper: runs fn(t) on nthreads std::threads.
 void parallel_for_threads(int nthreads, void (*fn)(int, void *), void *ctx);

@@ -100,6 +100,7 @@ struct InspectArgs {
   uint8_t *flags;
   int vpt;              // vertices per lane and chunk (0: by size)
   int full_diff;
+  int nw;               // waves per workgroup: 4, 8 or 16 (0: 8)
 };
 
 // Launchers (all asynchronous on `st`).  W = status words (u64) per vertex.

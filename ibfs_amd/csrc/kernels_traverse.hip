@@ -707,16 +707,19 @@ hipError_t launch_admit(int W, const int32_t *src, const int32_t *lane, int n, u
 // Vertices per lane and round set: 8/W, at least 2 (8 for 64-lane words, 4
 // for 128, 2 for 256 and 512: chunk registers within 8 words per lane, and
 // tiles large enough that the per-tile reservation atomics stay rare), or
-// less while the graph has fewer than 512 such tiles; a.vpt overrides
-// (capped for wide words, which would spill registers).
-int inspect_vpt(const InspectArgs &a, int W) {
+// less while the graph has fewer than 512 such tiles.  1,024-thread
+// workgroups have half the registers per lane, so their words per lane are
+// halved.  a.vpt overrides within the same register caps.
+int inspect_vpt(const InspectArgs &a, int W, int NW) {
+  const int cap = NW >= 16 ? 4 : 8;  // status words of chunk registers per lane
+  int v;
   if (a.vpt == 1 || a.vpt == 2 || a.vpt == 4 || a.vpt == 8 || a.vpt == 16) {
-    int v = a.vpt;
-    while (v > 2 && v * W > 16) v /= 2;
+    v = a.vpt > 8 ? 8 : a.vpt;
+    while (v > 1 && v * W > 2 * cap) v /= 2;
     return v;
   }
-  int v = 8 / W < 2 ? 2 : 8 / W;
-  while (v > 1 && a.V < (int64_t)512 * v * 512) v /= 2;
+  v = cap / W < 2 ? (NW >= 16 ? 1 : 2) : cap / W;
+  while (v > 1 && a.V < (int64_t)64 * NW * v * 512) v /= 2;
   return v;
 }
 
@@ -726,23 +729,30 @@ int inspect_vpt(const InspectArgs &a, int W) {
 #define IBFS_INSPECT_GRID_CAP 512
 #endif
 
-template <int W, int VPT>
+template <int W, int VPT, int NW>
 static void inspect_launch(const InspectArgs &a, hipStream_t st) {
-  constexpr int NW = 8;
   const int64_t tiles = (a.V + 64 * NW * VPT - 1) / (64 * NW * VPT);
   hipLaunchKernelGGL((k_inspect<W, VPT, NW>), dim3(grid_for(tiles, 1, IBFS_INSPECT_GRID_CAP)),
                      dim3(64 * NW), 0, st, a);
 }
 
+template <int W, int NW>
+static void inspect_nw(const InspectArgs &a, hipStream_t st) {
+  switch (inspect_vpt(a, W, NW)) {
+    case 8: inspect_launch<W, 8, NW>(a, st); break;
+    case 4: inspect_launch<W, 4, NW>(a, st); break;
+    case 2: inspect_launch<W, 2, NW>(a, st); break;
+    default: inspect_launch<W, 1, NW>(a, st); break;
+  }
+}
+
 template <int W>
 static void inspect_w(const InspectArgs &a, hipStream_t st) {
-  switch (inspect_vpt(a, W)) {
-    case 16: inspect_launch<W, 16>(a, st); break;
-    case 8: inspect_launch<W, 8>(a, st); break;
-    case 4: inspect_launch<W, 4>(a, st); break;
-    case 2: inspect_launch<W, 2>(a, st); break;
-    default: inspect_launch<W, 1>(a, st); break;
-  }
+  // 1,024-thread workgroups are not used for 512-lane words (register spills)
+  const int nw = (a.nw == 16 && W < 8) ? 16 : (a.nw == 4 ? 4 : 8);
+  if (nw == 16) inspect_nw<W, 16>(a, st);
+  else if (nw == 4) inspect_nw<W, 4>(a, st);
+  else inspect_nw<W, 8>(a, st);
 }
 
 hipError_t launch_inspect(int W, const InspectArgs &a, hipStream_t st) {

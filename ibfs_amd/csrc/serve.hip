@@ -276,6 +276,7 @@ int inspect_level(ibfs_server_s *s, SCtx &c, int L, const u64 *filter, const u64
   a.hs = c.hs;
   a.flags = c.flags;
   a.vpt = s->cfg.inspect_vpt;
+  a.nw = s->cfg.wg_size / 64;
   a.full_diff = full_diff;
   HIP_CHK(launch_inspect(W, a, c.st));
   HIP_CHK(hipMemcpyAsync(c.h_ctl, c.ctl + L % s->R, sizeof(LevelCtl), hipMemcpyDeviceToHost, c.st));
@@ -490,6 +491,7 @@ int ibfs_server_create(ibfs_graph_t g, const ibfs_config *cfg_in, int64_t status
     return fail(IBFS_ERR_ARG, "td_chunk must be a power of two >= 256");
   if (cfg.emit_mode < 0 || cfg.emit_mode > 2) return fail(IBFS_ERR_ARG, "bad emit_mode");
   if (!valid_inspect_vpt(cfg.inspect_vpt)) return fail(IBFS_ERR_ARG, "bad inspect_vpt");
+  if (!valid_wg_size(cfg.wg_size)) return fail(IBFS_ERR_ARG, "wg_size must be 256, 512 or 1024");
   if (cfg.device != g->device) return fail(IBFS_ERR_ARG, "config device differs from the graph's");
   int nd = 0;
   if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0)

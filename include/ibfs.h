@@ -47,9 +47,10 @@ typedef struct ibfs_engine_s *ibfs_engine_t;
 typedef struct ibfs_hostcsr_s *ibfs_hostcsr_t;
 typedef struct ibfs_server_s *ibfs_server_t;
 
-/* Engine configuration.  Defaults (ibfs_config_default) follow the reference:
- * 128 lanes (concurrent-count, readme.md:27), sml_shed=32 and lrg_shed=512
- * (main.cu:56-57). */
+/* Engine configuration.  Defaults (ibfs_config_default): 128 lanes
+ * (concurrent-count, readme.md:27), lrg_shed=512 as the reference
+ * (main.cu:56-57), sml_shed=64 where the reference has 32 (main.cu:56): the
+ * thread/wave class boundary follows the 64-wide wavefront. */
 typedef struct ibfs_config {
   int32_t lane_bits;   /* 64, 128, 256 or 512 source lanes per status word     */
   int32_t sml_shed;    /* degree < sml_shed        -> thread class             */
@@ -58,7 +59,7 @@ typedef struct ibfs_config {
   int32_t sw_level;    /* last top-down level for IBFS_POLICY_FIXED (argv[4])   */
   int32_t alpha;       /* TD->BU when m_f * alpha > m_u   (ALPHA_BETA)          */
   int32_t beta;        /* BU->TD when n_f * beta < V      (ALPHA_BETA)          */
-  int32_t wg_size;     /* threads per workgroup of the expanders: 256/512/1024 */
+  int32_t wg_size;     /* threads per workgroup of the inspector: 256/512/1024 */
   int32_t td_chunk;    /* edges per workgroup chunk of a top-down hub          */
   int32_t ring_levels; /* status snapshots kept on the device (>= 4)           */
   int32_t kernel_timing; /* 1: time every kernel with HIP events             */
@@ -284,6 +285,14 @@ int ibfs_server_destroy(ibfs_server_t s);
 int ibfs_autotune(ibfs_graph_t g, const int32_t *src, int64_t n,
                   const ibfs_config *start, int32_t passes, int32_t reps,
                   ibfs_config *best, char *report, size_t report_len);
+/* The same search through the batched launcher (ibfs_batch_*), which adds
+ * the device-driven expander's workgroup count (dd_grid), the number of
+ * contexts and the burst length; contexts and burst are in/out. */
+int ibfs_autotune_batch(ibfs_graph_t g, const int32_t *src, int64_t n,
+                        const ibfs_config *start, int32_t contexts, int32_t burst,
+                        int32_t passes, int32_t reps, ibfs_config *best,
+                        int32_t *best_contexts, int32_t *best_burst, char *report,
+                        size_t report_len);
 
 /* TEPS numerator of one depth array: sum of deg(v) over reached v
  * (validate.h:59-76 report()). depth is a host array. */

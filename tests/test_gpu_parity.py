@@ -326,10 +326,33 @@ def test_autotune_returns_a_valid_faster_or_equal_config(s16, tmp_path):
     cfg, rep = ibfs_amd.autotune(s16["g"], s16["src"][:128], passes=1, reps=1,
                                  save_to=str(tmp_path / "tune.json"))
     assert rep["best_gteps"] >= rep["trials"][0]["gteps"]
-    assert cfg.lane_bits in (64, 128, 256)
+    assert cfg.lane_bits in (64, 128, 256, 512)
     eng = ibfs_amd.Engine(s16["g"], cfg)
     np.testing.assert_array_equal(eng.run_groups(s16["src"]), s16["want"])
     assert (tmp_path / "tune.json").exists()
+
+
+def test_autotune_batch_returns_a_valid_config(s16, tmp_path):
+    say("[gpu] autotune batch")
+    src = s16["src"][:64]
+    cfg, ctx, burst, rep = ibfs_amd.autotune_batch(s16["g"], src, passes=1, reps=1,
+                                                   save_to=str(tmp_path / "tune.json"))
+    assert rep["mode"] == "batch" and rep["best_gteps"] >= rep["trials"][0]["gteps"]
+    assert 1 <= ctx <= 4 and burst in (4, 8, 16) and cfg.wg_size in (256, 512, 1024)
+    b = ibfs_amd.Batch(s16["g"], contexts=ctx, burst=burst, cfg=cfg)
+    np.testing.assert_array_equal(b.run(src), s16["want"][:64])
+    back = ibfs_amd.config_from_dict(rep["best"])
+    assert back.dd_grid == cfg.dd_grid and back.lane_bits == cfg.lane_bits
+
+
+@pytest.mark.parametrize("wg", [256, 1024])
+def test_inspector_workgroup_sizes(s16, wg):
+    say(f"[gpu] wg_size={wg}")
+    for lb in (64, 128, 256, 512):
+        eng = ibfs_amd.Engine(s16["g"], lane_bits=lb, wg_size=wg)
+        np.testing.assert_array_equal(eng.run_groups(s16["src"]), s16["want"])
+    b = ibfs_amd.Batch(s16["g"], contexts=2, burst=4, lane_bits=256, wg_size=wg)
+    np.testing.assert_array_equal(b.run(s16["src"]), s16["want"])
 
 
 def test_device_output_and_determinism(s16):
