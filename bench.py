@@ -182,6 +182,7 @@ def served_leg(graphs, rates, n_queries, seed, K, admit_window=None):
     rng = np.random.default_rng(seed)
     servers = [ibfs_amd.Server(gr["g"], lane_bits=K, status_bytes=gr["status_bytes"],
                                admit_window=admit_window) for gr in graphs]
+    checkers = [ibfs_amd.Batch(gr["g"], contexts=1, burst=8, lane_bits=256) for gr in graphs]
     results = []
     for lam in rates:
         gi = rng.integers(0, len(graphs), size=n_queries)
@@ -216,6 +217,16 @@ def served_leg(graphs, rates, n_queries, seed, K, admit_window=None):
         tc.start()
         tp.join()
         tc.join()
+        # untimed check: served depth arrays equal the batched form's for the
+        # first queries of the rate
+        verified = 0
+        for i in range(min(3, n_queries)):
+            gr = graphs[gi[i]]
+            want = checkers[gi[i]].run(qs[i])
+            got = bufs[i].to_host(count=len(qs[i]) * gr["g"].vert_count).reshape(want.shape)
+            if not np.array_equal(got, want):
+                raise RuntimeError(f"served query {i} differs from the batched form")
+            verified += 1
         for b in bufs:
             b.free()
         t_first = min(x["submit_ms"] for x in lat)
@@ -232,7 +243,8 @@ def served_leg(graphs, rates, n_queries, seed, K, admit_window=None):
                         "p50_ms": round(float(np.percentile(lats, 50)), 3),
                         "p99_ms": round(float(np.percentile(lats, 99)), 3),
                         "mean_ms": round(float(lats.mean()), 3),
-                        "gteps": round(edges / span_s / 1e9, 3), "span_ms": round(span_s * 1e3, 2)})
+                        "gteps": round(edges / span_s / 1e9, 3), "span_ms": round(span_s * 1e3, 2),
+                        "verified_queries": verified})
         log(f"served lambda={lam}/s: p50 {results[-1]['p50_ms']} ms p99 {results[-1]['p99_ms']} ms "
             f"{results[-1]['gteps']} GTEPS over {results[-1]['span_ms']} ms")
     info = {"graphs": [gr["name"] for gr in graphs], "lane_bits": K,

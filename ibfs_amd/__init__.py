@@ -145,10 +145,15 @@ class DeviceBuffer:
         self.ptr = p.value
         self.nbytes = nbytes
 
-    def to_host(self, dtype=np.int32, count: Optional[int] = None) -> np.ndarray:
-        n = self.nbytes // np.dtype(dtype).itemsize if count is None else count
+    def to_host(self, dtype=np.int32, count: Optional[int] = None, offset: int = 0) -> np.ndarray:
+        """Copies `count` elements starting at element `offset` to the host."""
+        item = np.dtype(dtype).itemsize
+        n = (self.nbytes // item - offset) if count is None else count
+        if offset < 0 or n < 0 or (offset + n) * item > self.nbytes:
+            raise ValueError("range outside the buffer")
         out = np.empty(n, dtype=dtype)
-        check(lib().ibfs_memcpy_d2h(out.ctypes.data, C.c_void_p(self.ptr), out.nbytes))
+        check(lib().ibfs_memcpy_d2h(out.ctypes.data, C.c_void_p(self.ptr + offset * item),
+                                    out.nbytes))
         return out
 
     def copy_from(self, other: "DeviceBuffer", nbytes: Optional[int] = None) -> None:
