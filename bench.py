@@ -276,7 +276,7 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-queries", type=int, default=120)
-    ap.add_argument("--serve-rates", default="200,800,2000")
+    ap.add_argument("--serve-rates", default="200,800,2000,4000")
     ap.add_argument("--admit-window", type=int, default=None)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device 4 times (HBM counter yardstick)")

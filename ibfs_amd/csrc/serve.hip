@@ -65,7 +65,9 @@ struct PendingSrc {
   int idx;
 };
 
-enum LaneState { CLEAN = 0, ACTIVE = 1, DIRTY = 2 };
+// CLEAN: free; ACTIVE: its BFS runs; FINISHED: its BFS ended, depths not yet
+// emitted (its bits stay in the ring); DIRTY: freed, bits being cleared.
+enum LaneState { CLEAN = 0, ACTIVE = 1, FINISHED = 2, DIRTY = 3 };
 
 struct Lane {
   int state = CLEAN;
@@ -338,17 +340,30 @@ int process_level(ibfs_server_s *s, SCtx &c) {
     std::lock_guard<std::mutex> lk(s->mu);
     mark_done(s, c);  // emissions issued before this level's event are complete
   }
-  // 1. lanes whose BFS ended: emit them from the earliest admission among
-  //    them (the window base if one is older), free them
+  // 1. lanes whose BFS ended finish.  With admission at cohort starts only,
+  //    their depths are emitted once no lane of the cohort is still active
+  //    (one pass over the ring window per cohort instead of one per level at
+  //    which lanes finish; a finished lane's bits no longer change, so a
+  //    longer window gives the same depths).  Otherwise they are emitted now.
+  //    Emission starts at the earliest admission among them (the window base
+  //    if one is older); emitted lanes are freed.
+  bool active_left = false;
+  for (int k = 0; k < s->K; ++k) {
+    Lane &l = c.lanes[k];
+    if (l.state == ACTIVE && l.admit < L && !has_bit(lc.lam, k)) l.state = FINISHED;
+    active_left |= (l.state == ACTIVE);
+  }
   std::vector<int> fin;
   std::vector<u64> finmask(W, 0);
   int fin_from = L;
-  for (int k = 0; k < s->K; ++k) {
-    Lane &l = c.lanes[k];
-    if (l.state == ACTIVE && l.admit < L && !has_bit(lc.lam, k)) {
-      fin.push_back(k);
-      set_bit(finmask.data(), k);
-      fin_from = std::min(fin_from, l.admit);
+  if (!(s->admit_window == 0 && active_left)) {
+    for (int k = 0; k < s->K; ++k) {
+      Lane &l = c.lanes[k];
+      if (l.state == FINISHED) {
+        fin.push_back(k);
+        set_bit(finmask.data(), k);
+        fin_from = std::min(fin_from, l.admit);
+      }
     }
   }
   int rc = emit_lanes(s, c, fin, std::max(c.base, fin_from), L - 1, 0);
@@ -366,7 +381,7 @@ int process_level(ibfs_server_s *s, SCtx &c) {
     }
     for (auto &l : c.lanes) {
       if (l.state == DIRTY && l.freed_at <= L - 1) l.state = CLEAN;  // cleared in S_{L+1}
-      any_active |= (l.state == ACTIVE);
+      any_active |= (l.state == ACTIVE || l.state == FINISHED);
     }
     if (!any_active && s->pending.empty()) {
       c.running = false;
@@ -387,20 +402,21 @@ int process_level(ibfs_server_s *s, SCtx &c) {
                              c.flags, lamp, c.q, lc.q[Q_BU_S], lc.q[Q_BU_M], lc.q[Q_BU_L], c.st));
   c.dir_prev = dir;
   c.nf_prev = lc.n_front;
-  // 3. snapshots older than every active lane's admission are no longer
-  //    needed: the window base follows the oldest active lane.  If the next
-  //    copy (slot Ln+1) would still overwrite the base, flush all active
-  //    lanes over [base, Ln].
+  // 3. snapshots older than every unemitted lane's admission are no longer
+  //    needed: the window base follows the oldest active or finished lane.
+  //    If the next copy (slot Ln+1) would still overwrite the base, flush all
+  //    of them over [base, Ln].
   {
     int oldest = Ln;
     for (int k = 0; k < s->K; ++k)
-      if (c.lanes[k].state == ACTIVE) oldest = std::min(oldest, c.lanes[k].admit);
+      if (c.lanes[k].state == ACTIVE || c.lanes[k].state == FINISHED)
+        oldest = std::min(oldest, c.lanes[k].admit);
     c.base = std::max(c.base, oldest);
   }
   if ((Ln + 1) - c.base >= s->R) {
     std::vector<int> act;
     for (int k = 0; k < s->K; ++k)
-      if (c.lanes[k].state == ACTIVE) act.push_back(k);
+      if (c.lanes[k].state == ACTIVE || c.lanes[k].state == FINISHED) act.push_back(k);
     rc = emit_lanes(s, c, act, c.base, Ln, 1);
     if (rc) return rc;
     c.base = Ln;
