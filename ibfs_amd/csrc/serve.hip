@@ -94,6 +94,14 @@ struct SCtx {
   int cohort = 0;  // level at which the current cohort's first sources were admitted
   u64 nf_prev = ~0ull;
   std::vector<int64_t> done_pending;
+  // Device-driven cohorts (admission at cohort starts only): a captured level
+  // graph (expand, inspect, direction decision) replayed in bursts.
+  Plan *d_plan = nullptr, *h_plan = nullptr;
+  u64 *d_all = nullptr, *h_all = nullptr;  // lanes of the cohort (lam of level 0)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int coh = 0;     // 0: host-driven levels, 1: cohort levels on the device, 2: cohort emitting
+  int dlevel = 0;  // cohort level the device expands next (host view)
 };
 
 template <class T>
@@ -128,7 +136,12 @@ struct ibfs_server_s {
   // sources join a running context only within this many levels of its
   // cohort's start, so that the lanes of a context stay close in phase
   int admit_window = 0;
+  int grid = 2048;  // workgroups of the device-driven expander
 };
+
+namespace {
+constexpr int kCohortBurst = 4;  // level graphs per host poll in cohort mode
+}
 
 namespace {
 
@@ -142,6 +155,14 @@ u64 *slot(const ibfs_server_s *s, const SCtx &c, int L) {
 
 void release_ctx(SCtx &c) {
   if (c.st) (void)hipStreamSynchronize(c.st);
+  if (c.exec) (void)hipGraphExecDestroy(c.exec);
+  if (c.graph) (void)hipGraphDestroy(c.graph);
+  c.exec = nullptr;
+  c.graph = nullptr;
+  dfree(c.d_plan);
+  hfree(c.h_plan);
+  dfree(c.d_all);
+  hfree(c.h_all);
   dfree(c.ring);
   dfree(c.ctl);
   hfree(c.h_ctl);
@@ -159,6 +180,30 @@ void release_ctx(SCtx &c) {
   c.ev = nullptr;
   if (c.st) (void)hipStreamDestroy(c.st);
   c.st = nullptr;
+}
+
+// Inspector arguments of a cohort level (plan mode: the level, its slots and
+// control blocks come from the plan on the device).
+InspectArgs plan_args(const ibfs_server_s *s, const SCtx &c) {
+  InspectArgs a{};
+  a.deg = s->g->d_deg;
+  a.V = s->g->Vt;
+  a.q = c.q;
+  a.sml = s->cfg.sml_shed;
+  a.lrg = s->cfg.lrg_shed;
+  a.chunk_shift = s->chunk_shift;
+  a.hubs = s->g->d_hubs;
+  a.H = s->g->H;
+  a.hs = c.hs;
+  a.plan = c.d_plan;
+  a.ring = c.ring;
+  a.R = s->R;
+  a.ctl_ring = c.ctl;
+  a.lanes_all = c.d_all;
+  a.flags = c.flags;
+  a.vpt = s->cfg.inspect_vpt;
+  a.nw = s->cfg.wg_size / 64;
+  return a;
 }
 
 int alloc_ctx(ibfs_server_s *s, SCtx &c) {
@@ -184,12 +229,28 @@ int alloc_ctx(ibfs_server_s *s, SCtx &c) {
   H((void **)&c.h_adm, sizeof(int32_t) * 2 * (size_t)s->K);
   A((void **)&c.d_lanes, sizeof(LaneOut) * (size_t)s->K);
   H((void **)&c.h_lanes, sizeof(LaneOut) * (size_t)s->K);
+  A((void **)&c.d_plan, sizeof(Plan));
+  H((void **)&c.h_plan, sizeof(Plan));
+  A((void **)&c.d_all, sizeof(u64) * (size_t)s->W);
+  H((void **)&c.h_all, sizeof(u64) * (size_t)s->W);
   if (err == hipSuccess) err = hipEventCreateWithFlags(&c.ev, hipEventDisableTiming);
   if (err != hipSuccess)
     return fail(err == hipErrorOutOfMemory ? IBFS_ERR_NOMEM : IBFS_ERR_HIP,
                 "server context allocation failed: %s", hipGetErrorString(err));
   std::memset(c.h_lanes, 0, sizeof(LaneOut) * (size_t)s->K);
   c.lanes.assign(s->K, Lane{});
+  // one cohort level: device-driven expansion, inspection, direction decision
+  HIP_CHK(hipStreamBeginCapture(c.st, hipStreamCaptureModeThreadLocal));
+  hipError_t e1 = launch_expand_dd(s->W, s->g->d_beg, s->g->d_adj, s->g->d_tadj, c.ring, c.flags,
+                                   s->R, s->g->Vt, c.ctl, c.d_plan, c.hs, c.q, s->chunk_shift,
+                                   s->grid, c.st);
+  hipError_t e2 = launch_inspect(s->W, plan_args(s, c), c.st);
+  if (e2 == hipSuccess) e2 = launch_plan_decide(c.d_plan, c.ctl, s->R, c.st);
+  hipError_t e3 = hipStreamEndCapture(c.st, &c.graph);
+  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+    return fail(IBFS_ERR_HIP, "cohort level capture failed: %s / %s / %s", hipGetErrorString(e1),
+                hipGetErrorString(e2), hipGetErrorString(e3));
+  HIP_CHK(hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0));
   return IBFS_OK;
 }
 
@@ -443,12 +504,98 @@ int process_level(ibfs_server_s *s, SCtx &c) {
   return inspect_level(s, c, Ln, filter.data(), keep.data(), nand.data(), dir == 0 ? 1 : 0);
 }
 
+// Launches up to kCohortBurst level graphs of the cohort (bounded so that no
+// snapshot of the emission window is overwritten; flushes the window first
+// when it is full), then a copy of the plan and the context's event.
+int cohort_burst(ibfs_server_s *s, SCtx &c) {
+  int B = std::min(kCohortBurst, s->R - 1 - (c.dlevel - c.base));
+  if (B < 1) {
+    std::vector<int> act;
+    for (int k = 0; k < s->K; ++k)
+      if (c.lanes[k].state == ACTIVE) act.push_back(k);
+    int rc = emit_lanes(s, c, act, c.base, c.dlevel - 1, 1);
+    if (rc) return rc;
+    c.base = c.dlevel - 1;
+    B = std::min(kCohortBurst, s->R - 1 - (c.dlevel - c.base));
+  }
+  for (int i = 0; i < B; ++i) HIP_CHK(hipGraphLaunch(c.exec, c.st));
+  HIP_CHK(hipMemcpyAsync(c.h_plan, c.d_plan, offsetof(Plan, dirlog), hipMemcpyDeviceToHost, c.st));
+  HIP_CHK(hipEventRecord(c.ev, c.st));
+  ++s->levels_issued;
+  return IBFS_OK;
+}
+
+// Starts a cohort on an idle context (caller holds the lock): level 0 holds
+// up to K waiting sources; its levels then run on the device.
+int start_cohort(ibfs_server_s *s, SCtx &c) {
+  const int W = s->W;
+  for (auto &l : c.lanes) l = Lane{};
+  HIP_CHK(launch_fill_u64(slot(s, c, 0), s->g->Vt * W, 0ull, c.st));
+  HIP_CHK(launch_fill_u64(reinterpret_cast<u64 *>(c.flags), (s->g->Vt + 7) / 8, 0ull, c.st));
+  HIP_CHK(launch_fill_u64(reinterpret_cast<u64 *>(c.ctl), (int64_t)s->R * CTL_WORDS, 0ull, c.st));
+  std::memset(c.h_all, 0, sizeof(u64) * W);
+  int n = 0;
+  int rc = admit(s, c, 0, c.h_all, &n);
+  if (rc) return rc;
+  HIP_CHK(hipMemcpyAsync(c.d_all, c.h_all, sizeof(u64) * W, hipMemcpyHostToDevice, c.st));
+  std::memset(c.h_plan, 0, sizeof(Plan));
+  c.h_plan->policy = s->cfg.policy;
+  c.h_plan->sw_level = s->cfg.sw_level;
+  c.h_plan->alpha = s->cfg.alpha;
+  c.h_plan->beta = s->cfg.beta;
+  c.h_plan->nf_prev = ~0ull;
+  c.h_plan->V = (u64)s->g->V;
+  c.h_plan->dir = -1;
+  HIP_CHK(hipMemcpyAsync(c.d_plan, c.h_plan, sizeof(Plan), hipMemcpyHostToDevice, c.st));
+  HIP_CHK(launch_inspect(W, plan_args(s, c), c.st));
+  HIP_CHK(launch_plan_decide(c.d_plan, c.ctl, s->R, c.st));
+  c.coh = 1;
+  c.dlevel = 1;
+  c.base = 0;
+  c.running = true;
+  return cohort_burst(s, c);
+}
+
+// A cohort's event completed: more levels, the emission of all its lanes once
+// the plan reports that no vertex changed, or (after the emission) completion
+// of its queries.
+int cohort_step(ibfs_server_s *s, SCtx &c) {
+  if (c.coh == 1) {
+    if (c.h_plan->done) {
+      std::vector<int> act;
+      for (int k = 0; k < s->K; ++k)
+        if (c.lanes[k].state == ACTIVE) act.push_back(k);
+      int rc = emit_lanes(s, c, act, c.base, c.h_plan->last, 0);
+      if (rc) return rc;
+      HIP_CHK(hipEventRecord(c.ev, c.st));
+      c.coh = 2;
+      return IBFS_OK;
+    }
+    c.dlevel = c.h_plan->level;
+    if (c.dlevel > s->g->Vt + 2) return fail(IBFS_ERR_STATE, "cohort exceeded V+2 levels");
+    return cohort_burst(s, c);
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  for (auto &l : c.lanes) {
+    if (l.state != ACTIVE) continue;
+    Query &q = s->queries[l.qid];
+    if (--q.remaining == 0) c.done_pending.push_back(l.qid);
+    l = Lane{};
+  }
+  mark_done(s, c);
+  c.coh = 0;
+  c.running = false;
+  return IBFS_OK;
+}
+
 void worker_main(ibfs_server_s *s) {
   (void)hipSetDevice(s->g->device);
   while (true) {
     {
       std::unique_lock<std::mutex> lk(s->mu);
-      bool busy = false;
+      // pending sources start an idle context at once (a cohort that ended
+      // leaves its context idle while sources still wait)
+      bool busy = !s->pending.empty();
       for (auto &c : s->ctx) busy |= c.running;
       if (!busy)
         s->cv_work.wait(lk, [&] { return s->stop || !s->incoming.empty(); });
@@ -466,14 +613,14 @@ void worker_main(ibfs_server_s *s) {
         {
           std::lock_guard<std::mutex> lk(s->mu);
           have = !s->pending.empty();
-          if (have) rc = start_ctx(s, c);
+          if (have) rc = s->admit_window == 0 ? start_cohort(s, c) : start_ctx(s, c);
         }
         progressed |= have;
       } else {
         hipError_t q = hipEventQuery(c.ev);
         if (q == hipErrorNotReady) continue;
         if (q != hipSuccess) rc = fail(IBFS_ERR_HIP, "server event: %s", hipGetErrorString(q));
-        else rc = process_level(s, c);
+        else rc = c.coh ? cohort_step(s, c) : process_level(s, c);
         progressed = true;
       }
       if (rc) {
@@ -530,6 +677,11 @@ int ibfs_server_create(ibfs_graph_t g, const ibfs_config *cfg_in, int64_t status
   s->R = cfg.ring_levels;
   s->C = C;
   s->chunk_shift = __builtin_ctz((unsigned)cfg.td_chunk);
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device) != hipSuccess ||
+      cus <= 0)
+    cus = 256;
+  s->grid = cfg.dd_grid > 0 ? cfg.dd_grid : cus * 32;
   s->ctx.resize(C);
   for (auto &c : s->ctx) {
     int rc = alloc_ctx(s, c);
