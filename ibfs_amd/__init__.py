@@ -14,12 +14,13 @@ import numpy as np
 
 from . import _lib
 from ._lib import (IbfsError, OUT_DEVICE, OUT_HOST, POLICY_ALPHA_BETA, POLICY_BU_ONLY,
-                   POLICY_FIXED, POLICY_TD_ONLY, Config, LevelInfo, Stats, check, lib)
+                   POLICY_FIXED, POLICY_HYBRID, POLICY_TD_ONLY, Config, LevelInfo, Stats, check,
+                   lib)
 
 __all__ = [
     "IbfsError", "HostCsr", "DeviceGraph", "Engine", "DeviceBuffer", "graph", "gpu_ibfs",
     "default_config", "device_count", "POLICY_FIXED", "POLICY_ALPHA_BETA", "POLICY_TD_ONLY",
-    "POLICY_BU_ONLY",
+    "POLICY_BU_ONLY", "POLICY_HYBRID",
 ]
 
 
@@ -292,6 +293,7 @@ class Engine:
                 "q_td": list(li.q_td), "q_bu": list(li.q_bu),
                 "bu_work": li.bu_work, "bu_prefix": li.bu_prefix,
                 "expand_ms": li.expand_ms, "inspect_ms": li.inspect_ms,
+                "m_frontier_nonhub": li.m_frontier_nonhub,
             })
         return out
 

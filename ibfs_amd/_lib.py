@@ -23,6 +23,7 @@ POLICY_FIXED = 0
 POLICY_ALPHA_BETA = 1
 POLICY_TD_ONLY = 2
 POLICY_BU_ONLY = 3
+POLICY_HYBRID = 4
 
 OUT_HOST = 0
 OUT_DEVICE = 1
@@ -47,7 +48,7 @@ class Config(C.Structure):
         ("beta", C.c_int32), ("wg_size", C.c_int32), ("td_chunk", C.c_int32),
         ("ring_levels", C.c_int32), ("kernel_timing", C.c_int32), ("device", C.c_int32),
         ("work_model", C.c_int32), ("emit_mode", C.c_int32), ("dd_grid", C.c_int32),
-        ("inspect_vpt", C.c_int32),
+        ("inspect_vpt", C.c_int32), ("hybrid_gamma", C.c_int32),
     ]
 
 
@@ -57,7 +58,7 @@ class LevelInfo(C.Structure):
         ("n_frontier", C.c_int64), ("m_frontier", C.c_int64), ("n_unvisited", C.c_int64),
         ("m_unvisited", C.c_int64), ("q_td", C.c_int64 * 3), ("q_bu", C.c_int64 * 3),
         ("bu_work", C.c_int64), ("bu_prefix", C.c_int64),
-        ("expand_ms", C.c_float), ("inspect_ms", C.c_float),
+        ("expand_ms", C.c_float), ("inspect_ms", C.c_float), ("m_frontier_nonhub", C.c_int64),
     ]
 
 
@@ -77,6 +78,7 @@ class Stats(C.Structure):
         ("expand_bu_ms", C.c_double), ("inspect_ms", C.c_double), ("emit_ms", C.c_double),
         ("n_td_launches", C.c_int32), ("n_bu_launches", C.c_int32),
         ("n_inspect_launches", C.c_int32), ("n_emit_launches", C.c_int32),
+        ("expand_hy_ms", C.c_double), ("n_hy_launches", C.c_int32), ("reserved", C.c_int32),
     ]
 
 

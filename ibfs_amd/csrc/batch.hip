@@ -152,6 +152,7 @@ InspectArgs dd_inspect_args(const ibfs_batch_s *b, const Ctx &c) {
   a.flags = c.flags;
   a.vpt = b->cfg.inspect_vpt;
   a.nw = b->cfg.wg_size / 64;
+  a.hsplit = HubSplit{b->g->hub_deg, b->g->hub_tie};
   return a;
 }
 
@@ -188,7 +189,8 @@ int alloc_ctx(ibfs_batch_s *b, Ctx &c) {
   HIP_TRY(hipStreamBeginCapture(c.st, hipStreamCaptureModeThreadLocal));
   hipError_t e1 = launch_expand_dd(b->W, b->g->d_beg, b->g->d_adj, b->g->d_tadj, c.ring, c.flags,
                                    b->R, Vt,
-                                   c.ctl, c.d_plan, c.hs, c.q, b->chunk_shift, b->grid, c.st);
+                                   c.ctl, c.d_plan, c.hs, c.q, b->chunk_shift,
+                                   HubSplit{b->g->hub_deg, b->g->hub_tie}, b->grid, c.st);
   hipError_t e2 = launch_inspect(b->W, dd_inspect_args(b, c), c.st);
   if (e2 == hipSuccess) e2 = launch_plan_decide(c.d_plan, c.ctl, b->R, c.st);
   hipError_t e3 = hipStreamEndCapture(c.st, &c.graph);
@@ -214,14 +216,7 @@ int start_group(ibfs_batch_s *b, Ctx &c, const int32_t *src, int64_t g0, int k, 
   }
   for (int w = 0; w < W; ++w) c.h_masks[W + w] = c.h_masks[w];
   for (size_t i = 0; i < (size_t)(b->R + 1) * W; ++i) c.h_masks[2 * W + i] = ~0ull;
-  std::memset(c.h_plan, 0, sizeof(Plan));
-  c.h_plan->policy = b->cfg.policy;
-  c.h_plan->sw_level = b->cfg.sw_level;
-  c.h_plan->alpha = b->cfg.alpha;
-  c.h_plan->beta = b->cfg.beta;
-  c.h_plan->nf_prev = ~0ull;
-  c.h_plan->V = (u64)V;
-  c.h_plan->dir = -1;
+  plan_init(c.h_plan, b->cfg, b->g->rho_q10, (u64)V);
   HIP_TRY(hipMemcpyAsync(c.d_masks, c.h_masks, sizeof(u64) * masks_words(b), hipMemcpyHostToDevice, c.st));
   HIP_TRY(hipMemcpyAsync(c.d_src, c.h_src, sizeof(int32_t) * k, hipMemcpyHostToDevice, c.st));
   HIP_TRY(hipMemcpyAsync(c.d_lanes, c.h_lanes, sizeof(LaneOut) * k, hipMemcpyHostToDevice, c.st));
@@ -289,8 +284,7 @@ int ibfs_batch_create(ibfs_graph_t g, const ibfs_config *cfg_in, int32_t context
     return fail(IBFS_ERR_ARG, "lane_bits must be 64, 128, 256 or 512 (got %d)", cfg.lane_bits);
   if (cfg.sml_shed < 1 || cfg.lrg_shed < cfg.sml_shed)
     return fail(IBFS_ERR_ARG, "need 1 <= sml_shed <= lrg_shed");
-  if (cfg.policy < IBFS_POLICY_FIXED || cfg.policy > IBFS_POLICY_BU_ONLY)
-    return fail(IBFS_ERR_ARG, "unknown policy %d", cfg.policy);
+  if (int rc = check_direction_config(cfg)) return rc;
   if (cfg.td_chunk < 256 || (cfg.td_chunk & (cfg.td_chunk - 1)))
     return fail(IBFS_ERR_ARG, "td_chunk must be a power of two >= 256");
   if (cfg.ring_levels < 4 || cfg.ring_levels > 255)

@@ -32,6 +32,17 @@ inline bool valid_inspect_vpt(int v) {
 // inspector workgroup size of ibfs_config
 inline bool valid_wg_size(int v) { return v == 256 || v == 512 || v == 1024; }
 
+// Direction settings of a config: policy, alpha/beta, hybrid_gamma.
+inline int check_direction_config(const ibfs_config &c) {
+  if (c.policy < IBFS_POLICY_FIXED || c.policy > IBFS_POLICY_HYBRID)
+    return fail(IBFS_ERR_ARG, "unknown policy %d", c.policy);
+  if (c.policy == IBFS_POLICY_ALPHA_BETA && (c.alpha < 1 || c.beta < 1))
+    return fail(IBFS_ERR_ARG, "alpha and beta must be >= 1");
+  if (c.hybrid_gamma < 0 || c.hybrid_gamma > 4096)
+    return fail(IBFS_ERR_ARG, "hybrid_gamma must be in [0, 4096] (got %d)", c.hybrid_gamma);
+  return IBFS_OK;
+}
+
 // Parallel helper: runs fn(t) on nthreads std::threads.
 void parallel_for_threads(int nthreads, void (*fn)(int, void *), void *ctx);
 
@@ -63,4 +74,7 @@ struct ibfs_graph_s {
   std::vector<int32_t> h_cid;
   int64_t H = 0;
   int64_t max_deg = 0;
+  // hub membership: deg > hub_deg, or deg == hub_deg and id <= hub_tie
+  int32_t hub_deg = INT32_MAX, hub_tie = -1;
+  int32_t rho_q10 = 0;  // hub entries of the traversal adjacency per entry, x1024
 };

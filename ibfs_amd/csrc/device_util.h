@@ -99,6 +99,10 @@ __device__ __forceinline__ void load_nbr(const Gather &gs, int32_t a, u64 (&y)[W
   load_word<W>(p, y);
 }
 
+__device__ __forceinline__ bool is_hub(const HubSplit &h, int64_t v, int d) {
+  return d > h.deg || (d == h.deg && v <= (int64_t)h.tie);
+}
+
 // Coherent read of a counter another workgroup updated with device atomics.
 __device__ __forceinline__ u64 atomic_read(const u64 *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -24,6 +24,7 @@ struct ibfs_engine_s {
   ibfs_graph_s *g = nullptr;
   ibfs_config cfg{};
   int K = 128, W = 2, R = 32, chunk_shift = 12;
+  int hy_grid = 8192;  // workgroups of a hybrid expansion
   hipStream_t st = nullptr;
   u64 *ring = nullptr;
   LevelCtl *d_ctl = nullptr;
@@ -123,17 +124,13 @@ int64_t isolated_sources(const ibfs_graph_s *g, const int32_t *src, int k) {
   return (int64_t)(std::unique(iso.begin(), iso.end()) - iso.begin());
 }
 
-// direction of level L (0 top-down, 1 bottom-up) from the counters of L-1
+// direction of level L (0 top-down, 1 bottom-up, 2 hybrid) from the
+// counters of L-1
 int decide(const ibfs_engine_s *e, int L, const LevelCtl &c, int dir_prev, u64 nf_prev) {
-  switch (e->cfg.policy) {
-    case IBFS_POLICY_TD_ONLY: return 0;
-    case IBFS_POLICY_BU_ONLY: return 1;
-    case IBFS_POLICY_FIXED: return L <= e->cfg.sw_level ? 0 : 1;
-    default: break;
-  }
-  if (dir_prev != 1) return (c.m_front * (u64)e->cfg.alpha > c.m_cand) ? 1 : 0;
-  const bool small = c.n_front * (u64)e->cfg.beta < (u64)e->g->V;
-  return (small && c.n_front < nf_prev) ? 0 : 1;
+  const ibfs_config &k = e->cfg;
+  return choose_direction(k.policy, k.sw_level, k.alpha, k.beta, k.hybrid_gamma, e->g->rho_q10,
+                          (u64)e->g->V, L, dir_prev, nf_prev, c.n_front, c.m_front, c.m_cand,
+                          c.m_front_nh);
 }
 
 // Host-driven inspector arguments (the pass also refreshes the hub cache
@@ -160,6 +157,7 @@ InspectArgs inspect_args(ibfs_engine_s *e, const u64 *prev, const u64 *cur, u64 
   a.flags = e->flags;
   a.vpt = e->cfg.inspect_vpt;
   a.nw = e->cfg.wg_size / 64;
+  a.hsplit = HubSplit{e->g->hub_deg, e->g->hub_tie};
   return a;
 }
 
@@ -242,6 +240,7 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
     }
     li.expand_ms = exp_ms;
     li.inspect_ms = insp_ms;
+    li.m_frontier_nonhub = (int64_t)c.m_front_nh;
     e->log.push_back(li);
   };
   float insp0 = 0;
@@ -274,6 +273,11 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
       HIP_TRY(launch_expand_td(W, g->d_beg, g->d_adj, slot(e, L - 1), slot(e, L), lamp, e->q,
                                c.q[Q_TD_S], c.q[Q_TD_M], c.q[Q_TD_L], e->chunk_shift, e->st));
       if (stats) stats->n_td_launches++;
+    } else if (dir == 2) {
+      HIP_TRY(launch_expand_hy(W, g->d_beg, g->d_adj, g->d_tadj, slot(e, L - 1), e->hs,
+                               slot(e, L), &e->d_ctl[(L - 1) % e->R], e->q, e->chunk_shift,
+                               HubSplit{g->hub_deg, g->hub_tie}, e->hy_grid, e->st));
+      if (stats) stats->n_hy_launches++;
     } else {
       if (e->cfg.work_model) {  // instrumentation: exact early-termination prefixes
         HIP_TRY(hipMemsetAsync(e->d_count + 1, 0, 2 * sizeof(u64), e->st));
@@ -299,7 +303,7 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
     if (timing) { y0 = next_event(e); HIP_TRY(hipEventRecord(y0, e->st)); }
     HIP_TRY(launch_inspect(W, inspect_args(e, slot(e, L - 1), slot(e, L), slot(e, L + 1), lamp,
                                            &e->d_ctl[L % e->R], &e->d_ctl[(L + 1) % e->R],
-                                           dir == 0 ? 1 : 0),
+                                           dir != 1 ? 1 : 0),
                            e->st));
     if (timing) { y1 = next_event(e); HIP_TRY(hipEventRecord(y1, e->st)); }
     HIP_TRY(hipMemcpyAsync(e->h_ctl, &e->d_ctl[L % e->R], sizeof(LevelCtl),
@@ -310,7 +314,7 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
       HIP_TRY(hipEventElapsedTime(&ems, x0, x1));
       HIP_TRY(hipEventElapsedTime(&ims, y0, y1));
       if (stats) {
-        (dir == 0 ? stats->expand_td_ms : stats->expand_bu_ms) += ems;
+        (dir == 0 ? stats->expand_td_ms : dir == 2 ? stats->expand_hy_ms : stats->expand_bu_ms) += ems;
         stats->inspect_ms += ims;
       }
     }
@@ -535,6 +539,15 @@ int ibfs_graph_create_ex(const int64_t *beg, int64_t V, const int32_t *adj, int6
   g->Vt = Vt;
   g->H = H;
   g->max_deg = maxd;
+  if (H > 0) {
+    g->hub_deg = tdeg[order[H - 1]];
+    g->hub_tie = order[H - 1];
+  }
+  {
+    int64_t hub_entries = 0;
+    for (int64_t j = 0; j < E; ++j) hub_entries += (tadj[j] < 0);
+    g->rho_q10 = E > 0 ? (int32_t)((hub_entries * 1024) / E) : 0;
+  }
   const size_t eb = sizeof(int32_t) * (size_t)std::max<int64_t>(E, 1);
   const size_t vtb = sizeof(int32_t) * (size_t)std::max<int64_t>(Vt, 1);
   hipError_t err = hipMalloc((void **)&g->d_beg, sizeof(int64_t) * (size_t)(Vt + 1));
@@ -636,10 +649,7 @@ int ibfs_engine_create(ibfs_graph_t g, const ibfs_config *cfg_in, ibfs_engine_t 
     return fail(IBFS_ERR_ARG, "lane_bits must be 64, 128, 256 or 512 (got %d)", cfg.lane_bits);
   if (cfg.sml_shed < 1 || cfg.lrg_shed < cfg.sml_shed)
     return fail(IBFS_ERR_ARG, "need 1 <= sml_shed <= lrg_shed (got %d, %d)", cfg.sml_shed, cfg.lrg_shed);
-  if (cfg.policy < IBFS_POLICY_FIXED || cfg.policy > IBFS_POLICY_BU_ONLY)
-    return fail(IBFS_ERR_ARG, "unknown policy %d", cfg.policy);
-  if (cfg.policy == IBFS_POLICY_ALPHA_BETA && (cfg.alpha < 1 || cfg.beta < 1))
-    return fail(IBFS_ERR_ARG, "alpha and beta must be >= 1");
+  if (int rc = check_direction_config(cfg)) return rc;
   if (cfg.td_chunk < 256 || (cfg.td_chunk & (cfg.td_chunk - 1)))
     return fail(IBFS_ERR_ARG, "td_chunk must be a power of two >= 256 (got %d)", cfg.td_chunk);
   if (cfg.emit_mode < 0 || cfg.emit_mode > 2)
@@ -660,6 +670,13 @@ int ibfs_engine_create(ibfs_graph_t g, const ibfs_config *cfg_in, ibfs_engine_t 
   e->K = cfg.lane_bits;
   e->W = cfg.lane_bits / 64;
   e->chunk_shift = __builtin_ctz((unsigned)cfg.td_chunk);
+  {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device) !=
+            hipSuccess || cus <= 0)
+      cus = 256;
+    e->hy_grid = cfg.dd_grid > 0 ? cfg.dd_grid : cus * 32;
+  }
   const int64_t Vt = std::max<int64_t>(g->Vt, 1);
   // ring: at most 8 GiB of snapshots, at least 4 levels
   const size_t slot_bytes = (size_t)Vt * (size_t)e->W * sizeof(u64);
@@ -762,6 +779,8 @@ int ibfs_run_groups(ibfs_engine_t e, const int32_t *src, int64_t n, int32_t *dep
       stats->n_bu_launches += s.n_bu_launches;
       stats->n_inspect_launches += s.n_inspect_launches;
       stats->n_emit_launches += s.n_emit_launches;
+      stats->expand_hy_ms += s.expand_hy_ms;
+      stats->n_hy_launches += s.n_hy_launches;
     }
   }
   return IBFS_OK;

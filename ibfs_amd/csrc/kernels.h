@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <cstring>
+
+#include "../../include/ibfs.h"
 
 namespace ibfs {
 
@@ -22,6 +25,7 @@ struct alignas(64) LevelCtl {
   u64 m_front;         // sum of their degrees
   u64 n_cand;          // bottom-up candidates for L+1
   u64 m_cand;          // sum of their degrees
+  u64 m_front_nh;      // sum of the degrees of frontier vertices that are not hubs
   unsigned int q[6];   // queue lengths: td sml, td mid, td lrg chunks, bu sml, bu mid, bu lrg
   unsigned int pad[14];
 };
@@ -37,16 +41,72 @@ enum { Q_TD_S = 0, Q_TD_M = 1, Q_TD_L = 2, Q_BU_S = 3, Q_BU_M = 4, Q_BU_L = 5 };
 // next level's direction here after each inspection, the expander reads it.
 struct Plan {
   int level;            // level to expand next (== level being inspected)
-  int dir;              // direction of `level`: 0 top-down, 1 bottom-up
+  int dir;              // direction of `level`: 0 top-down, 1 bottom-up, 2 hybrid
   int done;             // 1 once a level produced no change
   int last;             // deepest level with a change (valid when done)
   unsigned int blocks_done;  // unused (kept for the layout the host copies)
   int policy, sw_level, alpha, beta;
+  int gamma;            // hybrid cost factor (1/64 units; 0: never hybrid)
+  int rho;              // hub entries per adjacency entry of the graph (1/1024 units)
   int pad;
   u64 nf_prev;          // frontier size of the level before `level`
   u64 V;
   int dirlog[256];      // direction per level, indexed level & 255
 };
+
+// Plan of a device-driven traversal before its level 0 (V: vertex count
+// of the beta rule; rho: the graph's hub-entry fraction, x1024).
+inline void plan_init(Plan *p, const ibfs_config &c, int rho, u64 V) {
+  std::memset(p, 0, sizeof(Plan));
+  p->policy = c.policy;
+  p->sw_level = c.sw_level;
+  p->alpha = c.alpha;
+  p->beta = c.beta;
+  p->gamma = c.hybrid_gamma;
+  p->rho = rho;
+  p->nf_prev = ~0ull;
+  p->V = V;
+  p->dir = -1;
+}
+
+// Hub membership of a traversal vertex: the hubs are the H highest-degree
+// vertices, ties by id, so v is a hub iff deg(v) > deg or (deg(v) == deg and
+// v <= tie) (deg = degree of the last hub, tie = its id; no hubs: deg =
+// INT32_MAX, tie = -1).  Each traversal-adjacency list holds its hub
+// neighbours as a prefix.
+struct HubSplit {
+  int32_t deg;
+  int32_t tie;
+};
+
+// Direction of level Ln (0 top-down, 1 bottom-up, 2 hybrid) from the
+// counters of level Ln-1, shared by the host-driven drivers and the
+// device-driven plan.  alpha/beta: top-down -> bottom-up when m_f * alpha >
+// m_u, back when n_f * beta < V on a shrinking frontier.  A top-down choice
+// becomes hybrid (candidates pull through their hub prefixes, non-hub
+// frontier vertices push) when gamma/64 * (rho/1024 * m_u) + m_f(non-hub) <
+// m_f: the pulled hub entries, weighted by their cost relative to a pushed
+// edge, plus the edges still pushed, against the edges a top-down level
+// pushes.
+__host__ __device__ inline int choose_direction(int policy, int sw_level, int alpha, int beta,
+                                                int gamma, int rho, u64 V, int Ln, int dir_prev,
+                                                u64 nf_prev, u64 nf, u64 mf, u64 mc, u64 mf_nh) {
+  switch (policy) {
+    case IBFS_POLICY_TD_ONLY: return 0;
+    case IBFS_POLICY_BU_ONLY: return 1;
+    case IBFS_POLICY_HYBRID: return 2;
+    case IBFS_POLICY_FIXED: return Ln <= sw_level ? 0 : 1;
+    default: break;
+  }
+  int dir;
+  if (dir_prev != 1) dir = (mf * (u64)alpha > mc) ? 1 : 0;
+  else dir = (nf * (u64)beta < V && nf < nf_prev) ? 0 : 1;
+  if (dir == 0 && gamma > 0) {
+    const u64 pull = ((mc * (u64)rho) >> 10) * (u64)gamma / 64;
+    if (pull + mf_nh < mf) dir = 2;
+  }
+  return dir;
+}
 
 // Output slot of one lane for the depth emission.
 struct LaneOut {
@@ -101,6 +161,7 @@ struct InspectArgs {
   int vpt;              // vertices per lane and chunk (0: by size)
   int full_diff;
   int nw;               // waves per workgroup: 4, 8 or 16 (0: 8)
+  HubSplit hsplit;      // non-hub frontier degrees are counted separately
 };
 
 // Launchers (all asynchronous on `st`).  W = status words (u64) per vertex.
@@ -121,11 +182,17 @@ hipError_t launch_expand_td(int W, const int64_t *beg, const int32_t *adj, const
 hipError_t launch_expand_bu(int W, const int64_t *beg, const int32_t *tadj, const u64 *prev,
                             const u64 *hs, u64 *cur, uint8_t *flags, const u64 *lam_prev,
                             Queues q, int64_t ns, int64_t nm, int64_t nl, hipStream_t st);
-// Device-driven expansion of plan->level (either direction), fixed grid.
+// Hybrid level, fixed grid (queue lengths read from ctl_prev on the
+// device): the bottom-up candidates OR the hub-cache words of their hub
+// prefixes, the non-hub top-down frontier pushes; both with atomic ORs.
+hipError_t launch_expand_hy(int W, const int64_t *beg, const int32_t *adj, const int32_t *tadj,
+                            const u64 *prev, const u64 *hs, u64 *cur, const LevelCtl *ctl_prev,
+                            Queues q, int chunk_shift, HubSplit hsplit, int grid, hipStream_t st);
+// Device-driven expansion of plan->level (any direction), fixed grid.
 hipError_t launch_expand_dd(int W, const int64_t *beg, const int32_t *adj, const int32_t *tadj,
                             u64 *ring, uint8_t *flags, int R, int64_t V, const LevelCtl *ctl_ring,
                             const Plan *plan, const u64 *hs, Queues q, int chunk_shift,
-                            int grid, hipStream_t st);
+                            HubSplit hsplit, int grid, hipStream_t st);
 // Device-driven direction decision for plan->level + 1 (one thread).
 hipError_t launch_plan_decide(Plan *plan, const LevelCtl *ctl_ring, int R, hipStream_t st);
 // Depth emission over the snapshot window [b, e] (ring slots L % R, each

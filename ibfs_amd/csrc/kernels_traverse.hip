@@ -50,6 +50,7 @@ __device__ void plan_decide(Plan *p, const LevelCtl *c, int L) {
   const u64 nf = atomic_read(&c->n_front);
   const u64 mf = atomic_read(&c->m_front);
   const u64 mc = atomic_read(&c->m_cand);
+  const u64 mfn = atomic_read(&c->m_front_nh);
   if (nch == 0) {
     p->done = 1;
     // level 0 always ends a window: lanes whose source is isolated change
@@ -59,16 +60,8 @@ __device__ void plan_decide(Plan *p, const LevelCtl *c, int L) {
   }
   const int Ln = L + 1;
   const int dir_prev = (L == 0) ? -1 : p->dir;
-  int dir;
-  switch (p->policy) {
-    case IBFS_POLICY_TD_ONLY: dir = 0; break;
-    case IBFS_POLICY_BU_ONLY: dir = 1; break;
-    case IBFS_POLICY_FIXED: dir = (Ln <= p->sw_level) ? 0 : 1; break;
-    default:
-      if (dir_prev != 1) dir = (mf * (u64)p->alpha > mc) ? 1 : 0;
-      else dir = (nf * (u64)p->beta < p->V && nf < p->nf_prev) ? 0 : 1;
-      break;
-  }
+  const int dir = choose_direction(p->policy, p->sw_level, p->alpha, p->beta, p->gamma, p->rho,
+                                   p->V, Ln, dir_prev, p->nf_prev, nf, mf, mc, mfn);
   p->nf_prev = nf;
   p->dir = dir;
   p->dirlog[Ln & 255] = dir;
@@ -106,7 +99,7 @@ __global__ __launch_bounds__(NW * 64) void k_inspect(InspectArgs a) {
   constexpr int IBS = NW * 64;
   __shared__ unsigned wsum[2][NW][6];   // per-wave class totals, double-buffered by tile parity
   __shared__ unsigned wbase[2][NW][6];  // per-wave queue bases
-  __shared__ u64 red[NW][5 + W];
+  __shared__ u64 red[NW][6 + W];
 
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const u64 *prev = a.prev, *cur = a.cur;
@@ -142,7 +135,8 @@ __global__ __launch_bounds__(NW * 64) void k_inspect(InspectArgs a) {
     load_word<W>(cur + (int64_t)a.hubs[i] * W, x);
     store_word<W>(a.hs + i * W, x);
   }
-  u64 n_changed = 0, n_front = 0, m_front = 0, n_cand = 0, m_cand = 0;
+  u64 n_changed = 0, n_front = 0, m_front = 0, n_cand = 0, m_cand = 0, m_front_nh = 0;
+  const HubSplit hsp = a.hsplit;
   const int64_t V = a.V;
   const int sml = a.sml, lrg = a.lrg, cs = a.chunk_shift;
   uint8_t *flags = a.flags;
@@ -211,6 +205,7 @@ __global__ __launch_bounds__(NW * 64) void k_inspect(InspectArgs a) {
           if (dg > 0) {
             ++n_front;
             m_front += (u64)dg;
+            if (!is_hub(hsp, v, dg)) m_front_nh += (u64)dg;
             tc = degree_class(dg, sml, lrg);
             if (tc == 3) nck = (dg + (1 << cs) - 1) >> cs;
           }
@@ -288,26 +283,26 @@ __global__ __launch_bounds__(NW * 64) void k_inspect(InspectArgs a) {
   }
 
   // workgroup reduction of the level counters, one atomic each
-  u64 vals[5 + W] = {n_changed, n_front, m_front, n_cand, m_cand};
+  u64 vals[6 + W] = {n_changed, n_front, m_front, n_cand, m_cand, m_front_nh};
 #pragma unroll
-  for (int w = 0; w < W; ++w) vals[5 + w] = lamacc[w];
+  for (int w = 0; w < W; ++w) vals[6 + w] = lamacc[w];
 #pragma unroll
-  for (int i = 0; i < 5; ++i) vals[i] = wave_sum(vals[i]);
+  for (int i = 0; i < 6; ++i) vals[i] = wave_sum(vals[i]);
 #pragma unroll
-  for (int w = 0; w < W; ++w) vals[5 + w] = wave_or(vals[5 + w]);
+  for (int w = 0; w < W; ++w) vals[6 + w] = wave_or(vals[6 + w]);
   if (lane == 0) {
 #pragma unroll
-    for (int i = 0; i < 5 + W; ++i) red[wv][i] = vals[i];
+    for (int i = 0; i < 6 + W; ++i) red[wv][i] = vals[i];
   }
   __syncthreads();
-  if (t < 5) {
+  if (t < 6) {  // n_changed .. m_front_nh are consecutive
     u64 s = 0;
     for (int w = 0; w < NW; ++w) s += red[w][t];
     if (s) atomicAdd(&(&ctl->n_changed)[t], s);
-  } else if (t < 5 + W) {
+  } else if (t < 6 + W) {
     u64 s = 0;
     for (int w = 0; w < NW; ++w) s |= red[w][t];
-    if (s) atomicOr(&ctl->lam[t - 5], s);
+    if (s) atomicOr(&ctl->lam[t - 6], s);
   }
 }
 
@@ -398,15 +393,35 @@ __device__ __forceinline__ void td_chunk_range(const int64_t *__restrict__ beg, 
 #define IBFS_BU_STEP 4
 #endif
 
+// Result of a bottom-up candidate: x (= x0 | new bits) into cur.  A plain
+// store and a flag for the inspector's flagged diff; a hybrid level ORs the
+// changed words in atomically instead (pushes may target the same vertex).
+template <int W, bool HY>
+__device__ __forceinline__ void bu_commit(u64 *cur, uint8_t *flags, int64_t v, const u64 (&x)[W],
+                                          const u64 (&x0)[W]) {
+  if (!differs<W>(x, x0)) return;
+  if (HY) {
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+      if (x[w] != x0[w]) atomicOr(&cur[v * W + w], x[w]);
+  } else {
+    store_word<W>(cur + v * W, x);
+    flags[v] = 1;
+  }
+}
+
 // Bottom-up: a candidate v ORs S_{L-1} of its neighbours (restricted to
 // lam_{L-1}, the only lanes that can still be new) in traversal-adjacency
 // order (descending degree, hubs first) and stops as soon as its word
 // covers lam_{L-1} (bitwise early termination, expander.cu:258-359).
 // One candidate per thread, IBFS_BU_STEP neighbours per step.
-template <int W>
+// HY (hybrid level): only the hub prefix of the list is read (from the hub
+// cache), and new bits go in with atomic ORs (non-hub frontier vertices push
+// into the same words concurrently); nothing is flagged.
+template <int W, bool HY = false>
 __device__ __forceinline__ void bu_thread(const int64_t *__restrict__ beg,
                                           const int32_t *__restrict__ tadj, const Gather &gs,
-                                          u64 *__restrict__ cur, uint8_t *flags,
+                                          u64 *cur, uint8_t *flags,
                                           const u64 (&lam)[W], int64_t v) {
   u64 x[W], x0[W];
   load_word<W>(cur + v * W, x);
@@ -421,26 +436,32 @@ __device__ __forceinline__ void bu_thread(const int64_t *__restrict__ beg,
 #pragma unroll
     for (int r = 1; r < IBFS_BU_STEP; ++r) an[r] = (j + r < e) ? tadj[j + r] : an[0];
     u64 sn[IBFS_BU_STEP][W];
+    bool nonhub = false;
 #pragma unroll
-    for (int r = 0; r < IBFS_BU_STEP; ++r) load_nbr<W>(gs, an[r], sn[r]);
+    for (int r = 0; r < IBFS_BU_STEP; ++r) {
+      if (HY && an[r] >= 0) {
+        nonhub = true;
+#pragma unroll
+        for (int w = 0; w < W; ++w) sn[r][w] = 0;
+      } else {
+        load_nbr<W>(gs, an[r], sn[r]);
+      }
+    }
 #pragma unroll
     for (int r = 0; r < IBFS_BU_STEP; ++r)
 #pragma unroll
       for (int w = 0; w < W; ++w) x[w] |= sn[r][w] & lam[w];
     j += IBFS_BU_STEP;
-    if (covers<W>(x, lam)) break;
+    if (covers<W>(x, lam) || nonhub) break;  // hub prefix ended
   }
-  if (differs<W>(x, x0)) {
-    store_word<W>(cur + v * W, x);
-    flags[v] = 1;
-  }
+  bu_commit<W, HY>(cur, flags, v, x, x0);
 }
 
 // One candidate per wave: 64 neighbours per step, wave OR-reduction.
-template <int W>
+template <int W, bool HY = false>
 __device__ __forceinline__ void bu_wave(const int64_t *__restrict__ beg,
                                         const int32_t *__restrict__ tadj, const Gather &gs,
-                                        u64 *__restrict__ cur, uint8_t *flags,
+                                        u64 *cur, uint8_t *flags,
                                         const u64 (&lam)[W], int64_t v, int lane) {
   u64 x[W], x0[W];
   load_word<W>(cur + v * W, x);
@@ -451,29 +472,31 @@ __device__ __forceinline__ void bu_wave(const int64_t *__restrict__ beg,
   for (int64_t r = s; r < e; r += 64) {
     const int64_t j = r + lane;
     u64 y[W];
-    if (j < e) {
-      load_nbr<W>(gs, tadj[j], y);
-    } else {
 #pragma unroll
-      for (int w = 0; w < W; ++w) y[w] = 0;
+    for (int w = 0; w < W; ++w) y[w] = 0;
+    bool nonhub = false;
+    if (j < e) {
+      const int32_t a = tadj[j];
+      if (HY && a >= 0) nonhub = true;
+      else load_nbr<W>(gs, a, y);
     }
 #pragma unroll
     for (int w = 0; w < W; ++w) x[w] |= wave_or(y[w] & lam[w]);
     if (covers<W>(x, lam)) break;
+    if (HY && __ballot(nonhub)) break;  // hub prefix ended
   }
-  if (lane == 0 && differs<W>(x, x0)) {
-    store_word<W>(cur + v * W, x);
-    flags[v] = 1;
-  }
+  if (lane == 0) bu_commit<W, HY>(cur, flags, v, x, x0);
 }
 
 // One candidate per workgroup: 256 neighbours per step, the four wave
 // partials combined through LDS.  Must be called by the whole workgroup.
-template <int W>
+// nh: four ints of LDS (hub prefix ended, per wave; used by HY).
+template <int W, bool HY = false>
 __device__ __forceinline__ void bu_block(const int64_t *__restrict__ beg,
                                          const int32_t *__restrict__ tadj, const Gather &gs,
-                                         u64 *__restrict__ cur, uint8_t *flags,
-                                         const u64 (&lam)[W], int64_t v, u64 (*part)[W]) {
+                                         u64 *cur, uint8_t *flags,
+                                         const u64 (&lam)[W], int64_t v, u64 (*part)[W],
+                                         int *nh) {
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   u64 x[W], x0[W];
   load_word<W>(cur + v * W, x);
@@ -484,28 +507,30 @@ __device__ __forceinline__ void bu_block(const int64_t *__restrict__ beg,
   for (int64_t r = s; r < e; r += BS) {
     const int64_t j = r + t;
     u64 y[W];
-    if (j < e) {
-      load_nbr<W>(gs, tadj[j], y);
-    } else {
 #pragma unroll
-      for (int w = 0; w < W; ++w) y[w] = 0;
+    for (int w = 0; w < W; ++w) y[w] = 0;
+    bool nonhub = false;
+    if (j < e) {
+      const int32_t a = tadj[j];
+      if (HY && a >= 0) nonhub = true;
+      else load_nbr<W>(gs, a, y);
     }
 #pragma unroll
     for (int w = 0; w < W; ++w) y[w] = wave_or(y[w] & lam[w]);
+    const bool wnh = HY && __ballot(nonhub) != 0;
     if (lane == 0) {
 #pragma unroll
       for (int w = 0; w < W; ++w) part[wv][w] = y[w];
+      if (HY) nh[wv] = wnh;
     }
     __syncthreads();
 #pragma unroll
     for (int w = 0; w < W; ++w) x[w] |= part[0][w] | part[1][w] | part[2][w] | part[3][w];
+    const bool stop = HY && (nh[0] | nh[1] | nh[2] | nh[3]);
     __syncthreads();
-    if (covers<W>(x, lam)) break;
+    if (covers<W>(x, lam) || stop) break;
   }
-  if (t == 0 && differs<W>(x, x0)) {
-    store_word<W>(cur + v * W, x);
-    flags[v] = 1;
-  }
+  if (t == 0) bu_commit<W, HY>(cur, flags, v, x, x0);
 }
 
 // Host-driven top-down: workgroups [0, nbs) take small vertices one per
@@ -551,6 +576,7 @@ __global__ __launch_bounds__(BS) void k_expand_bu(const int64_t *__restrict__ be
                                                   int64_t ns, int64_t nm, int64_t nl, int64_t nbs,
                                                   int64_t nbm) {
   __shared__ u64 part[4][W];
+  __shared__ int nh[4];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int64_t b = blockIdx.x;
   u64 lam[W];
@@ -564,12 +590,69 @@ __global__ __launch_bounds__(BS) void k_expand_bu(const int64_t *__restrict__ be
     if (i < nm) bu_wave<W>(beg, tadj, gs, cur, flags, lam, q.q[Q_BU_M][i], lane);
   } else {
     const int64_t i = b - nbs - nbm;
-    if (i < nl) bu_block<W>(beg, tadj, gs, cur, flags, lam, q.q[Q_BU_L][i], part);
+    if (i < nl) bu_block<W>(beg, tadj, gs, cur, flags, lam, q.q[Q_BU_L][i], part, nh);
   }
 }
 
+// Hybrid level from the queues of ctl (the control block of the level
+// before), grid-strided: the bottom-up candidates pull through their hub
+// prefixes, then the top-down frontier vertices that are not hubs push.  A
+// hub frontier vertex u reaches every candidate neighbour v through v's hub
+// prefix (hub neighbours are a prefix of every traversal-adjacency list);
+// a neighbour that is no candidate already holds every lane u can bring.
+template <int W>
+__device__ __forceinline__ void expand_hybrid(const int64_t *__restrict__ beg,
+                                              const int32_t *__restrict__ adj,
+                                              const int32_t *__restrict__ tadj, const Gather &gs,
+                                              u64 *cur, const u64 (&lam)[W],
+                                              const LevelCtl *ctl, const Queues &q, int cs,
+                                              HubSplit hsp, u64 (*part)[W], int *nh) {
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int64_t G = gridDim.x;
+  const int64_t bs = ctl->q[Q_BU_S], bm = ctl->q[Q_BU_M], bl = ctl->q[Q_BU_L];
+  for (int64_t i = (int64_t)blockIdx.x * BS + t; i < bs; i += G * BS)
+    bu_thread<W, true>(beg, tadj, gs, cur, nullptr, lam, q.q[Q_BU_S][i]);
+  for (int64_t i = (int64_t)blockIdx.x * 4 + wv; i < bm; i += G * 4)
+    bu_wave<W, true>(beg, tadj, gs, cur, nullptr, lam, q.q[Q_BU_M][i], lane);
+  for (int64_t i = blockIdx.x; i < bl; i += G)
+    bu_block<W, true>(beg, tadj, gs, cur, nullptr, lam, q.q[Q_BU_L][i], part, nh);
+  const int64_t ts = ctl->q[Q_TD_S], tm = ctl->q[Q_TD_M], tl = ctl->q[Q_TD_L];
+  for (int64_t i = (int64_t)blockIdx.x * BS + t; i < ts; i += G * BS) {
+    const int32_t u = q.q[Q_TD_S][i];
+    if (!is_hub(hsp, u, (int)(beg[u + 1] - beg[u])))
+      td_push_small<W>(beg, adj, gs.prev, cur, lam, u);
+  }
+  for (int64_t i = (int64_t)blockIdx.x * 4 + wv; i < tm; i += G * 4) {
+    const int32_t u = q.q[Q_TD_M][i];
+    const int64_t b0 = beg[u], b1 = beg[u + 1];
+    if (!is_hub(hsp, u, (int)(b1 - b0))) td_push<W>(beg, adj, gs.prev, cur, lam, u, b0 + lane, b1, 64);
+  }
+  for (int64_t i = blockIdx.x; i < tl; i += G) {
+    const int2 e = q.chunks[i];
+    if (is_hub(hsp, e.x, (int)(beg[e.x + 1] - beg[e.x]))) continue;
+    int64_t lo, hi;
+    td_chunk_range(beg, e, cs, lo, hi);
+    td_push<W>(beg, adj, gs.prev, cur, lam, e.x, lo + t, hi, BS);
+  }
+}
+
+// Host-driven hybrid level (queue lengths read on the device from ctl).
+template <int W>
+__global__ __launch_bounds__(BS) void k_expand_hy(const int64_t *__restrict__ beg,
+                                                  const int32_t *__restrict__ adj,
+                                                  const int32_t *__restrict__ tadj, Gather gs,
+                                                  u64 *cur, const LevelCtl *__restrict__ ctl,
+                                                  Queues q, int cs, HubSplit hsp) {
+  __shared__ u64 part[4][W];
+  __shared__ int nh[4];
+  u64 lam[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) lam[w] = ctl->lam[w];
+  expand_hybrid<W>(beg, adj, tadj, gs, cur, lam, ctl, q, cs, hsp, part, nh);
+}
+
 // Device-driven expansion of level plan->level: a fixed grid strides over
-// the three class queues of the direction the inspector chose.
+// the class queues of the direction the inspector chose.
 template <int W>
 __global__ __launch_bounds__(BS) void k_expand_dd(const int64_t *__restrict__ beg,
                                                   const int32_t *__restrict__ adj,
@@ -577,8 +660,10 @@ __global__ __launch_bounds__(BS) void k_expand_dd(const int64_t *__restrict__ be
                                                   uint8_t *flags, int R, int64_t V,
                                                   const LevelCtl *__restrict__ ctl_ring,
                                                   const Plan *__restrict__ plan,
-                                                  const u64 *__restrict__ hs, Queues q, int cs) {
+                                                  const u64 *__restrict__ hs, Queues q, int cs,
+                                                  HubSplit hsp) {
   __shared__ u64 part[4][W];
+  __shared__ int nh[4];
   if (plan->done || plan->level < 1) return;
   const int L = plan->level, dir = plan->dir;
   const LevelCtl *c = ctl_ring + (L - 1) % R;
@@ -604,6 +689,9 @@ __global__ __launch_bounds__(BS) void k_expand_dd(const int64_t *__restrict__ be
       td_chunk_range(beg, e, cs, lo, hi);
       td_push<W>(beg, adj, prev, cur, lam, e.x, lo + t, hi, BS);
     }
+  } else if (dir == 2) {
+    const Gather gs{prev, hs};
+    expand_hybrid<W>(beg, adj, tadj, gs, cur, lam, c, q, cs, hsp, part, nh);
   } else {
     const int64_t ns = c->q[Q_BU_S], nm = c->q[Q_BU_M], nl = c->q[Q_BU_L];
     const Gather gs{prev, hs};
@@ -612,7 +700,7 @@ __global__ __launch_bounds__(BS) void k_expand_dd(const int64_t *__restrict__ be
     for (int64_t i = (int64_t)blockIdx.x * 4 + wv; i < nm; i += G * 4)
       bu_wave<W>(beg, tadj, gs, cur, flags, lam, q.q[Q_BU_M][i], lane);
     for (int64_t i = blockIdx.x; i < nl; i += G)
-      bu_block<W>(beg, tadj, gs, cur, flags, lam, q.q[Q_BU_L][i], part);
+      bu_block<W>(beg, tadj, gs, cur, flags, lam, q.q[Q_BU_L][i], part, nh);
   }
 }
 
@@ -791,10 +879,20 @@ hipError_t launch_expand_bu(int W, const int64_t *beg, const int32_t *tadj, cons
 hipError_t launch_expand_dd(int W, const int64_t *beg, const int32_t *adj, const int32_t *tadj,
                             u64 *ring, uint8_t *flags, int R, int64_t V, const LevelCtl *ctl_ring,
                             const Plan *plan, const u64 *hs, Queues q, int chunk_shift,
-                            int grid, hipStream_t st) {
+                            HubSplit hsplit, int grid, hipStream_t st) {
   dim3 g(grid > 0 ? grid : 1), blk(BS);
   IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_expand_dd<WW>, g, blk, 0, st, beg, adj, tadj, ring,
-                                        flags, R, V, ctl_ring, plan, hs, q, chunk_shift));
+                                        flags, R, V, ctl_ring, plan, hs, q, chunk_shift, hsplit));
+  return hipGetLastError();
+}
+
+hipError_t launch_expand_hy(int W, const int64_t *beg, const int32_t *adj, const int32_t *tadj,
+                            const u64 *prev, const u64 *hs, u64 *cur, const LevelCtl *ctl_prev,
+                            Queues q, int chunk_shift, HubSplit hsplit, int grid, hipStream_t st) {
+  dim3 g(grid > 0 ? grid : 1), blk(BS);
+  const Gather gs{prev, hs};
+  IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_expand_hy<WW>, g, blk, 0, st, beg, adj, tadj, gs, cur,
+                                        ctl_prev, q, chunk_shift, hsplit));
   return hipGetLastError();
 }
 

@@ -203,6 +203,7 @@ InspectArgs plan_args(const ibfs_server_s *s, const SCtx &c) {
   a.flags = c.flags;
   a.vpt = s->cfg.inspect_vpt;
   a.nw = s->cfg.wg_size / 64;
+  a.hsplit = HubSplit{s->g->hub_deg, s->g->hub_tie};
   return a;
 }
 
@@ -243,7 +244,7 @@ int alloc_ctx(ibfs_server_s *s, SCtx &c) {
   HIP_CHK(hipStreamBeginCapture(c.st, hipStreamCaptureModeThreadLocal));
   hipError_t e1 = launch_expand_dd(s->W, s->g->d_beg, s->g->d_adj, s->g->d_tadj, c.ring, c.flags,
                                    s->R, s->g->Vt, c.ctl, c.d_plan, c.hs, c.q, s->chunk_shift,
-                                   s->grid, c.st);
+                                   HubSplit{s->g->hub_deg, s->g->hub_tie}, s->grid, c.st);
   hipError_t e2 = launch_inspect(s->W, plan_args(s, c), c.st);
   if (e2 == hipSuccess) e2 = launch_plan_decide(c.d_plan, c.ctl, s->R, c.st);
   hipError_t e3 = hipStreamEndCapture(c.st, &c.graph);
@@ -340,6 +341,7 @@ int inspect_level(ibfs_server_s *s, SCtx &c, int L, const u64 *filter, const u64
   a.flags = c.flags;
   a.vpt = s->cfg.inspect_vpt;
   a.nw = s->cfg.wg_size / 64;
+  a.hsplit = HubSplit{s->g->hub_deg, s->g->hub_tie};
   a.full_diff = full_diff;
   HIP_CHK(launch_inspect(W, a, c.st));
   HIP_CHK(hipMemcpyAsync(c.h_ctl, c.ctl + L % s->R, sizeof(LevelCtl), hipMemcpyDeviceToHost, c.st));
@@ -368,14 +370,10 @@ int start_ctx(ibfs_server_s *s, SCtx &c) {
 }
 
 int decide(const ibfs_server_s *s, int Ln, const LevelCtl &lc, int dir_prev, u64 nf_prev) {
-  switch (s->cfg.policy) {
-    case IBFS_POLICY_TD_ONLY: return 0;
-    case IBFS_POLICY_BU_ONLY: return 1;
-    case IBFS_POLICY_FIXED: return Ln <= s->cfg.sw_level ? 0 : 1;
-    default: break;
-  }
-  if (dir_prev != 1) return (lc.m_front * (u64)s->cfg.alpha > lc.m_cand) ? 1 : 0;
-  return (lc.n_front * (u64)s->cfg.beta < (u64)s->g->V && lc.n_front < nf_prev) ? 0 : 1;
+  const ibfs_config &k = s->cfg;
+  return choose_direction(k.policy, k.sw_level, k.alpha, k.beta, k.hybrid_gamma, s->g->rho_q10,
+                          (u64)s->g->V, Ln, dir_prev, nf_prev, lc.n_front, lc.m_front, lc.m_cand,
+                          lc.m_front_nh);
 }
 
 void mark_done(ibfs_server_s *s, SCtx &c) {
@@ -458,6 +456,10 @@ int process_level(ibfs_server_s *s, SCtx &c) {
   if (dir == 0)
     HIP_CHK(launch_expand_td(W, s->g->d_beg, s->g->d_adj, slot(s, c, L), slot(s, c, Ln), lamp,
                              c.q, lc.q[Q_TD_S], lc.q[Q_TD_M], lc.q[Q_TD_L], s->chunk_shift, c.st));
+  else if (dir == 2)
+    HIP_CHK(launch_expand_hy(W, s->g->d_beg, s->g->d_adj, s->g->d_tadj, slot(s, c, L), c.hs,
+                             slot(s, c, Ln), c.ctl + L % s->R, c.q, s->chunk_shift,
+                             HubSplit{s->g->hub_deg, s->g->hub_tie}, s->grid, c.st));
   else
     HIP_CHK(launch_expand_bu(W, s->g->d_beg, s->g->d_tadj, slot(s, c, L), c.hs, slot(s, c, Ln),
                              c.flags, lamp, c.q, lc.q[Q_BU_S], lc.q[Q_BU_M], lc.q[Q_BU_L], c.st));
@@ -501,7 +503,7 @@ int process_level(ibfs_server_s *s, SCtx &c) {
     nand[w] = ~finmask[w];
   }
   // 5. inspection of Ln (clears the lanes freed at L in slot Ln+1)
-  return inspect_level(s, c, Ln, filter.data(), keep.data(), nand.data(), dir == 0 ? 1 : 0);
+  return inspect_level(s, c, Ln, filter.data(), keep.data(), nand.data(), dir != 1 ? 1 : 0);
 }
 
 // Launches up to kCohortBurst level graphs of the cohort (bounded so that no
@@ -538,14 +540,7 @@ int start_cohort(ibfs_server_s *s, SCtx &c) {
   int rc = admit(s, c, 0, c.h_all, &n);
   if (rc) return rc;
   HIP_CHK(hipMemcpyAsync(c.d_all, c.h_all, sizeof(u64) * W, hipMemcpyHostToDevice, c.st));
-  std::memset(c.h_plan, 0, sizeof(Plan));
-  c.h_plan->policy = s->cfg.policy;
-  c.h_plan->sw_level = s->cfg.sw_level;
-  c.h_plan->alpha = s->cfg.alpha;
-  c.h_plan->beta = s->cfg.beta;
-  c.h_plan->nf_prev = ~0ull;
-  c.h_plan->V = (u64)s->g->V;
-  c.h_plan->dir = -1;
+  plan_init(c.h_plan, s->cfg, s->g->rho_q10, (u64)s->g->V);
   HIP_CHK(hipMemcpyAsync(c.d_plan, c.h_plan, sizeof(Plan), hipMemcpyHostToDevice, c.st));
   HIP_CHK(launch_inspect(W, plan_args(s, c), c.st));
   HIP_CHK(launch_plan_decide(c.d_plan, c.ctl, s->R, c.st));
@@ -655,6 +650,7 @@ int ibfs_server_create(ibfs_graph_t g, const ibfs_config *cfg_in, int64_t status
   if (cfg.emit_mode < 0 || cfg.emit_mode > 2) return fail(IBFS_ERR_ARG, "bad emit_mode");
   if (!valid_inspect_vpt(cfg.inspect_vpt)) return fail(IBFS_ERR_ARG, "bad inspect_vpt");
   if (!valid_wg_size(cfg.wg_size)) return fail(IBFS_ERR_ARG, "wg_size must be 256, 512 or 1024");
+  if (int rc = check_direction_config(cfg)) return rc;
   if (cfg.device != g->device) return fail(IBFS_ERR_ARG, "config device differs from the graph's");
   int nd = 0;
   if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0)

@@ -37,6 +37,8 @@ extern "C" {
 #define IBFS_POLICY_ALPHA_BETA 1 /* Beamer-style switch on frontier/unvisited edges  */
 #define IBFS_POLICY_TD_ONLY 2
 #define IBFS_POLICY_BU_ONLY 3
+#define IBFS_POLICY_HYBRID 4     /* every level hybrid: candidates pull through their
+                                    hub neighbours, non-hub frontier vertices push */
 
 /* output placement for depth arrays */
 #define IBFS_OUT_HOST 0
@@ -71,13 +73,17 @@ typedef struct ibfs_config {
   int32_t dd_grid;     /* workgroups of the device-driven expander (0: auto)  */
   int32_t inspect_vpt; /* inspector vertices per lane and tile: 1, 2, 4, 8 or
                           16, capped at 16 words per lane (0: by size)       */
+  int32_t hybrid_gamma; /* ALPHA_BETA: a top-down level runs hybrid when
+                          gamma/64 * (hub entries of the candidates) + (edges of
+                          non-hub frontier vertices) < (frontier edges);
+                          0: never                                            */
 } ibfs_config;
 
 /* Per-level record of the last traversal (the build's monitoring log, in place
  * of ENABLE_MONITORING prints at traversal.cu:127-128). */
 typedef struct ibfs_level_info {
   int32_t level;       /* level L: the pass that produced status_L             */
-  int32_t direction;   /* 0 top-down, 1 bottom-up, -1 init                     */
+  int32_t direction;   /* 0 top-down, 1 bottom-up, 2 hybrid, -1 init          */
   int64_t n_changed;   /* vertices whose status gained bits at L               */
   int64_t n_frontier;  /* changed vertices with degree > 0                     */
   int64_t m_frontier;  /* sum of degrees of the frontier                       */
@@ -89,6 +95,7 @@ typedef struct ibfs_level_info {
   int64_t bu_prefix;   /* ... sum of their early-termination prefixes (edges)  */
   float expand_ms;     /* kernel time of the expansion producing status_L      */
   float inspect_ms;    /* kernel time of the inspection of status_L            */
+  int64_t m_frontier_nonhub; /* ... of the frontier vertices that are not hubs */
 } ibfs_level_info;
 
 typedef struct ibfs_stats {
@@ -98,6 +105,9 @@ typedef struct ibfs_stats {
   double total_ms;         /* device time from first to last kernel (events)   */
   double init_ms, expand_td_ms, expand_bu_ms, inspect_ms, emit_ms; /* timing=1 */
   int32_t n_td_launches, n_bu_launches, n_inspect_launches, n_emit_launches;
+  double expand_hy_ms;     /* hybrid expansions (timing=1)                     */
+  int32_t n_hy_launches;
+  int32_t reserved;
 } ibfs_stats;
 
 /* ---- errors / version ---------------------------------------------------- */

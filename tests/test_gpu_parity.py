@@ -17,6 +17,9 @@ POLICIES = {
     "bu": dict(policy=ibfs_amd.POLICY_BU_ONLY),
     "sw1": dict(policy=ibfs_amd.POLICY_FIXED, sw_level=1),
     "sw2": dict(policy=ibfs_amd.POLICY_FIXED, sw_level=2),
+    "hy": dict(policy=ibfs_amd.POLICY_HYBRID),
+    # alpha/beta with hybrid levels whenever pulling is priced at 1/64 of a push
+    "abh": dict(policy=ibfs_amd.POLICY_ALPHA_BETA, hybrid_gamma=1),
 }
 
 
@@ -49,7 +52,7 @@ def test_golden_all_sources(name, pol):
 
 
 @pytest.mark.parametrize("lane_bits", [64, 128, 256, 512])
-@pytest.mark.parametrize("pol", ["ab", "bu", "td"])
+@pytest.mark.parametrize("pol", ["ab", "bu", "td", "hy", "abh"])
 def test_s16_128_sources(s16, lane_bits, pol):
     say(f"[gpu] s16 lane_bits={lane_bits} policy={pol}")
     eng = ibfs_amd.Engine(s16["g"], lane_bits=lane_bits, **POLICIES[pol])
@@ -449,3 +452,58 @@ def test_cli_runs_reference_arguments(tmp_path):
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.count("Traversal-time:") == 2 and "Total time:" in r.stdout
+
+
+@pytest.mark.parametrize("hubs", [0, 64, 4096, None])
+@pytest.mark.parametrize("pol", ["hy", "abh"])
+def test_hybrid_levels_every_hub_split(s16, hubs, pol):
+    """Hybrid levels (candidates pull through their hub prefixes, non-hub
+    frontier vertices push) with no hubs (all push), few, many and the
+    default count (every vertex a hub on s16: all pull), through the engine,
+    the batched launcher and both serving paths."""
+    say(f"[gpu] hybrid hubs={hubs} policy={pol}")
+    g = ibfs_amd.DeviceGraph(s16["h"], hub_count=hubs) if hubs is not None else s16["g"]
+    src, want = s16["src"], s16["want"]
+    for lb in (64, 256):
+        eng = ibfs_amd.Engine(g, lane_bits=lb, kernel_timing=1, **POLICIES[pol])
+        np.testing.assert_array_equal(eng.run_groups(src), want)
+        dirs = [x["direction"] for x in eng.level_log()]
+        if pol == "hy":
+            assert set(dirs[1:]) == {2} and eng.last_stats.n_hy_launches > 0
+        b = ibfs_amd.Batch(g, contexts=2, burst=3, lane_bits=lb, **POLICIES[pol])
+        np.testing.assert_array_equal(b.run(src), want)
+    V = s16["h"].vert_count
+    for window in (0, 2):
+        srv = ibfs_amd.Server(g, lane_bits=64, ring_levels=8, admit_window=window, **POLICIES[pol])
+        queries = [src[0:40], src[40:41], src[41:128]]
+        outs, _ = _serve(srv, queries, V, gap_s=0.0005)
+        for q, o in zip(queries, outs):
+            np.testing.assert_array_equal(o, oracle.bfs_many(s16["beg"], s16["adj"], q, threads=8))
+        srv.close()
+
+
+def test_hybrid_is_chosen_on_hub_heavy_levels(s16):
+    """With pulling priced low, an alpha/beta traversal of R-MAT runs its
+    hub-heavy top-down levels hybrid, and the non-hub frontier edges it
+    reports never exceed the frontier edges."""
+    g = ibfs_amd.DeviceGraph(s16["h"], hub_count=2048)
+    eng = ibfs_amd.Engine(g, lane_bits=128, **POLICIES["abh"])
+    np.testing.assert_array_equal(eng.traversal(s16["src"]), s16["want"])
+    log = eng.level_log()
+    assert 2 in [x["direction"] for x in log]
+    assert all(0 <= x["m_frontier_nonhub"] <= x["m_frontier"] for x in log)
+    # no hubs: every frontier edge is a non-hub edge
+    g0 = ibfs_amd.DeviceGraph(s16["h"], hub_count=0)
+    eng0 = ibfs_amd.Engine(g0, lane_bits=128)
+    eng0.traversal(s16["src"])
+    assert all(x["m_frontier_nonhub"] == x["m_frontier"] for x in eng0.level_log())
+
+
+def test_direction_config_checks(s16):
+    for bad in (dict(policy=5), dict(hybrid_gamma=-1), dict(hybrid_gamma=5000)):
+        with pytest.raises(ibfs_amd.IbfsError):
+            ibfs_amd.Engine(s16["g"], **bad)
+        with pytest.raises(ibfs_amd.IbfsError):
+            ibfs_amd.Batch(s16["g"], **bad)
+        with pytest.raises(ibfs_amd.IbfsError):
+            ibfs_amd.Server(s16["g"], **bad)
