@@ -364,6 +364,14 @@ def main() -> int:
                "kind": "port",
                "sample": f"textbook BFS of the first {len(sample)} of the {n} GroupBy-ordered "
                          f"sources on the same graph, {cpu_s:.1f} s"}
+        # the same BFS on one thread, over the first two sources of the sample
+        one = sample[:2]
+        t0 = time.perf_counter()
+        want1 = oracle.bfs_many(beg, adj, one, threads=1)
+        one_s = time.perf_counter() - t0
+        one_edges = sum(oracle.traversed_edges(beg, d) for d in want1)
+        cpu["single_thread"] = {"value": round(one_edges / one_s / 1e9, 4), "unit": "GTEPS",
+                                "cores": 1, "sample": f"first {len(one)} sources, {one_s:.1f} s"}
         # untimed check: the device depth arrays of the sample equal the CPU ones
         got = out.to_host(count=len(sample) * V).reshape(len(sample), V)
         if not np.array_equal(got, want):
