@@ -52,6 +52,16 @@ namespace {
 
 size_t masks_words(const ibfs_engine_s *e) { return (size_t)(2 + (e->R + 1)) * e->W; }
 
+// Waits for the stream by polling it.  The host-driven level loop waits on
+// the control-block copy after every inspection; a blocking wait's wake-up
+// cost tens of microseconds per level, more than the sparse levels' kernels.
+hipError_t poll_stream(hipStream_t st) {
+  for (;;) {
+    const hipError_t q = hipStreamQuery(st);
+    if (q != hipErrorNotReady) return q;
+  }
+}
+
 hipEvent_t next_event(ibfs_engine_s *e) {
   if (e->ev_used == e->ev.size()) {
     hipEvent_t x;
@@ -224,7 +234,7 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
                          e->st));
   if (timing) { ti1 = next_event(e); HIP_TRY(hipEventRecord(ti1, e->st)); }
   HIP_TRY(hipMemcpyAsync(e->h_ctl, &e->d_ctl[0], sizeof(LevelCtl), hipMemcpyDeviceToHost, e->st));
-  HIP_TRY(hipStreamSynchronize(e->st));
+  HIP_TRY(poll_stream(e->st));
   auto record = [&](int L, int dir, const LevelCtl &c, float exp_ms, float insp_ms) {
     ibfs_level_info li{};
     li.level = L;
@@ -293,14 +303,16 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
       if (stats) stats->n_bu_launches++;
     }
     if (timing) { x1 = next_event(e); HIP_TRY(hipEventRecord(x1, e->st)); }
+    // the inspection's start event is the expansion's end event unless a
+    // window flush runs in between
+    hipEvent_t y0 = x1;
     if ((L + 1) - base >= e->R) {  // the next copy would overwrite S_base
       int rc = emit_window(e, base, L, k, stats);
       if (rc) return rc;
       if (stats) stats->emit_windows++;
       base = L;
+      if (timing) { y0 = next_event(e); HIP_TRY(hipEventRecord(y0, e->st)); }
     }
-    hipEvent_t y0 = nullptr;
-    if (timing) { y0 = next_event(e); HIP_TRY(hipEventRecord(y0, e->st)); }
     HIP_TRY(launch_inspect(W, inspect_args(e, slot(e, L - 1), slot(e, L), slot(e, L + 1), lamp,
                                            &e->d_ctl[L % e->R], &e->d_ctl[(L + 1) % e->R],
                                            dir != 1 ? 1 : 0),
@@ -308,7 +320,7 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
     if (timing) { y1 = next_event(e); HIP_TRY(hipEventRecord(y1, e->st)); }
     HIP_TRY(hipMemcpyAsync(e->h_ctl, &e->d_ctl[L % e->R], sizeof(LevelCtl),
                            hipMemcpyDeviceToHost, e->st));
-    HIP_TRY(hipStreamSynchronize(e->st));
+    HIP_TRY(poll_stream(e->st));
     float ems = 0, ims = 0;
     if (timing) {
       HIP_TRY(hipEventElapsedTime(&ems, x0, x1));
