@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -134,8 +135,9 @@ struct ibfs_server_s {
   std::deque<PendingSrc> pending;   // worker-owned FIFO of sources to admit
   int64_t levels_issued = 0, lanes_admitted = 0;
   // sources join a running context only within this many levels of its
-  // cohort's start, so that the lanes of a context stay close in phase
-  int admit_window = 0;
+  // cohort's start, so that the lanes of a context stay close in phase (set
+  // from any thread, read by the worker)
+  std::atomic<int> admit_window{0};
   int grid = 2048;  // workgroups of the device-driven expander
 };
 
