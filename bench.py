@@ -46,12 +46,16 @@ def log(msg: str) -> None:
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
-def alg_bytes_per_kernel(log_levels, V, Vt, W, k):
+def alg_bytes_per_kernel(log_levels, V, Vt, W, k, hub_frac=0.0):
     """Algorithmic bytes of each kernel launch of one traversal, from the
     level log (B=8 offset, A=4 adjacency, Q=4 queue entry, W status bytes,
-    F=1 changed flag; status words exist for the Vt vertices with an edge)."""
+    F=1 changed flag; status words exist for the Vt vertices with an edge).
+    A hybrid level reads each candidate's word and its entries to the pulled
+    hubs, counted as hub_frac (the pulled hubs' share of all adjacency
+    entries) of the candidates' edges, an upper bound since a candidate stops
+    once it covers the active lanes; the other frontier vertices push."""
     B, A, Q, F = 8, 4, 4, 1
-    out = {"expand_td": 0, "expand_bu": 0, "inspect": 0, "emit": 0}
+    out = {"expand_td": 0, "expand_bu": 0, "expand_hy": 0, "inspect": 0, "emit": 0}
     for i, li in enumerate(log_levels):
         L = li["level"]
         q = sum(li["q_td"][:2]) * Q + li["q_td"][2] * 8 + sum(li["q_bu"]) * Q
@@ -64,6 +68,10 @@ def alg_bytes_per_kernel(log_levels, V, Vt, W, k):
         if li["direction"] == 0:
             out["expand_td"] += (prev["n_frontier"] * (2 * B + W) + prev["m_frontier"] * (A + W)
                                  + (W + F) * li["n_changed"])
+        elif li["direction"] == 2:
+            out["expand_hy"] += (prev["n_unvisited"] * (2 * B + W)
+                                 + int(hub_frac * prev["m_unvisited"]) * (A + W)
+                                 + prev["m_frontier_nonhub"] * (A + W) + W * li["n_changed"])
         else:
             out["expand_bu"] += (li["bu_work"] * (2 * B + W) + li["bu_prefix"] * (A + W)
                                  + (W + F) * li["n_changed"])
@@ -76,7 +84,7 @@ def plain_leg(g, src, K, steps, warmup, tune, emit_mode, out):
     eng = ibfs_amd.Engine(g, lane_bits=K, kernel_timing=1, emit_mode=emit_mode, **tune)
     for i in range(warmup):
         eng.run_groups(src, out=out)
-    kern = {"expand_td": 0.0, "expand_bu": 0.0, "inspect": 0.0, "emit": 0.0}
+    kern = {"expand_td": 0.0, "expand_bu": 0.0, "expand_hy": 0.0, "inspect": 0.0, "emit": 0.0}
     launches = dict.fromkeys(kern, 0)
     edges = 0
     t0 = time.perf_counter()
@@ -86,10 +94,12 @@ def plain_leg(g, src, K, steps, warmup, tune, emit_mode, out):
         edges += st.traversed_edges
         kern["expand_td"] += st.expand_td_ms
         kern["expand_bu"] += st.expand_bu_ms
+        kern["expand_hy"] += st.expand_hy_ms
         kern["inspect"] += st.inspect_ms
         kern["emit"] += st.emit_ms
         launches["expand_td"] += st.n_td_launches
         launches["expand_bu"] += st.n_bu_launches
+        launches["expand_hy"] += st.n_hy_launches
         launches["inspect"] += st.n_inspect_launches
         launches["emit"] += st.n_emit_launches
     wall = time.perf_counter() - t0
@@ -102,14 +112,15 @@ def roofline_leg(g, src, K, V, tune, kern_ms, launches, steps):
     Vt = g.traversal_vertices
     model = ibfs_amd.Engine(g, lane_bits=K, work_model=1, **tune)
     scratch = ibfs_amd.DeviceBuffer(4 * K * V)
-    alg = {"expand_td": 0, "expand_bu": 0, "inspect": 0, "emit": 0}
+    alg = {"expand_td": 0, "expand_bu": 0, "expand_hy": 0, "inspect": 0, "emit": 0}
+    hub_frac = g.hub_entries(model.cfg.hybrid_hubs) / max(g.edge_count, 1)
     levels = []
     for g0 in range(0, len(src), K):
         grp = src[g0:g0 + K]
         model.traversal(grp, out=scratch)
         lv = model.level_log()
         levels.append([(x["level"], x["direction"]) for x in lv])
-        for kname, b in alg_bytes_per_kernel(lv, V, Vt, K // 8, len(grp)).items():
+        for kname, b in alg_bytes_per_kernel(lv, V, Vt, K // 8, len(grp), hub_frac).items():
             alg[kname] += b
     dom = max(kern_ms, key=kern_ms.get)
     achieved = alg[dom] / (kern_ms[dom] * 1e-3) / 1e9 if kern_ms[dom] > 0 else 0.0
@@ -122,8 +133,8 @@ def roofline_leg(g, src, K, V, tune, kern_ms, launches, steps):
     return roof, kernels, levels
 
 
-KERNEL_NAMES = {"expand_bu": "k_expand_bu", "expand_td": "k_expand_td", "inspect": "k_inspect",
-                "emit": "k_emit"}
+KERNEL_NAMES = {"expand_bu": "k_expand_bu", "expand_td": "k_expand_td", "expand_hy": "k_expand_hy",
+                "inspect": "k_inspect", "emit": "k_emit"}
 
 
 def measured_traffic(workload, K, kernel):

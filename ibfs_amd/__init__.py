@@ -199,6 +199,12 @@ class DeviceGraph:
         self.traversal_vertices = vt.value  # vertices with an edge (status words)
         self.hub_count = hubs.value
 
+    def hub_entries(self, top: int = 0) -> int:
+        """Adjacency entries that point to the top `top` hubs (0: all hubs)."""
+        n = C.c_int64()
+        check(lib().ibfs_graph_hub_entries(self._h, top, C.byref(n)))
+        return n.value
+
     @classmethod
     def load_files(cls, beg_file: str, csr_file: str, device: int = 0) -> "DeviceGraph":
         return cls(HostCsr.load(beg_file, csr_file), device)

@@ -48,7 +48,7 @@ class Config(C.Structure):
         ("beta", C.c_int32), ("wg_size", C.c_int32), ("td_chunk", C.c_int32),
         ("ring_levels", C.c_int32), ("kernel_timing", C.c_int32), ("device", C.c_int32),
         ("work_model", C.c_int32), ("emit_mode", C.c_int32), ("dd_grid", C.c_int32),
-        ("inspect_vpt", C.c_int32), ("hybrid_gamma", C.c_int32),
+        ("inspect_vpt", C.c_int32), ("hybrid_gamma", C.c_int32), ("hybrid_hubs", C.c_int32),
     ]
 
 
@@ -117,6 +117,7 @@ SIGNATURES = {
     "ibfs_graph_load_files": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.POINTER(P)]),
     "ibfs_graph_info": (C.c_int, [P, I64P, I64P]),
     "ibfs_graph_layout": (C.c_int, [P, I64P, I64P]),
+    "ibfs_graph_hub_entries": (C.c_int, [P, C.c_int64, I64P]),
     "ibfs_graph_destroy": (C.c_int, [P]),
     "ibfs_groupby": (C.c_int, [P, I32P, C.c_int64, I32P]),
     "ibfs_engine_create": (C.c_int, [P, C.POINTER(Config), C.POINTER(P)]),

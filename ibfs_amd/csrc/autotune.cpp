@@ -74,9 +74,10 @@ void append_setting(std::string &s, const Setting &x, bool batch) {
   std::snprintf(buf, sizeof buf,
                 "{\"lane_bits\": %d, \"sml_shed\": %d, \"lrg_shed\": %d, \"policy\": %d, "
                 "\"sw_level\": %d, \"alpha\": %d, \"beta\": %d, \"td_chunk\": %d, "
-                "\"emit_mode\": %d, \"inspect_vpt\": %d, \"wg_size\": %d, \"hybrid_gamma\": %d",
+                "\"emit_mode\": %d, \"inspect_vpt\": %d, \"wg_size\": %d, \"hybrid_gamma\": %d, "
+                "\"hybrid_hubs\": %d",
                 c.lane_bits, c.sml_shed, c.lrg_shed, c.policy, c.sw_level, c.alpha, c.beta,
-                c.td_chunk, c.emit_mode, c.inspect_vpt, c.wg_size, c.hybrid_gamma);
+                c.td_chunk, c.emit_mode, c.inspect_vpt, c.wg_size, c.hybrid_gamma, c.hybrid_hubs);
   s += buf;
   if (batch) {
     std::snprintf(buf, sizeof buf, ", \"dd_grid\": %d, \"contexts\": %d, \"burst\": %d",
@@ -114,7 +115,8 @@ int tune(ibfs_graph_t g, const int32_t *src, int64_t n, const ibfs_config *start
       {"emit_mode", &ibfs_config::emit_mode, nullptr, {0, 1, 2}},
       {"inspect_vpt", &ibfs_config::inspect_vpt, nullptr, {0, 1, 2, 4, 8}},
       {"wg_size", &ibfs_config::wg_size, nullptr, {256, 512, 1024}},
-      {"hybrid_gamma", &ibfs_config::hybrid_gamma, nullptr, {0, 4, 8, 16}},
+      {"hybrid_gamma", &ibfs_config::hybrid_gamma, nullptr, {0, 8, 16, 24, 32}},
+      {"hybrid_hubs", &ibfs_config::hybrid_hubs, nullptr, {1024, 4096, 16384, 0}},
   };
   if (batch) {
     params.push_back({"dd_grid", &ibfs_config::dd_grid, nullptr, {0, 2048, 4096, 16384}});

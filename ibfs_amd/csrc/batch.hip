@@ -68,6 +68,7 @@ struct ibfs_batch_s {
   ibfs_graph_s *g = nullptr;
   ibfs_config cfg{};
   int K = 128, W = 2, R = 16, C = 2, burst = 8, chunk_shift = 12, grid = 2048;
+  HubCut hc;  // hubs a hybrid level pulls through
   std::vector<Ctx> ctx;
   u64 *d_edges = nullptr, *h_edges = nullptr;
   // closeness runs: per-source depth sums and reach counts instead of depths
@@ -152,7 +153,7 @@ InspectArgs dd_inspect_args(const ibfs_batch_s *b, const Ctx &c) {
   a.flags = c.flags;
   a.vpt = b->cfg.inspect_vpt;
   a.nw = b->cfg.wg_size / 64;
-  a.hsplit = HubSplit{b->g->hub_deg, b->g->hub_tie};
+  a.hsplit = HubSplit{b->hc.deg, b->hc.tie, b->hc.cut};
   return a;
 }
 
@@ -190,7 +191,7 @@ int alloc_ctx(ibfs_batch_s *b, Ctx &c) {
   hipError_t e1 = launch_expand_dd(b->W, b->g->d_beg, b->g->d_adj, b->g->d_tadj, c.ring, c.flags,
                                    b->R, Vt,
                                    c.ctl, c.d_plan, c.hs, c.q, b->chunk_shift,
-                                   HubSplit{b->g->hub_deg, b->g->hub_tie}, b->grid, c.st);
+                                   HubSplit{b->hc.deg, b->hc.tie, b->hc.cut}, b->grid, c.st);
   hipError_t e2 = launch_inspect(b->W, dd_inspect_args(b, c), c.st);
   if (e2 == hipSuccess) e2 = launch_plan_decide(c.d_plan, c.ctl, b->R, c.st);
   hipError_t e3 = hipStreamEndCapture(c.st, &c.graph);
@@ -216,7 +217,7 @@ int start_group(ibfs_batch_s *b, Ctx &c, const int32_t *src, int64_t g0, int k, 
   }
   for (int w = 0; w < W; ++w) c.h_masks[W + w] = c.h_masks[w];
   for (size_t i = 0; i < (size_t)(b->R + 1) * W; ++i) c.h_masks[2 * W + i] = ~0ull;
-  plan_init(c.h_plan, b->cfg, b->g->rho_q10, (u64)V);
+  plan_init(c.h_plan, b->cfg, b->hc.rho_q10, (u64)V);
   HIP_TRY(hipMemcpyAsync(c.d_masks, c.h_masks, sizeof(u64) * masks_words(b), hipMemcpyHostToDevice, c.st));
   HIP_TRY(hipMemcpyAsync(c.d_src, c.h_src, sizeof(int32_t) * k, hipMemcpyHostToDevice, c.st));
   HIP_TRY(hipMemcpyAsync(c.d_lanes, c.h_lanes, sizeof(LaneOut) * k, hipMemcpyHostToDevice, c.st));
@@ -317,6 +318,7 @@ int ibfs_batch_create(ibfs_graph_t g, const ibfs_config *cfg_in, int32_t context
   // 32 workgroups per CU: a fixed grid strides over the queues, and fewer
   // workgroups leave too few gathers in flight (measured on R-MAT s22)
   b->grid = cfg.dd_grid > 0 ? cfg.dd_grid : cus * 32;
+  b->hc = hub_cut(b->g, cfg.hybrid_hubs);
   b->ctx.resize(contexts);
   hipError_t err = hipMalloc((void **)&b->d_edges, sizeof(u64));
   if (err == hipSuccess) err = hipHostMalloc((void **)&b->h_edges, sizeof(u64), hipHostMallocDefault);

@@ -40,6 +40,7 @@ inline int check_direction_config(const ibfs_config &c) {
     return fail(IBFS_ERR_ARG, "alpha and beta must be >= 1");
   if (c.hybrid_gamma < 0 || c.hybrid_gamma > 4096)
     return fail(IBFS_ERR_ARG, "hybrid_gamma must be in [0, 4096] (got %d)", c.hybrid_gamma);
+  if (c.hybrid_hubs < 0) return fail(IBFS_ERR_ARG, "hybrid_hubs must be >= 0");
   return IBFS_OK;
 }
 
@@ -74,7 +75,17 @@ struct ibfs_graph_s {
   std::vector<int32_t> h_cid;
   int64_t H = 0;
   int64_t max_deg = 0;
-  // hub membership: deg > hub_deg, or deg == hub_deg and id <= hub_tie
-  int32_t hub_deg = INT32_MAX, hub_tie = -1;
-  int32_t rho_q10 = 0;  // hub entries of the traversal adjacency per entry, x1024
+  // per hub slot (descending degree, ties by id): degree and traversal id;
+  // hub_entries[i] = traversal-adjacency entries that point to slots < i
+  std::vector<int32_t> h_hub_deg, h_hub_id;
+  std::vector<int64_t> h_hub_entries;
 };
+
+// Pull cut of hybrid levels at the top r hubs (r <= 0 or r > H: all H hubs):
+// the degree and id of the last pulled hub (deg INT32_MAX, tie -1 when there
+// is none), the number of pulled hubs, and the fraction of adjacency entries
+// that point to them (x1024).
+struct HubCut {
+  int32_t deg = INT32_MAX, tie = -1, cut = 0, rho_q10 = 0;
+};
+HubCut hub_cut(const ibfs_graph_s *g, int64_t r);

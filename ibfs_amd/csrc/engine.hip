@@ -25,6 +25,7 @@ struct ibfs_engine_s {
   ibfs_config cfg{};
   int K = 128, W = 2, R = 32, chunk_shift = 12;
   int hy_grid = 8192;  // workgroups of a hybrid expansion
+  HubCut hc;           // hubs a hybrid level pulls through
   hipStream_t st = nullptr;
   u64 *ring = nullptr;
   LevelCtl *d_ctl = nullptr;
@@ -138,7 +139,7 @@ int64_t isolated_sources(const ibfs_graph_s *g, const int32_t *src, int k) {
 // counters of L-1
 int decide(const ibfs_engine_s *e, int L, const LevelCtl &c, int dir_prev, u64 nf_prev) {
   const ibfs_config &k = e->cfg;
-  return choose_direction(k.policy, k.sw_level, k.alpha, k.beta, k.hybrid_gamma, e->g->rho_q10,
+  return choose_direction(k.policy, k.sw_level, k.alpha, k.beta, k.hybrid_gamma, e->hc.rho_q10,
                           (u64)e->g->V, L, dir_prev, nf_prev, c.n_front, c.m_front, c.m_cand,
                           c.m_front_nh);
 }
@@ -167,7 +168,7 @@ InspectArgs inspect_args(ibfs_engine_s *e, const u64 *prev, const u64 *cur, u64 
   a.flags = e->flags;
   a.vpt = e->cfg.inspect_vpt;
   a.nw = e->cfg.wg_size / 64;
-  a.hsplit = HubSplit{e->g->hub_deg, e->g->hub_tie};
+  a.hsplit = HubSplit{e->hc.deg, e->hc.tie, e->hc.cut};
   return a;
 }
 
@@ -286,7 +287,7 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
     } else if (dir == 2) {
       HIP_TRY(launch_expand_hy(W, g->d_beg, g->d_adj, g->d_tadj, slot(e, L - 1), e->hs,
                                slot(e, L), &e->d_ctl[(L - 1) % e->R], e->q, e->chunk_shift,
-                               HubSplit{g->hub_deg, g->hub_tie}, e->hy_grid, e->st));
+                               HubSplit{e->hc.deg, e->hc.tie, e->hc.cut}, e->hy_grid, e->st));
       if (stats) stats->n_hy_launches++;
     } else {
       if (e->cfg.work_model) {  // instrumentation: exact early-termination prefixes
@@ -377,6 +378,19 @@ int ensure_out(ibfs_engine_s *e, size_t elems) {
 
 }  // namespace
 
+HubCut hub_cut(const ibfs_graph_s *g, int64_t r) {
+  HubCut c;
+  const int64_t H = g->H;
+  if (r <= 0 || r > H) r = H;
+  c.cut = (int32_t)r;
+  if (r > 0) {
+    c.deg = g->h_hub_deg[(size_t)r - 1];
+    c.tie = g->h_hub_id[(size_t)r - 1];
+  }
+  c.rho_q10 = g->E > 0 ? (int32_t)((g->h_hub_entries[(size_t)r] * 1024) / g->E) : 0;
+  return c;
+}
+
 extern "C" {
 
 int ibfs_config_default(ibfs_config *cfg) {
@@ -395,6 +409,13 @@ int ibfs_config_default(ibfs_config *cfg) {
   // switch when the frontier's edges exceed half of the candidates' edges.
   cfg->alpha = 2;
   cfg->beta = 48;
+  // A top-down level whose frontier edges mostly leave the top 4,096 hubs
+  // runs hybrid: the candidates pull through those hubs (a pulled entry
+  // costs about a quarter of a pushed edge), the rest of the frontier pushes.
+  // Measured on R-MAT s22: plain 7.5K -> 7.85K GTEPS (128 lanes), batched
+  // 9.7K -> 10.4K.
+  cfg->hybrid_gamma = 16;
+  cfg->hybrid_hubs = 4096;
   cfg->wg_size = 512;
   cfg->td_chunk = 2048;
   cfg->ring_levels = 32;
@@ -551,14 +572,18 @@ int ibfs_graph_create_ex(const int64_t *beg, int64_t V, const int32_t *adj, int6
   g->Vt = Vt;
   g->H = H;
   g->max_deg = maxd;
-  if (H > 0) {
-    g->hub_deg = tdeg[order[H - 1]];
-    g->hub_tie = order[H - 1];
+  g->h_hub_deg.resize((size_t)H);
+  g->h_hub_id.resize((size_t)H);
+  for (int64_t i = 0; i < H; ++i) {
+    g->h_hub_id[i] = order[i];
+    g->h_hub_deg[i] = tdeg[order[i]];
   }
-  {
-    int64_t hub_entries = 0;
-    for (int64_t j = 0; j < E; ++j) hub_entries += (tadj[j] < 0);
-    g->rho_q10 = E > 0 ? (int32_t)((hub_entries * 1024) / E) : 0;
+  {  // entries per hub slot, then the prefix over slots
+    std::vector<int64_t> per((size_t)H + 1, 0);
+    for (int64_t j = 0; j < E; ++j)
+      if (tadj[j] < 0) ++per[(size_t)(tadj[j] & 0x7fffffff) + 1];
+    for (int64_t i = 1; i <= H; ++i) per[i] += per[i - 1];
+    g->h_hub_entries = std::move(per);
   }
   const size_t eb = sizeof(int32_t) * (size_t)std::max<int64_t>(E, 1);
   const size_t vtb = sizeof(int32_t) * (size_t)std::max<int64_t>(Vt, 1);
@@ -616,6 +641,13 @@ int ibfs_graph_layout(ibfs_graph_t g, int64_t *traversal_vertices, int64_t *hubs
   if (!g) return fail(IBFS_ERR_ARG, "null graph");
   if (traversal_vertices) *traversal_vertices = g->Vt;
   if (hubs) *hubs = g->H;
+  return IBFS_OK;
+}
+
+int ibfs_graph_hub_entries(ibfs_graph_t g, int64_t top, int64_t *entries) {
+  if (!g || !entries) return fail(IBFS_ERR_ARG, "ibfs_graph_hub_entries: null argument");
+  if (top <= 0 || top > g->H) top = g->H;
+  *entries = g->h_hub_entries[(size_t)top];
   return IBFS_OK;
 }
 
@@ -689,6 +721,7 @@ int ibfs_engine_create(ibfs_graph_t g, const ibfs_config *cfg_in, ibfs_engine_t 
       cus = 256;
     e->hy_grid = cfg.dd_grid > 0 ? cfg.dd_grid : cus * 32;
   }
+  e->hc = hub_cut(g, cfg.hybrid_hubs);
   const int64_t Vt = std::max<int64_t>(g->Vt, 1);
   // ring: at most 8 GiB of snapshots, at least 4 levels
   const size_t slot_bytes = (size_t)Vt * (size_t)e->W * sizeof(u64);

@@ -69,14 +69,15 @@ inline void plan_init(Plan *p, const ibfs_config &c, int rho, u64 V) {
   p->dir = -1;
 }
 
-// Hub membership of a traversal vertex: the hubs are the H highest-degree
-// vertices, ties by id, so v is a hub iff deg(v) > deg or (deg(v) == deg and
-// v <= tie) (deg = degree of the last hub, tie = its id; no hubs: deg =
-// INT32_MAX, tie = -1).  Each traversal-adjacency list holds its hub
-// neighbours as a prefix.
+// Pulled hubs of a hybrid level: the top `cut` hubs by degree, ties by id,
+// so v is one iff deg(v) > deg or (deg(v) == deg and v <= tie) (deg = degree
+// of the last pulled hub, tie = its id; none: deg = INT32_MAX, tie = -1).
+// Each traversal-adjacency list holds its hub neighbours as a prefix in slot
+// order, so the pulled ones are a prefix too.
 struct HubSplit {
   int32_t deg;
   int32_t tie;
+  int32_t cut;  // hub slots a hybrid level pulls through: [0, cut)
 };
 
 // Direction of level Ln (0 top-down, 1 bottom-up, 2 hybrid) from the

@@ -415,14 +415,19 @@ __device__ __forceinline__ void bu_commit(u64 *cur, uint8_t *flags, int64_t v, c
 // order (descending degree, hubs first) and stops as soon as its word
 // covers lam_{L-1} (bitwise early termination, expander.cu:258-359).
 // One candidate per thread, IBFS_BU_STEP neighbours per step.
-// HY (hybrid level): only the hub prefix of the list is read (from the hub
-// cache), and new bits go in with atomic ORs (non-hub frontier vertices push
-// into the same words concurrently); nothing is flagged.
+// HY (hybrid level): only the prefix of hub entries with slot < cut is read
+// (from the hub cache), and new bits go in with atomic ORs (the other
+// frontier vertices push into the same words concurrently); nothing is
+// flagged.
+__device__ __forceinline__ bool pulled(int32_t a, int cut) {
+  return a < 0 && (a & 0x7fffffff) < cut;
+}
+
 template <int W, bool HY = false>
 __device__ __forceinline__ void bu_thread(const int64_t *__restrict__ beg,
                                           const int32_t *__restrict__ tadj, const Gather &gs,
                                           u64 *cur, uint8_t *flags,
-                                          const u64 (&lam)[W], int64_t v) {
+                                          const u64 (&lam)[W], int64_t v, int cut = 0) {
   u64 x[W], x0[W];
   load_word<W>(cur + v * W, x);
 #pragma unroll
@@ -439,7 +444,7 @@ __device__ __forceinline__ void bu_thread(const int64_t *__restrict__ beg,
     bool nonhub = false;
 #pragma unroll
     for (int r = 0; r < IBFS_BU_STEP; ++r) {
-      if (HY && an[r] >= 0) {
+      if (HY && !pulled(an[r], cut)) {
         nonhub = true;
 #pragma unroll
         for (int w = 0; w < W; ++w) sn[r][w] = 0;
@@ -462,7 +467,7 @@ template <int W, bool HY = false>
 __device__ __forceinline__ void bu_wave(const int64_t *__restrict__ beg,
                                         const int32_t *__restrict__ tadj, const Gather &gs,
                                         u64 *cur, uint8_t *flags,
-                                        const u64 (&lam)[W], int64_t v, int lane) {
+                                        const u64 (&lam)[W], int64_t v, int lane, int cut = 0) {
   u64 x[W], x0[W];
   load_word<W>(cur + v * W, x);
 #pragma unroll
@@ -477,7 +482,7 @@ __device__ __forceinline__ void bu_wave(const int64_t *__restrict__ beg,
     bool nonhub = false;
     if (j < e) {
       const int32_t a = tadj[j];
-      if (HY && a >= 0) nonhub = true;
+      if (HY && !pulled(a, cut)) nonhub = true;
       else load_nbr<W>(gs, a, y);
     }
 #pragma unroll
@@ -496,10 +501,26 @@ __device__ __forceinline__ void bu_block(const int64_t *__restrict__ beg,
                                          const int32_t *__restrict__ tadj, const Gather &gs,
                                          u64 *cur, uint8_t *flags,
                                          const u64 (&lam)[W], int64_t v, u64 (*part)[W],
-                                         int *nh) {
+                                         int *nh, int cut = 0) {
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   u64 x[W], x0[W];
-  load_word<W>(cur + v * W, x);
+  if (HY) {
+    // One read of the candidate's word for the whole workgroup: in a hybrid
+    // level pushes from other workgroups can change it between the waves'
+    // own reads, and every wave must take the same covers/stop branches
+    // around the barriers below.
+    if (t == 0) {
+      load_word<W>(cur + v * W, x);
+#pragma unroll
+      for (int w = 0; w < W; ++w) part[0][w] = x[w];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < W; ++w) x[w] = part[0][w];
+    __syncthreads();
+  } else {
+    load_word<W>(cur + v * W, x);
+  }
 #pragma unroll
   for (int w = 0; w < W; ++w) x0[w] = x[w];
   if (covers<W>(x, lam)) return;
@@ -512,7 +533,7 @@ __device__ __forceinline__ void bu_block(const int64_t *__restrict__ beg,
     bool nonhub = false;
     if (j < e) {
       const int32_t a = tadj[j];
-      if (HY && a >= 0) nonhub = true;
+      if (HY && !pulled(a, cut)) nonhub = true;
       else load_nbr<W>(gs, a, y);
     }
 #pragma unroll
@@ -595,11 +616,11 @@ __global__ __launch_bounds__(BS) void k_expand_bu(const int64_t *__restrict__ be
 }
 
 // Hybrid level from the queues of ctl (the control block of the level
-// before), grid-strided: the bottom-up candidates pull through their hub
-// prefixes, then the top-down frontier vertices that are not hubs push.  A
-// hub frontier vertex u reaches every candidate neighbour v through v's hub
-// prefix (hub neighbours are a prefix of every traversal-adjacency list);
-// a neighbour that is no candidate already holds every lane u can bring.
+// before), grid-strided: the bottom-up candidates pull through their prefix
+// of pulled hubs (the top hsp.cut), then the other top-down frontier
+// vertices push.  A pulled-hub frontier vertex u reaches every candidate
+// neighbour v through that prefix of v's traversal-adjacency list; a
+// neighbour that is no candidate already holds every lane u can bring.
 template <int W>
 __device__ __forceinline__ void expand_hybrid(const int64_t *__restrict__ beg,
                                               const int32_t *__restrict__ adj,
@@ -611,11 +632,11 @@ __device__ __forceinline__ void expand_hybrid(const int64_t *__restrict__ beg,
   const int64_t G = gridDim.x;
   const int64_t bs = ctl->q[Q_BU_S], bm = ctl->q[Q_BU_M], bl = ctl->q[Q_BU_L];
   for (int64_t i = (int64_t)blockIdx.x * BS + t; i < bs; i += G * BS)
-    bu_thread<W, true>(beg, tadj, gs, cur, nullptr, lam, q.q[Q_BU_S][i]);
+    bu_thread<W, true>(beg, tadj, gs, cur, nullptr, lam, q.q[Q_BU_S][i], hsp.cut);
   for (int64_t i = (int64_t)blockIdx.x * 4 + wv; i < bm; i += G * 4)
-    bu_wave<W, true>(beg, tadj, gs, cur, nullptr, lam, q.q[Q_BU_M][i], lane);
+    bu_wave<W, true>(beg, tadj, gs, cur, nullptr, lam, q.q[Q_BU_M][i], lane, hsp.cut);
   for (int64_t i = blockIdx.x; i < bl; i += G)
-    bu_block<W, true>(beg, tadj, gs, cur, nullptr, lam, q.q[Q_BU_L][i], part, nh);
+    bu_block<W, true>(beg, tadj, gs, cur, nullptr, lam, q.q[Q_BU_L][i], part, nh, hsp.cut);
   const int64_t ts = ctl->q[Q_TD_S], tm = ctl->q[Q_TD_M], tl = ctl->q[Q_TD_L];
   for (int64_t i = (int64_t)blockIdx.x * BS + t; i < ts; i += G * BS) {
     const int32_t u = q.q[Q_TD_S][i];

@@ -139,6 +139,7 @@ struct ibfs_server_s {
   // from any thread, read by the worker)
   std::atomic<int> admit_window{0};
   int grid = 2048;  // workgroups of the device-driven expander
+  HubCut hc;        // hubs a hybrid level pulls through
 };
 
 namespace {
@@ -205,7 +206,7 @@ InspectArgs plan_args(const ibfs_server_s *s, const SCtx &c) {
   a.flags = c.flags;
   a.vpt = s->cfg.inspect_vpt;
   a.nw = s->cfg.wg_size / 64;
-  a.hsplit = HubSplit{s->g->hub_deg, s->g->hub_tie};
+  a.hsplit = HubSplit{s->hc.deg, s->hc.tie, s->hc.cut};
   return a;
 }
 
@@ -246,7 +247,7 @@ int alloc_ctx(ibfs_server_s *s, SCtx &c) {
   HIP_CHK(hipStreamBeginCapture(c.st, hipStreamCaptureModeThreadLocal));
   hipError_t e1 = launch_expand_dd(s->W, s->g->d_beg, s->g->d_adj, s->g->d_tadj, c.ring, c.flags,
                                    s->R, s->g->Vt, c.ctl, c.d_plan, c.hs, c.q, s->chunk_shift,
-                                   HubSplit{s->g->hub_deg, s->g->hub_tie}, s->grid, c.st);
+                                   HubSplit{s->hc.deg, s->hc.tie, s->hc.cut}, s->grid, c.st);
   hipError_t e2 = launch_inspect(s->W, plan_args(s, c), c.st);
   if (e2 == hipSuccess) e2 = launch_plan_decide(c.d_plan, c.ctl, s->R, c.st);
   hipError_t e3 = hipStreamEndCapture(c.st, &c.graph);
@@ -343,7 +344,7 @@ int inspect_level(ibfs_server_s *s, SCtx &c, int L, const u64 *filter, const u64
   a.flags = c.flags;
   a.vpt = s->cfg.inspect_vpt;
   a.nw = s->cfg.wg_size / 64;
-  a.hsplit = HubSplit{s->g->hub_deg, s->g->hub_tie};
+  a.hsplit = HubSplit{s->hc.deg, s->hc.tie, s->hc.cut};
   a.full_diff = full_diff;
   HIP_CHK(launch_inspect(W, a, c.st));
   HIP_CHK(hipMemcpyAsync(c.h_ctl, c.ctl + L % s->R, sizeof(LevelCtl), hipMemcpyDeviceToHost, c.st));
@@ -373,7 +374,7 @@ int start_ctx(ibfs_server_s *s, SCtx &c) {
 
 int decide(const ibfs_server_s *s, int Ln, const LevelCtl &lc, int dir_prev, u64 nf_prev) {
   const ibfs_config &k = s->cfg;
-  return choose_direction(k.policy, k.sw_level, k.alpha, k.beta, k.hybrid_gamma, s->g->rho_q10,
+  return choose_direction(k.policy, k.sw_level, k.alpha, k.beta, k.hybrid_gamma, s->hc.rho_q10,
                           (u64)s->g->V, Ln, dir_prev, nf_prev, lc.n_front, lc.m_front, lc.m_cand,
                           lc.m_front_nh);
 }
@@ -461,7 +462,7 @@ int process_level(ibfs_server_s *s, SCtx &c) {
   else if (dir == 2)
     HIP_CHK(launch_expand_hy(W, s->g->d_beg, s->g->d_adj, s->g->d_tadj, slot(s, c, L), c.hs,
                              slot(s, c, Ln), c.ctl + L % s->R, c.q, s->chunk_shift,
-                             HubSplit{s->g->hub_deg, s->g->hub_tie}, s->grid, c.st));
+                             HubSplit{s->hc.deg, s->hc.tie, s->hc.cut}, s->grid, c.st));
   else
     HIP_CHK(launch_expand_bu(W, s->g->d_beg, s->g->d_tadj, slot(s, c, L), c.hs, slot(s, c, Ln),
                              c.flags, lamp, c.q, lc.q[Q_BU_S], lc.q[Q_BU_M], lc.q[Q_BU_L], c.st));
@@ -542,7 +543,7 @@ int start_cohort(ibfs_server_s *s, SCtx &c) {
   int rc = admit(s, c, 0, c.h_all, &n);
   if (rc) return rc;
   HIP_CHK(hipMemcpyAsync(c.d_all, c.h_all, sizeof(u64) * W, hipMemcpyHostToDevice, c.st));
-  plan_init(c.h_plan, s->cfg, s->g->rho_q10, (u64)s->g->V);
+  plan_init(c.h_plan, s->cfg, s->hc.rho_q10, (u64)s->g->V);
   HIP_CHK(hipMemcpyAsync(c.d_plan, c.h_plan, sizeof(Plan), hipMemcpyHostToDevice, c.st));
   HIP_CHK(launch_inspect(W, plan_args(s, c), c.st));
   HIP_CHK(launch_plan_decide(c.d_plan, c.ctl, s->R, c.st));
@@ -680,6 +681,7 @@ int ibfs_server_create(ibfs_graph_t g, const ibfs_config *cfg_in, int64_t status
       cus <= 0)
     cus = 256;
   s->grid = cfg.dd_grid > 0 ? cfg.dd_grid : cus * 32;
+  s->hc = hub_cut(s->g, cfg.hybrid_hubs);
   s->ctx.resize(C);
   for (auto &c : s->ctx) {
     int rc = alloc_ctx(s, c);

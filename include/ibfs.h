@@ -77,6 +77,8 @@ typedef struct ibfs_config {
                           gamma/64 * (hub entries of the candidates) + (edges of
                           non-hub frontier vertices) < (frontier edges);
                           0: never                                            */
+  int32_t hybrid_hubs; /* hubs a hybrid level pulls through: the top
+                          hybrid_hubs by degree (0: every cached hub)         */
 } ibfs_config;
 
 /* Per-level record of the last traversal (the build's monitoring log, in place
@@ -197,6 +199,10 @@ int ibfs_graph_info(ibfs_graph_t g, int64_t *vert_count, int64_t *edge_count);
 /* Device layout sizes: vertices with status words (an edge in or out) and
  * hub-cache entries.  No reference counterpart (layout introspection). */
 int ibfs_graph_layout(ibfs_graph_t g, int64_t *traversal_vertices, int64_t *hubs);
+/* Adjacency entries that point to the top `top` hubs by degree (top <= 0 or
+ * above the hub count: all hubs), the entries a hybrid level pulling through
+ * them can read.  No reference counterpart (layout introspection). */
+int ibfs_graph_hub_entries(ibfs_graph_t g, int64_t top, int64_t *entries);
 int ibfs_graph_destroy(ibfs_graph_t g);
 
 /* GroupBy on the device: a permutation of src_list in which sources of degree

@@ -77,3 +77,26 @@ def test_chung_lu_3m_512_sources():
     for i, r in enumerate(rows):
         np.testing.assert_array_equal(out.to_host(count=V, offset=int(r) * V), want[i],
                                       err_msg=f"batched, source row {r}")
+
+
+def test_hybrid_levels_with_pushes_into_workgroup_candidates():
+    """Hybrid levels on a Chung-Lu graph whose high-degree vertices are
+    bottom-up candidates of the workgroup class while the frontier vertices
+    outside the pulled hubs push into them: every workgroup must branch on one
+    read of its candidate's word (regression: the waves used to read it
+    separately, disagree on covering the active lanes after a concurrent push,
+    and meet at mismatched barriers).  Every level is hybrid, with 64 and
+    4,096 pulled hubs, three runs each, against the oracle."""
+    h = ibfs_amd.HostCsr.chung_lu(400_000, 2.2, 76.0, 20000.0, seed=5, threads=8)
+    beg, adj = h.arrays()
+    g = ibfs_amd.DeviceGraph(h)
+    src = g.groupby(h.gen_sources(512, seed=6))
+    want = oracle.bfs_many(beg, adj, src, threads=8)
+    for cut in (64, 4096):
+        opts = dict(policy=ibfs_amd.POLICY_HYBRID, hybrid_hubs=cut, lane_bits=256)
+        eng = ibfs_amd.Engine(g, **opts)
+        b = ibfs_amd.Batch(g, contexts=2, burst=4, **opts)
+        for rep in range(3):
+            say(f"[gpu] chung-lu 400K hybrid cut={cut} run {rep}")
+            np.testing.assert_array_equal(eng.run_groups(src), want)
+            np.testing.assert_array_equal(b.run(src), want)

@@ -20,6 +20,8 @@ POLICIES = {
     "hy": dict(policy=ibfs_amd.POLICY_HYBRID),
     # alpha/beta with hybrid levels whenever pulling is priced at 1/64 of a push
     "abh": dict(policy=ibfs_amd.POLICY_ALPHA_BETA, hybrid_gamma=1),
+    # alpha/beta without hybrid levels
+    "ab0": dict(policy=ibfs_amd.POLICY_ALPHA_BETA, hybrid_gamma=0),
 }
 
 
@@ -454,27 +456,30 @@ def test_cli_runs_reference_arguments(tmp_path):
     assert r.stdout.count("Traversal-time:") == 2 and "Total time:" in r.stdout
 
 
+@pytest.mark.parametrize("cut", [0, 100])
 @pytest.mark.parametrize("hubs", [0, 64, 4096, None])
 @pytest.mark.parametrize("pol", ["hy", "abh"])
-def test_hybrid_levels_every_hub_split(s16, hubs, pol):
-    """Hybrid levels (candidates pull through their hub prefixes, non-hub
-    frontier vertices push) with no hubs (all push), few, many and the
-    default count (every vertex a hub on s16: all pull), through the engine,
-    the batched launcher and both serving paths."""
-    say(f"[gpu] hybrid hubs={hubs} policy={pol}")
+def test_hybrid_levels_every_hub_split(s16, hubs, pol, cut):
+    """Hybrid levels (candidates pull through their prefix of pulled hubs,
+    the other frontier vertices push) with no hubs (all push), few, many and
+    the default count (every vertex a hub on s16: all pull), pulling through
+    every cached hub (cut 0) or the top 100, through the engine, the batched
+    launcher and both serving paths."""
+    say(f"[gpu] hybrid hubs={hubs} policy={pol} cut={cut}")
     g = ibfs_amd.DeviceGraph(s16["h"], hub_count=hubs) if hubs is not None else s16["g"]
     src, want = s16["src"], s16["want"]
+    pol = dict(POLICIES[pol], hybrid_hubs=cut)
     for lb in (64, 256):
-        eng = ibfs_amd.Engine(g, lane_bits=lb, kernel_timing=1, **POLICIES[pol])
+        eng = ibfs_amd.Engine(g, lane_bits=lb, kernel_timing=1, **pol)
         np.testing.assert_array_equal(eng.run_groups(src), want)
         dirs = [x["direction"] for x in eng.level_log()]
-        if pol == "hy":
+        if pol["policy"] == ibfs_amd.POLICY_HYBRID:
             assert set(dirs[1:]) == {2} and eng.last_stats.n_hy_launches > 0
-        b = ibfs_amd.Batch(g, contexts=2, burst=3, lane_bits=lb, **POLICIES[pol])
+        b = ibfs_amd.Batch(g, contexts=2, burst=3, lane_bits=lb, **pol)
         np.testing.assert_array_equal(b.run(src), want)
     V = s16["h"].vert_count
     for window in (0, 2):
-        srv = ibfs_amd.Server(g, lane_bits=64, ring_levels=8, admit_window=window, **POLICIES[pol])
+        srv = ibfs_amd.Server(g, lane_bits=64, ring_levels=8, admit_window=window, **pol)
         queries = [src[0:40], src[40:41], src[41:128]]
         outs, _ = _serve(srv, queries, V, gap_s=0.0005)
         for q, o in zip(queries, outs):
@@ -500,10 +505,26 @@ def test_hybrid_is_chosen_on_hub_heavy_levels(s16):
 
 
 def test_direction_config_checks(s16):
-    for bad in (dict(policy=5), dict(hybrid_gamma=-1), dict(hybrid_gamma=5000)):
+    for bad in (dict(policy=5), dict(hybrid_gamma=-1), dict(hybrid_gamma=5000),
+                dict(hybrid_hubs=-1)):
         with pytest.raises(ibfs_amd.IbfsError):
             ibfs_amd.Engine(s16["g"], **bad)
         with pytest.raises(ibfs_amd.IbfsError):
             ibfs_amd.Batch(s16["g"], **bad)
         with pytest.raises(ibfs_amd.IbfsError):
             ibfs_amd.Server(s16["g"], **bad)
+
+
+def test_hub_entries_match_the_adjacency(s16):
+    """ibfs_graph_hub_entries counts the adjacency entries pointing to the
+    top hubs by degree (ties by id), for every cut."""
+    beg, adj = s16["beg"], s16["adj"]
+    deg = np.diff(beg)
+    nz = np.flatnonzero((deg > 0) | (np.bincount(adj, minlength=len(deg)) > 0))
+    order = nz[np.argsort(-deg[nz], kind="stable")]
+    g = ibfs_amd.DeviceGraph(s16["h"], hub_count=5000)
+    for top in (1, 100, 4096, 5000, 0, 10 ** 9):
+        r = 5000 if top <= 0 or top > 5000 else top
+        hub = np.zeros(len(deg), bool)
+        hub[order[:r]] = True
+        assert g.hub_entries(top) == int(hub[adj].sum()), top

@@ -38,6 +38,8 @@ def test_abi_version_and_defaults():
     assert _lib.lib().ibfs_abi_version() == 1
     cfg = ibfs_amd.default_config()
     assert cfg.lane_bits == 128 and cfg.sml_shed == 64 and cfg.lrg_shed == 512
+    assert cfg.policy == _lib.POLICY_ALPHA_BETA
+    assert cfg.hybrid_gamma == 16 and cfg.hybrid_hubs == 4096
     assert cfg.ring_levels >= 4
 
 
@@ -186,6 +188,7 @@ def test_c_abi_argument_checks():
     assert l.ibfs_groupby(None, None, 0, None) == _lib.IBFS_ERR_ARG
     assert l.ibfs_graph_info(None, None, None) == _lib.IBFS_ERR_ARG
     assert l.ibfs_graph_layout(None, None, None) == _lib.IBFS_ERR_ARG
+    assert l.ibfs_graph_hub_entries(None, 0, None) == _lib.IBFS_ERR_ARG
     assert l.ibfs_server_set_admission(None, 1) == _lib.IBFS_ERR_ARG
     assert l.ibfs_autotune_batch(None, None, 0, None, 2, 8, 1, 1, None, None, None, None, 0) \
         == _lib.IBFS_ERR_ARG
