@@ -157,8 +157,8 @@ def measured_traffic(workload, K, kernel):
 
 # batched-leg settings found by ibfs_autotune_batch on this workload (R-MAT s22,
 # the 1,024 GroupBy-ordered sources): 256 lanes, 2 contexts, bursts of 4,
-# lrg_shed 4096, beta 96
-BATCH_TUNED = {"lrg_shed": 4096, "beta": 96}
+# lrg_shed 4096, beta 96, top-down hub chunks of 8,192 edges
+BATCH_TUNED = {"lrg_shed": 4096, "beta": 96, "td_chunk": 8192}
 
 
 def batch_leg(g, src, K, contexts, burst, steps, warmup, tune, emit_mode, out):
