@@ -125,13 +125,15 @@ void release_batch(ibfs_batch_s *b) {
 
 // Output of the snapshot window [lo, hi] of context c: depth arrays, or the
 // closeness sums of the group's sources.
-hipError_t window_out(ibfs_batch_s *b, Ctx &c, int lo, int hi) {
-  const EmitSpace sp{b->g->d_cid, b->g->V, b->g->Vt};
+// edges (a final depth window only, or null): the emission also adds the
+// TEPS numerator.
+hipError_t window_out(ibfs_batch_s *b, Ctx &c, int lo, int hi, u64 *edges = nullptr) {
+  const EmitSpace sp{b->g->d_cid, b->g->V, b->g->Vt, b->g->d_deg};
   if (b->closeness)
     return launch_close(b->W, c.ring, b->R, sp, lo, hi, c.d_lanes, c.d_masks + 2 * b->W,
                         c.d_masks + b->W, c.k, b->d_sum + c.g0, b->d_reach + c.g0, c.st);
   return launch_emit(b->W, b->cfg.emit_mode, 0, c.ring, b->R, sp, lo, hi, c.d_lanes,
-                     c.d_masks + 2 * b->W, c.d_masks + b->W, c.k, c.st);
+                     c.d_masks + 2 * b->W, c.d_masks + b->W, c.k, edges, c.st);
 }
 
 InspectArgs dd_inspect_args(const ibfs_batch_s *b, const Ctx &c) {
@@ -253,14 +255,17 @@ int launch_burst(ibfs_batch_s *b, Ctx &c, ibfs_stats *stats) {
 }
 
 int finish_group(ibfs_batch_s *b, Ctx &c, int last, ibfs_stats *stats) {
+  // a final depth window counts the TEPS numerator during its pass
+  const bool fused = stats && !b->closeness && last >= c.base;
   if (last >= c.base) {
-    HIP_TRY(window_out(b, c, c.base, last));
+    HIP_TRY(window_out(b, c, c.base, last, fused ? b->d_edges : nullptr));
     if (stats) stats->emit_windows++;
   }
   if (stats) {
     stats->levels = std::max(stats->levels, last + 1);
-    HIP_TRY(launch_count_edges(b->W, slot(b, c, last), b->g->d_deg, b->g->Vt, c.d_masks,
-                               b->d_edges, c.st));
+    if (!fused)
+      HIP_TRY(launch_count_edges(b->W, slot(b, c, last), b->g->d_deg, b->g->Vt, c.d_masks,
+                                 b->d_edges, c.st));
   }
   if (c.out_host)
     HIP_TRY(hipMemcpyAsync(c.out_host, c.d_stage, sizeof(int32_t) * (size_t)c.k * (size_t)b->g->V,

@@ -123,7 +123,7 @@ u64 *slot(ibfs_engine_s *e, int L) {
   return e->ring + (size_t)(L % e->R) * (size_t)e->g->Vt * (size_t)e->W;
 }
 
-EmitSpace emit_space(const ibfs_graph_s *g) { return EmitSpace{g->d_cid, g->V, g->Vt}; }
+EmitSpace emit_space(const ibfs_graph_s *g) { return EmitSpace{g->d_cid, g->V, g->Vt, g->d_deg}; }
 
 // Distinct sources without a status word (isolated vertices): they change
 // at their admission level like any source, but outside the status words.
@@ -173,14 +173,16 @@ InspectArgs inspect_args(ibfs_engine_s *e, const u64 *prev, const u64 *cur, u64 
 }
 
 // Emits the window [b, en] for the lanes in e->h_lanes (already on device).
-int emit_window(ibfs_engine_s *e, int b, int en, int k, ibfs_stats *stats) {
+// edges (final window only, or null): the pass also adds the TEPS numerator.
+int emit_window(ibfs_engine_s *e, int b, int en, int k, ibfs_stats *stats, u64 *edges) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (e->cfg.kernel_timing) {
     e0 = next_event(e);
     HIP_TRY(hipEventRecord(e0, e->st));
   }
   HIP_TRY(launch_emit(e->W, e->cfg.emit_mode, e->emit_vec4, e->ring, e->R, emit_space(e->g), b,
-                      en, e->d_lanes, e->d_masks + 2 * e->W, e->d_masks + e->W, k, e->st));
+                      en, e->d_lanes, e->d_masks + 2 * e->W, e->d_masks + e->W, k, edges,
+                      e->st));
   if (e->cfg.kernel_timing) {
     e1 = next_event(e);
     HIP_TRY(hipEventRecord(e1, e->st));
@@ -308,7 +310,7 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
     // window flush runs in between
     hipEvent_t y0 = x1;
     if ((L + 1) - base >= e->R) {  // the next copy would overwrite S_base
-      int rc = emit_window(e, base, L, k, stats);
+      int rc = emit_window(e, base, L, k, stats, nullptr);
       if (rc) return rc;
       if (stats) stats->emit_windows++;
       base = L;
@@ -344,10 +346,15 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
       break;
     }
   }
+  // the final window also counts the TEPS numerator (over S_last, which its
+  // pass reads anyway)
+  bool counted = false;
   if (last >= base) {
-    int rc = emit_window(e, base, last, k, stats);
+    if (stats) HIP_TRY(hipMemsetAsync(e->d_count, 0, sizeof(u64), e->st));
+    int rc = emit_window(e, base, last, k, stats, stats ? e->d_count : nullptr);
     if (rc) return rc;
     if (stats) stats->emit_windows++;
+    counted = stats != nullptr;
   }
   hipEvent_t t_end = next_event(e);
   HIP_TRY(hipEventRecord(t_end, e->st));
@@ -357,11 +364,12 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
     HIP_TRY(hipEventElapsedTime(&ms, t_begin, t_end));
     stats->total_ms += ms;
     stats->levels = std::max(stats->levels, last + 1);
-    // TEPS numerator over the final snapshot (outside the timed span)
-    HIP_TRY(hipMemsetAsync(e->d_count, 0, sizeof(u64), e->st));
-    HIP_TRY(launch_count_edges(W, slot(e, last), g->d_deg, Vt, e->d_masks, e->d_count, e->st));
+    if (!counted) {  // no final window (the last flush ended at last + 1)
+      HIP_TRY(hipMemsetAsync(e->d_count, 0, sizeof(u64), e->st));
+      HIP_TRY(launch_count_edges(W, slot(e, last), g->d_deg, Vt, e->d_masks, e->d_count, e->st));
+    }
     HIP_TRY(hipMemcpyAsync(e->h_count, e->d_count, sizeof(u64), hipMemcpyDeviceToHost, e->st));
-    HIP_TRY(hipStreamSynchronize(e->st));
+    HIP_TRY(poll_stream(e->st));
     stats->traversed_edges += (int64_t)*e->h_count;
   }
   return IBFS_OK;

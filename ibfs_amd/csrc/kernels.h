@@ -123,6 +123,7 @@ struct EmitSpace {
   const int32_t *cid;
   int64_t V;
   int64_t Vt;
+  const int32_t *deg;  // traversal-space degrees (the edge count of a final window)
 };
 
 struct Queues {
@@ -201,9 +202,11 @@ hipError_t launch_plan_decide(Plan *plan, const LevelCtl *ctl_ring, int R, hipSt
 // mode 0: one vertex per thread, per-lane 4-byte stores; mode 1: LDS-tiled,
 // 16-byte stores when vec4 (all lane outputs 16-byte aligned, V % 4 == 0);
 // mode 2: as 0 with non-temporal stores.
+// edges_out (final window only, or null): += sum over v of deg(v) * |S_e[v] &
+// sel|, the TEPS numerator of the selected lanes, counted during the pass.
 hipError_t launch_emit(int W, int mode, int vec4, const u64 *ring, int R, EmitSpace sp, int b,
                        int e, const LaneOut *lanes, const u64 *level_masks, const u64 *sel,
-                       int k_lanes, hipStream_t st);
+                       int k_lanes, u64 *edges_out, hipStream_t st);
 // Closeness reduction of the window [b, e]: sum_out[k] += sum of depths and
 // reach_out[k] += vertices reached, over the bits of lane k that arrived in
 // the window (lane table and masks as for launch_emit).  Isolated sources

@@ -86,10 +86,13 @@ template <int W, int NS>
 __global__ __launch_bounds__(BS) void k_emit(const u64 *__restrict__ ring, int R, EmitSpace sp,
                                              int b, int e, const LaneOut *__restrict__ lanes,
                                              const u64 *__restrict__ level_masks,
-                                             const u64 *__restrict__ sel, int k_lanes, int nt) {
+                                             const u64 *__restrict__ sel, int k_lanes, int nt,
+                                             u64 *edges_out) {
   __shared__ int32_t *s_out[MAX_LANES];
   __shared__ int32_t s_off[MAX_LANES];  // b - admit: <= 0 in the lane's first window
   __shared__ int32_t s_src[MAX_LANES];
+  __shared__ u64 red[BS / 64];
+  u64 acc = 0;  // edges of the selected lanes' reached vertices (final window)
   for (int i = threadIdx.x; i < k_lanes; i += BS) {
     const bool on = (sel[i >> 6] >> (i & 63)) & 1ull;
     s_out[i] = on ? lanes[i].out : nullptr;
@@ -104,6 +107,12 @@ __global__ __launch_bounds__(BS) void k_emit(const u64 *__restrict__ ring, int R
     u64 c[NS][W], fin[W], bs[W];
     if (!iso) emit_count<W, NS>(ring, R, sp.Vt, b, e, ci, level_masks, c, fin, bs);
     else emit_zero<W, NS>(c, fin, bs);
+    if (edges_out && !iso) {
+      int pc = 0;
+#pragma unroll
+      for (int w = 0; w < W; ++w) pc += __popcll(fin[w] & sel[w]);
+      acc += (u64)pc * (u64)sp.deg[ci];
+    }
 #pragma unroll
     for (int w = 0; w < W; ++w) {
       for (int bit = 0; bit < 64; ++bit) {
@@ -115,6 +124,16 @@ __global__ __launch_bounds__(BS) void k_emit(const u64 *__restrict__ ring, int R
             lane_depth<W, NS>(c, fin, bs, w, bit, s_off[k], iso, s_src[k] == (int32_t)v);
         if (val != SKIP) put_depth(out, v, val, nt);
       }
+    }
+  }
+  if (edges_out) {  // uniform: one atomic per workgroup
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      u64 s = 0;
+      for (int i = 0; i < BS / 64; ++i) s += red[i];
+      if (s) atomicAdd(edges_out, s);
     }
   }
 }
@@ -374,40 +393,46 @@ __global__ __launch_bounds__(BS) void k_groupby_place(const int32_t *__restrict_
 
 template <int W, int NS>
 static void emit_ns(int mode, int vec4, const u64 *ring, int R, EmitSpace sp, int b, int e,
-                    const LaneOut *lanes, const u64 *masks, const u64 *sel, int k,
+                    const LaneOut *lanes, const u64 *masks, const u64 *sel, int k, u64 *edges_out,
                     hipStream_t st) {
   if (mode == 1) {
     dim3 g(grid_for(sp.V, BS, 4096)), blk(BS);
     hipLaunchKernelGGL((k_emit_tiled<W, NS>), g, blk, 0, st, ring, R, sp, b, e, lanes, masks, sel,
                        k, vec4);
+    if (edges_out) {
+      dim3 gc(grid_for(sp.Vt, BS * 4, 4096));
+      hipLaunchKernelGGL(k_count_edges<W>, gc, blk, 0, st, ring + (int64_t)(e % R) * sp.Vt * W,
+                         sp.deg, sp.Vt, sel, edges_out);
+    }
   } else {
     dim3 g(grid_for(sp.V, BS, IBFS_EMIT_GRID_CAP)), blk(BS);
     hipLaunchKernelGGL((k_emit<W, NS>), g, blk, 0, st, ring, R, sp, b, e, lanes, masks, sel, k,
-                       mode == 2 ? 1 : 0);
+                       mode == 2 ? 1 : 0, edges_out);
   }
 }
 
 template <int W>
 static hipError_t emit_w(int mode, int vec4, const u64 *ring, int R, EmitSpace sp, int b, int e,
                          const LaneOut *lanes, const u64 *masks, const u64 *sel, int k,
-                         hipStream_t st) {
+                         u64 *edges_out, hipStream_t st) {
   const int span = e - b + 1;
-  if (span < 16) emit_ns<W, 4>(mode, vec4, ring, R, sp, b, e, lanes, masks, sel, k, st);
-  else if (span < 64) emit_ns<W, 6>(mode, vec4, ring, R, sp, b, e, lanes, masks, sel, k, st);
-  else if (span < 256) emit_ns<W, 8>(mode, vec4, ring, R, sp, b, e, lanes, masks, sel, k, st);
+  if (span < 16) emit_ns<W, 4>(mode, vec4, ring, R, sp, b, e, lanes, masks, sel, k, edges_out, st);
+  else if (span < 64) emit_ns<W, 6>(mode, vec4, ring, R, sp, b, e, lanes, masks, sel, k, edges_out, st);
+  else if (span < 256) emit_ns<W, 8>(mode, vec4, ring, R, sp, b, e, lanes, masks, sel, k, edges_out, st);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
 hipError_t launch_emit(int W, int mode, int vec4, const u64 *ring, int R, EmitSpace sp, int b,
                        int e, const LaneOut *lanes, const u64 *level_masks, const u64 *sel,
-                       int k_lanes, hipStream_t st) {
+                       int k_lanes, u64 *edges_out, hipStream_t st) {
   if (sp.V <= 0 || e < b) return hipSuccess;
+  if (edges_out && !sp.deg) return hipErrorInvalidValue;
   switch (W) {
-    case 1: return emit_w<1>(mode, vec4, ring, R, sp, b, e, lanes, level_masks, sel, k_lanes, st);
-    case 2: return emit_w<2>(mode, vec4, ring, R, sp, b, e, lanes, level_masks, sel, k_lanes, st);
-    case 4: return emit_w<4>(mode, vec4, ring, R, sp, b, e, lanes, level_masks, sel, k_lanes, st);
-    case 8: return emit_w<8>(mode, vec4, ring, R, sp, b, e, lanes, level_masks, sel, k_lanes, st);
+    case 1: return emit_w<1>(mode, vec4, ring, R, sp, b, e, lanes, level_masks, sel, k_lanes, edges_out, st);
+    case 2: return emit_w<2>(mode, vec4, ring, R, sp, b, e, lanes, level_masks, sel, k_lanes, edges_out, st);
+    case 4: return emit_w<4>(mode, vec4, ring, R, sp, b, e, lanes, level_masks, sel, k_lanes, edges_out, st);
+    case 8: return emit_w<8>(mode, vec4, ring, R, sp, b, e, lanes, level_masks, sel, k_lanes, edges_out, st);
     default: return hipErrorInvalidValue;
   }
 }

@@ -273,9 +273,9 @@ int emit_lanes(ibfs_server_s *s, SCtx &c, const std::vector<int> &lanes, int b, 
     for (int k : lanes)
       if (c.lanes[k].admit <= L) set_bit(h + W + (size_t)(L - b) * W, k);
   HIP_CHK(hipMemcpyAsync(d, h, sizeof(u64) * emit_words(s), hipMemcpyHostToDevice, c.st));
-  const EmitSpace sp{s->g->d_cid, s->g->V, s->g->Vt};
+  const EmitSpace sp{s->g->d_cid, s->g->V, s->g->Vt, s->g->d_deg};
   HIP_CHK(launch_emit(W, s->cfg.emit_mode, 0, c.ring, s->R, sp, b, e, c.d_lanes, d + W, d, s->K,
-                      c.st));
+                      nullptr, c.st));
   return IBFS_OK;
 }
 
