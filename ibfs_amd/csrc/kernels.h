@@ -84,11 +84,11 @@ struct HubSplit {
 // counters of level Ln-1, shared by the host-driven drivers and the
 // device-driven plan.  alpha/beta: top-down -> bottom-up when m_f * alpha >
 // m_u, back when n_f * beta < V on a shrinking frontier.  A top-down choice
-// becomes hybrid (candidates pull through their hub prefixes, non-hub
-// frontier vertices push) when gamma/64 * (rho/1024 * m_u) + m_f(non-hub) <
-// m_f: the pulled hub entries, weighted by their cost relative to a pushed
-// edge, plus the edges still pushed, against the edges a top-down level
-// pushes.
+// becomes hybrid (candidates pull through their entries to the pulled hubs,
+// the other frontier vertices push) when gamma/64 * (rho/1024 * m_u) +
+// m_f(rest) < m_f: the pulled entries (rho = the pulled hubs' share of all
+// adjacency entries), weighted by their cost relative to a pushed edge, plus
+// the edges still pushed, against the edges a top-down level pushes.
 __host__ __device__ inline int choose_direction(int policy, int sw_level, int alpha, int beta,
                                                 int gamma, int rho, u64 V, int Ln, int dir_prev,
                                                 u64 nf_prev, u64 nf, u64 mf, u64 mc, u64 mf_nh) {
@@ -163,7 +163,7 @@ struct InspectArgs {
   int vpt;              // vertices per lane and chunk (0: by size)
   int full_diff;
   int nw;               // waves per workgroup: 4, 8 or 16 (0: 8)
-  HubSplit hsplit;      // non-hub frontier degrees are counted separately
+  HubSplit hsplit;      // frontier degrees outside the pulled hubs are counted separately
 };
 
 // Launchers (all asynchronous on `st`).  W = status words (u64) per vertex.
@@ -185,8 +185,9 @@ hipError_t launch_expand_bu(int W, const int64_t *beg, const int32_t *tadj, cons
                             const u64 *hs, u64 *cur, uint8_t *flags, const u64 *lam_prev,
                             Queues q, int64_t ns, int64_t nm, int64_t nl, hipStream_t st);
 // Hybrid level, fixed grid (queue lengths read from ctl_prev on the
-// device): the bottom-up candidates OR the hub-cache words of their hub
-// prefixes, the non-hub top-down frontier pushes; both with atomic ORs.
+// device): the bottom-up candidates OR the hub-cache words of their entries
+// to the pulled hubs, the rest of the top-down frontier pushes; both with
+// atomic ORs.
 hipError_t launch_expand_hy(int W, const int64_t *beg, const int32_t *adj, const int32_t *tadj,
                             const u64 *prev, const u64 *hs, u64 *cur, const LevelCtl *ctl_prev,
                             Queues q, int chunk_shift, HubSplit hsplit, int grid, hipStream_t st);

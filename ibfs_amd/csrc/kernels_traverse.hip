@@ -457,7 +457,7 @@ __device__ __forceinline__ void bu_thread(const int64_t *__restrict__ beg,
 #pragma unroll
       for (int w = 0; w < W; ++w) x[w] |= sn[r][w] & lam[w];
     j += IBFS_BU_STEP;
-    if (covers<W>(x, lam) || nonhub) break;  // hub prefix ended
+    if (covers<W>(x, lam) || nonhub) break;  // pulled prefix ended
   }
   bu_commit<W, HY>(cur, flags, v, x, x0);
 }
@@ -488,14 +488,14 @@ __device__ __forceinline__ void bu_wave(const int64_t *__restrict__ beg,
 #pragma unroll
     for (int w = 0; w < W; ++w) x[w] |= wave_or(y[w] & lam[w]);
     if (covers<W>(x, lam)) break;
-    if (HY && __ballot(nonhub)) break;  // hub prefix ended
+    if (HY && __ballot(nonhub)) break;  // pulled prefix ended
   }
   if (lane == 0) bu_commit<W, HY>(cur, flags, v, x, x0);
 }
 
 // One candidate per workgroup: 256 neighbours per step, the four wave
 // partials combined through LDS.  Must be called by the whole workgroup.
-// nh: four ints of LDS (hub prefix ended, per wave; used by HY).
+// nh: four ints of LDS (pulled prefix ended, per wave; used by HY).
 template <int W, bool HY = false>
 __device__ __forceinline__ void bu_block(const int64_t *__restrict__ beg,
                                          const int32_t *__restrict__ tadj, const Gather &gs,

@@ -38,7 +38,8 @@ extern "C" {
 #define IBFS_POLICY_TD_ONLY 2
 #define IBFS_POLICY_BU_ONLY 3
 #define IBFS_POLICY_HYBRID 4     /* every level hybrid: candidates pull through their
-                                    hub neighbours, non-hub frontier vertices push */
+                                    pulled-hub neighbours, the rest of the frontier
+                                    pushes */
 
 /* output placement for depth arrays */
 #define IBFS_OUT_HOST 0
@@ -74,9 +75,9 @@ typedef struct ibfs_config {
   int32_t inspect_vpt; /* inspector vertices per lane and tile: 1, 2, 4, 8 or
                           16, capped at 16 words per lane (0: by size)       */
   int32_t hybrid_gamma; /* ALPHA_BETA: a top-down level runs hybrid when
-                          gamma/64 * (hub entries of the candidates) + (edges of
-                          non-hub frontier vertices) < (frontier edges);
-                          0: never                                            */
+                          gamma/64 * (candidates' entries to the pulled hubs) +
+                          (edges of the other frontier vertices) < (frontier
+                          edges); 0: never                                    */
   int32_t hybrid_hubs; /* hubs a hybrid level pulls through: the top
                           hybrid_hubs by degree (0: every cached hub)         */
 } ibfs_config;
