@@ -739,12 +739,20 @@ int ibfs_server_wait(ibfs_server_t s, int64_t id, double timeout_ms, ibfs_query_
   std::unique_lock<std::mutex> lk(s->mu);
   auto it = s->queries.find(id);
   if (it == s->queries.end()) return fail(IBFS_ERR_ARG, "unknown query %lld", (long long)id);
-  auto ready = [&] { return s->error || s->queries[id].done; };
+  // a query released by another thread while this one waits ends the wait
+  // (looked up, never inserted)
+  auto ready = [&] {
+    auto q = s->queries.find(id);
+    return s->error || q == s->queries.end() || q->second.done;
+  };
   if (timeout_ms < 0) s->cv_done.wait(lk, ready);
   else if (!s->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), ready))
     return fail(IBFS_ERR_STATE, "query %lld not finished within %.1f ms", (long long)id, timeout_ms);
   if (s->error) return fail(s->error, "server stopped: %s", s->error_msg.c_str());
-  const Query &q = s->queries[id];
+  it = s->queries.find(id);
+  if (it == s->queries.end())
+    return fail(IBFS_ERR_STATE, "query %lld was released while waiting", (long long)id);
+  const Query &q = it->second;
   if (info) {
     info->id = q.id;
     info->n_sources = q.n;
