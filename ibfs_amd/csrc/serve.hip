@@ -99,8 +99,11 @@ struct SCtx {
   // graph (expand, inspect, direction decision) replayed in bursts.
   Plan *d_plan = nullptr, *h_plan = nullptr;
   u64 *d_all = nullptr, *h_all = nullptr;  // lanes of the cohort (lam of level 0)
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // one captured level graph per status-word width (1, 2, 4, 8 words), so a
+  // cohort runs at the narrowest width that holds its sources
+  hipGraph_t graph[4] = {};
+  hipGraphExec_t exec[4] = {};
+  int W = 0, K = 0;  // current width: words per vertex and lanes (<= the server's)
   int coh = 0;     // 0: host-driven levels, 1: cohort levels on the device, 2: cohort emitting
   int dlevel = 0;  // cohort level the device expands next (host view)
 };
@@ -153,15 +156,17 @@ size_t emit_words(const ibfs_server_s *s) { return (size_t)(s->R + 2) * s->W; }
 size_t mask_words(const ibfs_server_s *s) { return 2 * emit_words(s) + 3 * (size_t)s->W; }
 
 u64 *slot(const ibfs_server_s *s, const SCtx &c, int L) {
-  return c.ring + (size_t)(L % s->R) * (size_t)s->g->Vt * (size_t)s->W;
+  return c.ring + (size_t)(L % s->R) * (size_t)s->g->Vt * (size_t)c.W;
 }
 
 void release_ctx(SCtx &c) {
   if (c.st) (void)hipStreamSynchronize(c.st);
-  if (c.exec) (void)hipGraphExecDestroy(c.exec);
-  if (c.graph) (void)hipGraphDestroy(c.graph);
-  c.exec = nullptr;
-  c.graph = nullptr;
+  for (int i = 0; i < 4; ++i) {
+    if (c.exec[i]) (void)hipGraphExecDestroy(c.exec[i]);
+    if (c.graph[i]) (void)hipGraphDestroy(c.graph[i]);
+    c.exec[i] = nullptr;
+    c.graph[i] = nullptr;
+  }
   dfree(c.d_plan);
   hfree(c.h_plan);
   dfree(c.d_all);
@@ -243,18 +248,24 @@ int alloc_ctx(ibfs_server_s *s, SCtx &c) {
                 "server context allocation failed: %s", hipGetErrorString(err));
   std::memset(c.h_lanes, 0, sizeof(LaneOut) * (size_t)s->K);
   c.lanes.assign(s->K, Lane{});
-  // one cohort level: device-driven expansion, inspection, direction decision
-  HIP_CHK(hipStreamBeginCapture(c.st, hipStreamCaptureModeThreadLocal));
-  hipError_t e1 = launch_expand_dd(s->W, s->g->d_beg, s->g->d_adj, s->g->d_tadj, c.ring, c.flags,
-                                   s->R, s->g->Vt, c.ctl, c.d_plan, c.hs, c.q, s->chunk_shift,
-                                   HubSplit{s->hc.deg, s->hc.tie, s->hc.cut}, s->grid, c.st);
-  hipError_t e2 = launch_inspect(s->W, plan_args(s, c), c.st);
-  if (e2 == hipSuccess) e2 = launch_plan_decide(c.d_plan, c.ctl, s->R, c.st);
-  hipError_t e3 = hipStreamEndCapture(c.st, &c.graph);
-  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
-    return fail(IBFS_ERR_HIP, "cohort level capture failed: %s / %s / %s", hipGetErrorString(e1),
-                hipGetErrorString(e2), hipGetErrorString(e3));
-  HIP_CHK(hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0));
+  // one cohort level per width: device-driven expansion, inspection,
+  // direction decision (every kernel derives its slot stride from its width)
+  c.W = s->W;
+  c.K = s->K;
+  for (int i = 0; i < 4 && (1 << i) <= s->W; ++i) {
+    const int Wc = 1 << i;
+    HIP_CHK(hipStreamBeginCapture(c.st, hipStreamCaptureModeThreadLocal));
+    hipError_t e1 = launch_expand_dd(Wc, s->g->d_beg, s->g->d_adj, s->g->d_tadj, c.ring, c.flags,
+                                     s->R, s->g->Vt, c.ctl, c.d_plan, c.hs, c.q, s->chunk_shift,
+                                     HubSplit{s->hc.deg, s->hc.tie, s->hc.cut}, s->grid, c.st);
+    hipError_t e2 = launch_inspect(Wc, plan_args(s, c), c.st);
+    if (e2 == hipSuccess) e2 = launch_plan_decide(c.d_plan, c.ctl, s->R, c.st);
+    hipError_t e3 = hipStreamEndCapture(c.st, &c.graph[i]);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+      return fail(IBFS_ERR_HIP, "cohort level capture failed: %s / %s / %s", hipGetErrorString(e1),
+                  hipGetErrorString(e2), hipGetErrorString(e3));
+    HIP_CHK(hipGraphInstantiate(&c.exec[i], c.graph[i], nullptr, nullptr, 0));
+  }
   return IBFS_OK;
 }
 
@@ -264,7 +275,7 @@ bool has_bit(const u64 *m, int k) { return (m[k >> 6] >> (k & 63)) & 1ull; }
 // Emission of lanes `sel` over [b, e] using staging slot `which` (0/1).
 int emit_lanes(ibfs_server_s *s, SCtx &c, const std::vector<int> &lanes, int b, int e, int which) {
   if (lanes.empty() || e < b) return IBFS_OK;
-  const int W = s->W;
+  const int W = c.W;
   u64 *h = c.h_mask + (size_t)which * emit_words(s);
   u64 *d = c.d_mask + (size_t)which * emit_words(s);
   std::memset(h, 0, sizeof(u64) * emit_words(s));
@@ -274,7 +285,7 @@ int emit_lanes(ibfs_server_s *s, SCtx &c, const std::vector<int> &lanes, int b, 
       if (c.lanes[k].admit <= L) set_bit(h + W + (size_t)(L - b) * W, k);
   HIP_CHK(hipMemcpyAsync(d, h, sizeof(u64) * emit_words(s), hipMemcpyHostToDevice, c.st));
   const EmitSpace sp{s->g->d_cid, s->g->V, s->g->Vt, s->g->d_deg};
-  HIP_CHK(launch_emit(W, s->cfg.emit_mode, 0, c.ring, s->R, sp, b, e, c.d_lanes, d + W, d, s->K,
+  HIP_CHK(launch_emit(W, s->cfg.emit_mode, 0, c.ring, s->R, sp, b, e, c.d_lanes, d + W, d, c.K,
                       nullptr, c.st));
   return IBFS_OK;
 }
@@ -283,7 +294,7 @@ int emit_lanes(ibfs_server_s *s, SCtx &c, const std::vector<int> &lanes, int b, 
 // (their bits go into slot L); returns the admitted lane mask in A.
 int admit(ibfs_server_s *s, SCtx &c, int L, u64 *A, int *n_adm) {
   int n = 0;
-  for (int k = 0; k < s->K && !s->pending.empty(); ++k) {
+  for (int k = 0; k < c.K && !s->pending.empty(); ++k) {
     if (c.lanes[k].state != CLEAN) continue;
     const PendingSrc p = s->pending.front();
     s->pending.pop_front();
@@ -307,14 +318,14 @@ int admit(ibfs_server_s *s, SCtx &c, int L, u64 *A, int *n_adm) {
                          hipMemcpyHostToDevice, c.st));
   HIP_CHK(hipMemcpyAsync(c.d_lanes, c.h_lanes, sizeof(LaneOut) * (size_t)s->K,
                          hipMemcpyHostToDevice, c.st));
-  HIP_CHK(launch_admit(s->W, c.d_adm, c.d_adm + s->K, n, slot(s, c, L), c.flags, c.st));
+  HIP_CHK(launch_admit(c.W, c.d_adm, c.d_adm + s->K, n, slot(s, c, L), c.flags, c.st));
   return IBFS_OK;
 }
 
 // Inspection of level L (masks in staging slot I) + control-block copy + event.
 int inspect_level(ibfs_server_s *s, SCtx &c, int L, const u64 *filter, const u64 *keep,
                   const u64 *nand, int full_diff) {
-  const int W = s->W;
+  const int W = c.W;
   u64 *h = c.h_mask + 2 * emit_words(s);
   u64 *d = c.d_mask + 2 * emit_words(s);
   for (int w = 0; w < W; ++w) {
@@ -355,7 +366,9 @@ int inspect_level(ibfs_server_s *s, SCtx &c, int L, const u64 *filter, const u64
 }
 
 int start_ctx(ibfs_server_s *s, SCtx &c) {
-  const int W = s->W;
+  c.W = s->W;  // admission mid-flight: the full width
+  c.K = s->K;
+  const int W = c.W;
   for (auto &l : c.lanes) l = Lane{};
   c.base = 0;
   c.cohort = 0;
@@ -395,7 +408,7 @@ void mark_done(ibfs_server_s *s, SCtx &c) {
 
 // Level L of context c has completed (its control block is in h_ctl).
 int process_level(ibfs_server_s *s, SCtx &c) {
-  const int W = s->W;
+  const int W = c.W;
   const int L = c.L;
   const LevelCtl lc = *c.h_ctl;
   {
@@ -410,7 +423,7 @@ int process_level(ibfs_server_s *s, SCtx &c) {
   //    Emission starts at the earliest admission among them (the window base
   //    if one is older); emitted lanes are freed.
   bool active_left = false;
-  for (int k = 0; k < s->K; ++k) {
+  for (int k = 0; k < c.K; ++k) {
     Lane &l = c.lanes[k];
     if (l.state == ACTIVE && l.admit < L && !has_bit(lc.lam, k)) l.state = FINISHED;
     active_left |= (l.state == ACTIVE);
@@ -419,7 +432,7 @@ int process_level(ibfs_server_s *s, SCtx &c) {
   std::vector<u64> finmask(W, 0);
   int fin_from = L;
   if (!(s->admit_window == 0 && active_left)) {
-    for (int k = 0; k < s->K; ++k) {
+    for (int k = 0; k < c.K; ++k) {
       Lane &l = c.lanes[k];
       if (l.state == FINISHED) {
         fin.push_back(k);
@@ -474,14 +487,14 @@ int process_level(ibfs_server_s *s, SCtx &c) {
   //    of them over [base, Ln].
   {
     int oldest = Ln;
-    for (int k = 0; k < s->K; ++k)
+    for (int k = 0; k < c.K; ++k)
       if (c.lanes[k].state == ACTIVE || c.lanes[k].state == FINISHED)
         oldest = std::min(oldest, c.lanes[k].admit);
     c.base = std::max(c.base, oldest);
   }
   if ((Ln + 1) - c.base >= s->R) {
     std::vector<int> act;
-    for (int k = 0; k < s->K; ++k)
+    for (int k = 0; k < c.K; ++k)
       if (c.lanes[k].state == ACTIVE || c.lanes[k].state == FINISHED) act.push_back(k);
     rc = emit_lanes(s, c, act, c.base, Ln, 1);
     if (rc) return rc;
@@ -516,14 +529,15 @@ int cohort_burst(ibfs_server_s *s, SCtx &c) {
   int B = std::min(kCohortBurst, s->R - 1 - (c.dlevel - c.base));
   if (B < 1) {
     std::vector<int> act;
-    for (int k = 0; k < s->K; ++k)
+    for (int k = 0; k < c.K; ++k)
       if (c.lanes[k].state == ACTIVE) act.push_back(k);
     int rc = emit_lanes(s, c, act, c.base, c.dlevel - 1, 1);
     if (rc) return rc;
     c.base = c.dlevel - 1;
     B = std::min(kCohortBurst, s->R - 1 - (c.dlevel - c.base));
   }
-  for (int i = 0; i < B; ++i) HIP_CHK(hipGraphLaunch(c.exec, c.st));
+  hipGraphExec_t x = c.exec[__builtin_ctz((unsigned)c.W)];
+  for (int i = 0; i < B; ++i) HIP_CHK(hipGraphLaunch(x, c.st));
   HIP_CHK(hipMemcpyAsync(c.h_plan, c.d_plan, offsetof(Plan, dirlog), hipMemcpyDeviceToHost, c.st));
   HIP_CHK(hipEventRecord(c.ev, c.st));
   ++s->levels_issued;
@@ -531,9 +545,14 @@ int cohort_burst(ibfs_server_s *s, SCtx &c) {
 }
 
 // Starts a cohort on an idle context (caller holds the lock): level 0 holds
-// up to K waiting sources; its levels then run on the device.
+// up to c.K waiting sources; its levels then run on the device.
 int start_cohort(ibfs_server_s *s, SCtx &c) {
-  const int W = s->W;
+  // the narrowest width (from 64 lanes) that holds the waiting sources:
+  // narrower words make every pass of a small cohort cheaper
+  c.K = 64;
+  while (c.K < s->K && (size_t)c.K < s->pending.size()) c.K *= 2;
+  c.W = c.K / 64;
+  const int W = c.W;
   for (auto &l : c.lanes) l = Lane{};
   HIP_CHK(launch_fill_u64(slot(s, c, 0), s->g->Vt * W, 0ull, c.st));
   HIP_CHK(launch_fill_u64(reinterpret_cast<u64 *>(c.flags), (s->g->Vt + 7) / 8, 0ull, c.st));
@@ -561,7 +580,7 @@ int cohort_step(ibfs_server_s *s, SCtx &c) {
   if (c.coh == 1) {
     if (c.h_plan->done) {
       std::vector<int> act;
-      for (int k = 0; k < s->K; ++k)
+      for (int k = 0; k < c.K; ++k)
         if (c.lanes[k].state == ACTIVE) act.push_back(k);
       int rc = emit_lanes(s, c, act, c.base, c.h_plan->last, 0);
       if (rc) return rc;
