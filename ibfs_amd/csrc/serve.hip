@@ -15,7 +15,10 @@
 // have finished (a new cohort) and, within admit_window levels of that
 // start, into its remaining clean lanes.  A level's direction and cost are
 // shared by all lanes of a context, so lanes far apart in BFS phase would
-// make every level as expensive as the densest phase among them.
+// make every level as expensive as the densest phase among them.  With
+// admit_window 0 (the default) a cohort runs on the device: captured level
+// graphs replayed in bursts, one host poll per burst, its depths emitted once
+// it ends, at the narrowest status-word width that holds its sources.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
