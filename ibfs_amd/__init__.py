@@ -440,6 +440,11 @@ class Server:
         check(lib().ibfs_server_contexts(h, C.byref(c), C.byref(l)))
         self.contexts, self.lanes = c.value, l.value
 
+    def set_admission(self, window_levels: int) -> None:
+        """Sources join a running context only within window_levels levels of
+        its cohort's start (0: only at cohort starts, cohorts run device-driven)."""
+        check(lib().ibfs_server_set_admission(self._h, window_levels))
+
     def submit(self, sources, out: DeviceBuffer) -> int:
         s = np.ascontiguousarray(sources, dtype=np.int32)
         if out.nbytes < 4 * len(s) * self.graph.vert_count:

@@ -528,3 +528,27 @@ def test_hub_entries_match_the_adjacency(s16):
         hub = np.zeros(len(deg), bool)
         hub[order[:r]] = True
         assert g.hub_entries(top) == int(hub[adj].sum()), top
+
+
+def test_server_cohort_widths_and_window_changes(s16):
+    """One server whose admission window changes between batches of queries:
+    device-driven cohorts at the narrowest width that holds their sources
+    (64/128/256 lanes: query sizes 1, 100, 200) alternate with host-driven
+    full-width levels on the same contexts; every depth array equals the
+    oracle's."""
+    say("[gpu] server cohort widths / window changes")
+    V = s16["h"].vert_count
+    pool = s16["h"].gen_sources(600, seed=21)
+    srv = ibfs_amd.Server(s16["g"], lane_bits=256, ring_levels=8)
+    off = 0
+    for window in (0, 3, 0, 1, 0):
+        srv.set_admission(window)
+        sizes = (1, 100, 200) if window == 0 else (5, 40)
+        queries = []
+        for n in sizes:
+            queries.append(pool[off:off + n])
+            off = (off + n) % (len(pool) - 200)
+        outs, _ = _serve(srv, queries, V, gap_s=0.002)
+        for q, o in zip(queries, outs):
+            np.testing.assert_array_equal(o, oracle.bfs_many(s16["beg"], s16["adj"], q, threads=8))
+    srv.close()
