@@ -23,7 +23,10 @@ using namespace ibfs;
 struct ibfs_engine_s {
   ibfs_graph_s *g = nullptr;
   ibfs_config cfg{};
-  int K = 128, W = 2, R = 32, chunk_shift = 12;
+  // K: lanes per group (lane_bits); W: status words per vertex of the current
+  // traversal, the narrowest power of two that holds its sources (at most
+  // Wmax = lane_bits / 64, which sizes the buffers)
+  int K = 128, W = 2, Wmax = 2, R = 32, chunk_shift = 12;
   int hy_grid = 8192;  // workgroups of a hybrid expansion
   HubCut hc;           // hubs a hybrid level pulls through
   hipStream_t st = nullptr;
@@ -199,6 +202,9 @@ int emit_window(ibfs_engine_s *e, int b, int en, int k, ibfs_stats *stats, u64 *
 int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_stats *stats) {
   ibfs_graph_s *g = e->g;
   const int64_t V = g->V, Vt = g->Vt;
+  // a partial group streams only the status words its lanes need
+  e->W = 1;
+  while (e->W * 64 < k && e->W < e->Wmax) e->W *= 2;
   const int W = e->W;
   e->ev_used = 0;
   e->log.clear();
@@ -720,7 +726,7 @@ int ibfs_engine_create(ibfs_graph_t g, const ibfs_config *cfg_in, ibfs_engine_t 
   e->g = g;
   e->cfg = cfg;
   e->K = cfg.lane_bits;
-  e->W = cfg.lane_bits / 64;
+  e->W = e->Wmax = cfg.lane_bits / 64;
   e->chunk_shift = __builtin_ctz((unsigned)cfg.td_chunk);
   {
     int cus = 0;
