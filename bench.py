@@ -259,7 +259,8 @@ def served_leg(graphs, rates, n_queries, seed, K, admit_window=None):
         log(f"served lambda={lam}/s: p50 {results[-1]['p50_ms']} ms p99 {results[-1]['p99_ms']} ms "
             f"{results[-1]['gteps']} GTEPS over {results[-1]['span_ms']} ms")
     info = {"graphs": [gr["name"] for gr in graphs], "lane_bits": K,
-            "cohort_lanes": f"narrowest of 64..{K} holding the waiting sources",
+            "cohort_lanes": (f"narrowest of 64..{K} holding the waiting sources"
+                             if not admit_window else K),
             "admit_window": 0 if admit_window is None else admit_window,
             "contexts": [s.contexts for s in servers], "rates": results}
     for s in servers:
