@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cstddef>
 #include <cstring>
+#include <numeric>
 #include <thread>
 #include <vector>
 
@@ -48,7 +49,8 @@ struct Ctx {
   int k = 0;
   int base = 0;
   int level = 0;  // level the device will expand next (host view)
-  int32_t *out_host = nullptr;
+  int32_t *out_host = nullptr;    // host-placed output (rows of the caller's order)
+  const int64_t *rows = nullptr;  // caller's index of each lane of the group
 };
 
 template <class T>
@@ -70,6 +72,15 @@ struct ibfs_batch_s {
   int K = 128, W = 2, R = 16, C = 2, burst = 8, chunk_shift = 12, grid = 2048;
   HubCut hc;  // hubs a hybrid level pulls through
   std::vector<Ctx> ctx;
+  // grouping order of the current run: order[p] = caller's index of the
+  // source at position p (groups are consecutive positions); psrc = sources
+  // in that order; two-hop keys staged on the device
+  std::vector<int64_t> order;
+  std::vector<int32_t> psrc;
+  std::vector<u64> h_keys;
+  int32_t *d_keysrc = nullptr;
+  u64 *d_keys = nullptr;
+  int64_t key_cap = 0;
   u64 *d_edges = nullptr, *h_edges = nullptr;
   // closeness runs: per-source depth sums and reach counts instead of depths
   bool closeness = false;
@@ -117,6 +128,8 @@ void release_batch(ibfs_batch_s *b) {
   for (auto &c : b->ctx) release_ctx(c);
   b->ctx.clear();
   dfree(b->d_edges);
+  dfree(b->d_keysrc);
+  dfree(b->d_keys);
   hfree(b->h_edges);
   dfree(b->d_sum);
   dfree(b->d_reach);
@@ -206,17 +219,22 @@ int alloc_ctx(ibfs_batch_s *b, Ctx &c) {
 
 // Starts group [g0, g0+k) of src on context c: lane table, masks, S_0 and the
 // level-0 inspection (whose last workgroup plans level 1).
-int start_group(ibfs_batch_s *b, Ctx &c, const int32_t *src, int64_t g0, int k, int32_t *dout) {
+// src: sources in grouping order; rows[i] = caller's index of lane i (its
+// depth row in dev_out); stage: host-placed output staging (lane i at row i).
+int start_group(ibfs_batch_s *b, Ctx &c, const int32_t *src, int64_t g0, int k, int32_t *dev_out,
+                int32_t *stage, const int64_t *rows) {
   const int W = b->W;
   const int64_t V = b->g->V;
   std::memset(c.h_masks, 0, sizeof(u64) * masks_words(b));
   for (int i = 0; i < k; ++i) {
     c.h_masks[i >> 6] |= 1ull << (i & 63);
     c.h_src[i] = b->g->h_cid[src[g0 + i]];
-    c.h_lanes[i].out = dout ? dout + (size_t)i * (size_t)V : nullptr;
+    c.h_lanes[i].out = dev_out ? dev_out + (size_t)rows[i] * (size_t)V
+                               : (stage ? stage + (size_t)i * (size_t)V : nullptr);
     c.h_lanes[i].admit = 0;
     c.h_lanes[i].src = src[g0 + i];
   }
+  c.rows = rows;
   for (int w = 0; w < W; ++w) c.h_masks[W + w] = c.h_masks[w];
   for (size_t i = 0; i < (size_t)(b->R + 1) * W; ++i) c.h_masks[2 * W + i] = ~0ull;
   plan_init(c.h_plan, b->cfg, b->hc.rho_q10, (u64)V);
@@ -267,9 +285,18 @@ int finish_group(ibfs_batch_s *b, Ctx &c, int last, ibfs_stats *stats) {
       HIP_TRY(launch_count_edges(b->W, slot(b, c, last), b->g->d_deg, b->g->Vt, c.d_masks,
                                  b->d_edges, c.st));
   }
-  if (c.out_host)
-    HIP_TRY(hipMemcpyAsync(c.out_host, c.d_stage, sizeof(int32_t) * (size_t)c.k * (size_t)b->g->V,
-                           hipMemcpyDeviceToHost, c.st));
+  if (c.out_host) {  // rows of the caller's order: one copy when consecutive
+    const size_t V = (size_t)b->g->V;
+    bool consecutive = true;
+    for (int i = 1; i < c.k && consecutive; ++i) consecutive = c.rows[i] == c.rows[0] + i;
+    if (consecutive)
+      HIP_TRY(hipMemcpyAsync(c.out_host + (size_t)c.rows[0] * V, c.d_stage,
+                             sizeof(int32_t) * (size_t)c.k * V, hipMemcpyDeviceToHost, c.st));
+    else
+      for (int i = 0; i < c.k; ++i)
+        HIP_TRY(hipMemcpyAsync(c.out_host + (size_t)c.rows[i] * V, c.d_stage + (size_t)i * V,
+                               sizeof(int32_t) * V, hipMemcpyDeviceToHost, c.st));
+  }
   HIP_TRY(hipEventRecord(c.ev, c.st));
   return IBFS_OK;
 }
@@ -367,6 +394,29 @@ static int batch_run_impl(ibfs_batch_s *b, const int32_t *src, int64_t n, int32_
     for (auto &c : b->ctx)
       if (!c.d_stage) HIP_TRY(hipMalloc((void **)&c.d_stage, sizeof(int32_t) * (size_t)b->K * (size_t)V));
   }
+  // grouping order: the caller's, or by descending two-hop size so that the
+  // lanes of a group progress alike (results are per lane and unchanged)
+  b->order.resize((size_t)n);
+  std::iota(b->order.begin(), b->order.end(), (int64_t)0);
+  if (b->cfg.pack_sources && n > b->K) {
+    if (b->key_cap < n) {
+      dfree(b->d_keysrc);
+      dfree(b->d_keys);
+      b->key_cap = 0;
+      HIP_TRY(hipMalloc((void **)&b->d_keysrc, sizeof(int32_t) * (size_t)n));
+      HIP_TRY(hipMalloc((void **)&b->d_keys, sizeof(u64) * (size_t)n));
+      b->key_cap = n;
+    }
+    HIP_TRY(hipMemcpy(b->d_keysrc, src, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice));
+    HIP_TRY(launch_two_hop(b->g->d_beg, b->g->d_adj, b->g->d_deg, b->g->d_cid, b->d_keysrc, n,
+                           b->d_keys, nullptr));
+    b->h_keys.resize((size_t)n);
+    HIP_TRY(hipMemcpy(b->h_keys.data(), b->d_keys, sizeof(u64) * (size_t)n, hipMemcpyDeviceToHost));
+    std::stable_sort(b->order.begin(), b->order.end(),
+                     [&](int64_t x, int64_t y) { return b->h_keys[x] > b->h_keys[y]; });
+  }
+  b->psrc.resize((size_t)n);
+  for (int64_t p = 0; p < n; ++p) b->psrc[p] = src[b->order[p]];
   HIP_TRY(hipMemset(b->d_edges, 0, sizeof(u64)));
   HIP_TRY(hipDeviceSynchronize());
   const auto t0 = std::chrono::steady_clock::now();
@@ -381,14 +431,17 @@ static int batch_run_impl(ibfs_batch_s *b, const int32_t *src, int64_t n, int32_
         if (next >= groups) continue;
         const int64_t g0 = next * b->K;
         const int k = (int)std::min<int64_t>(b->K, n - g0);
-        int32_t *dout = nullptr;
+        int32_t *dev_out = nullptr, *stage = nullptr;
         c.out_host = nullptr;
         if (!b->closeness) {
-          dout = out_place == IBFS_OUT_DEVICE ? depth_out + (size_t)g0 * (size_t)V : c.d_stage;
-          c.out_host = out_place == IBFS_OUT_HOST ? depth_out + (size_t)g0 * (size_t)V : nullptr;
+          if (out_place == IBFS_OUT_DEVICE) dev_out = depth_out;
+          else {
+            stage = c.d_stage;
+            c.out_host = depth_out;
+          }
         }
         ++next;
-        rc = start_group(b, c, src, g0, k, dout);
+        rc = start_group(b, c, b->psrc.data(), g0, k, dev_out, stage, b->order.data() + g0);
         if (!rc) rc = launch_burst(b, c, stats);
         c.state = RUNNING;
         progressed = true;
@@ -468,8 +521,14 @@ int ibfs_batch_closeness(ibfs_batch_t b, const int32_t *src, int64_t n, int64_t 
   int rc = batch_run_impl(b, src, n, nullptr, IBFS_OUT_DEVICE, stats);
   b->closeness = false;
   if (rc) return rc;
-  HIP_TRY(hipMemcpy(depth_sum, b->d_sum, sizeof(u64) * (size_t)n, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(reach, b->d_reach, sizeof(u64) * (size_t)n, hipMemcpyDeviceToHost));
+  // the device sums are in grouping order; back to the caller's
+  std::vector<u64> hs((size_t)n), hr((size_t)n);
+  HIP_TRY(hipMemcpy(hs.data(), b->d_sum, sizeof(u64) * (size_t)n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(hr.data(), b->d_reach, sizeof(u64) * (size_t)n, hipMemcpyDeviceToHost));
+  for (int64_t p = 0; p < n; ++p) {
+    depth_sum[b->order[p]] = (int64_t)hs[p];
+    reach[b->order[p]] = (int64_t)hr[p];
+  }
   // an isolated source reaches itself (depth 0) and has no status word
   for (int64_t i = 0; i < n; ++i)
     if (b->g->h_cid[src[i]] < 0) reach[i] += 1;

@@ -222,6 +222,10 @@ hipError_t launch_count_edges(int W, const u64 *S, const int32_t *deg, int64_t V
 hipError_t launch_bu_model(int W, const int64_t *beg, const int32_t *tadj, const u64 *prev,
                            const u64 *hs, const u64 *lam_prev, int64_t V, u64 *out,
                            hipStream_t st);
+// key[i] = sum of the degrees of src[i]'s neighbours (src in original ids).
+hipError_t launch_two_hop(const int64_t *beg, const int32_t *adj, const int32_t *deg,
+                          const int32_t *cid, const int32_t *src, int64_t n, u64 *key,
+                          hipStream_t st);
 // GroupBy (groupby.cu:36-149): key[i] = first neighbour of degree > 128 of a
 // source of degree <= 128, else -1; then the stable placement.  src holds
 // original ids (cid maps them to the traversal CSR); keys are traversal ids,

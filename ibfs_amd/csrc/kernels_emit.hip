@@ -326,6 +326,25 @@ __global__ __launch_bounds__(BS) void k_count_edges(const u64 *__restrict__ S,
 // GroupBy.  Key of source i: its first neighbour (adjacency order) of degree
 // > 128 if deg(src[i]) <= 128, else -1 (groupby.cu:49-66); one wave per
 // source scans 64 neighbours per step and takes the lowest set ballot lane.
+// Two-hop size of source i: the sum of its neighbours' degrees (0 for an
+// isolated source), one wave per source.
+__global__ __launch_bounds__(BS) void k_two_hop(const int64_t *__restrict__ beg,
+                                                const int32_t *__restrict__ adj,
+                                                const int32_t *__restrict__ deg,
+                                                const int32_t *__restrict__ cid,
+                                                const int32_t *__restrict__ src, int64_t n,
+                                                u64 *__restrict__ key) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= n) return;
+  const int32_t s = cid[src[i]];
+  u64 sum = 0;
+  if (s >= 0)
+    for (int64_t j = beg[s] + lane; j < beg[s + 1]; j += 64) sum += (u64)deg[adj[j]];
+  sum = wave_sum(sum);
+  if (lane == 0) key[i] = sum;
+}
+
 __global__ __launch_bounds__(BS) void k_groupby_key(const int64_t *__restrict__ beg,
                                                     const int32_t *__restrict__ adj,
                                                     const int32_t *__restrict__ deg,
@@ -479,6 +498,15 @@ hipError_t launch_close(int W, const u64 *ring, int R, EmitSpace sp, int b, int 
     case 8: return close_w<8>(ring, R, sp, b, e, lanes, level_masks, sel, k_lanes, sum_out, reach_out, st);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_two_hop(const int64_t *beg, const int32_t *adj, const int32_t *deg,
+                          const int32_t *cid, const int32_t *src, int64_t n, u64 *key,
+                          hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_two_hop, dim3(grid_for(n, 4, 1 << 30)), dim3(BS), 0, st, beg, adj, deg, cid,
+                     src, n, key);
+  return hipGetLastError();
 }
 
 hipError_t launch_groupby(const int64_t *beg, const int32_t *adj, const int32_t *deg,

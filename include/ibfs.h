@@ -80,6 +80,10 @@ typedef struct ibfs_config {
                           edges); 0: never                                    */
   int32_t hybrid_hubs; /* hubs a hybrid level pulls through: the top
                           hybrid_hubs by degree (0: every cached hub)         */
+  int32_t pack_sources; /* batched runs: 1 = groups of sources with similar
+                          two-hop sizes (their lanes progress alike), 0 =
+                          groups in the caller's order; outputs stay in the
+                          caller's order either way                          */
 } ibfs_config;
 
 /* Per-level record of the last traversal (the build's monitoring log, in place

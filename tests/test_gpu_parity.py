@@ -552,3 +552,25 @@ def test_server_cohort_widths_and_window_changes(s16):
         for q, o in zip(queries, outs):
             np.testing.assert_array_equal(o, oracle.bfs_many(s16["beg"], s16["adj"], q, threads=8))
     srv.close()
+
+
+def test_batch_packing_keeps_caller_order(s16):
+    """Batched runs group sources by two-hop size (default) or in the
+    caller's order; depth arrays (host and device output) and closeness come
+    back in the caller's order either way, equal to the oracle's."""
+    say("[gpu] batch packing")
+    V = s16["h"].vert_count
+    pool = s16["h"].gen_sources(700, seed=31)
+    src = np.concatenate([pool, pool[:50]])  # duplicates across groups
+    want = oracle.bfs_many(s16["beg"], s16["adj"], src, threads=8)
+    want_sum, want_reach = _close_ref(want)
+    for pack in (1, 0):
+        b = ibfs_amd.Batch(s16["g"], contexts=3, burst=4, lane_bits=128, pack_sources=pack)
+        np.testing.assert_array_equal(b.run(src), want)
+        buf = ibfs_amd.DeviceBuffer(4 * len(src) * V)
+        b.run(src, out=buf)
+        np.testing.assert_array_equal(buf.to_host().reshape(len(src), V), want)
+        dsum, reach = b.closeness(src)
+        np.testing.assert_array_equal(dsum, want_sum)
+        np.testing.assert_array_equal(reach, want_reach)
+        b.close()
