@@ -28,6 +28,7 @@
 #include <cstring>
 #include <deque>
 #include <mutex>
+#include <numeric>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -146,6 +147,7 @@ struct ibfs_server_s {
   std::atomic<int> admit_window{0};
   int grid = 2048;  // workgroups of the device-driven expander
   HubCut hc;        // hubs a hybrid level pulls through
+  std::vector<u64> h_key;  // two-hop size of every vertex (cohort composition)
 };
 
 namespace {
@@ -547,6 +549,36 @@ int cohort_burst(ibfs_server_s *s, SCtx &c) {
   return IBFS_OK;
 }
 
+// Cohort composition when more sources wait than a cohort holds (caller
+// holds the lock): of the oldest 8K waiting sources, the K nearest in two-hop
+// size to the oldest one move to the front, the oldest included, so that the
+// cohort's lanes progress alike; everything else keeps its arrival order.
+void pick_cohort(ibfs_server_s *s, int K) {
+  const size_t n = s->pending.size();
+  if (n <= (size_t)K || s->h_key.empty()) return;
+  const size_t win = std::min(n, (size_t)K * 8);
+  std::vector<u64> key(win);
+  for (size_t i = 0; i < win; ++i) {
+    const PendingSrc &p = s->pending[i];
+    key[i] = s->h_key[(size_t)s->queries[p.qid].src[p.idx]];
+  }
+  std::vector<size_t> idx(win);
+  std::iota(idx.begin(), idx.end(), (size_t)0);
+  std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return key[a] > key[b]; });
+  const size_t pos = (size_t)(std::find(idx.begin(), idx.end(), (size_t)0) - idx.begin());
+  size_t start = pos >= (size_t)K / 2 ? pos - (size_t)K / 2 : 0;
+  if (start + (size_t)K > win) start = win - (size_t)K;
+  std::vector<char> take(win, 0);
+  for (size_t i = start; i < start + (size_t)K; ++i) take[idx[i]] = 1;
+  std::deque<PendingSrc> next;
+  for (size_t i = 0; i < win; ++i)
+    if (take[i]) next.push_back(s->pending[i]);
+  for (size_t i = 0; i < win; ++i)
+    if (!take[i]) next.push_back(s->pending[i]);
+  for (size_t i = win; i < n; ++i) next.push_back(s->pending[i]);
+  s->pending.swap(next);
+}
+
 // Starts a cohort on an idle context (caller holds the lock): level 0 holds
 // up to c.K waiting sources; its levels then run on the device.
 int start_cohort(ibfs_server_s *s, SCtx &c) {
@@ -555,6 +587,7 @@ int start_cohort(ibfs_server_s *s, SCtx &c) {
   c.K = 64;
   while (c.K < s->K && (size_t)c.K < s->pending.size()) c.K *= 2;
   c.W = c.K / 64;
+  pick_cohort(s, c.K);
   const int W = c.W;
   for (auto &l : c.lanes) l = Lane{};
   HIP_CHK(launch_fill_u64(slot(s, c, 0), s->g->Vt * W, 0ull, c.st));
@@ -704,6 +737,27 @@ int ibfs_server_create(ibfs_graph_t g, const ibfs_config *cfg_in, int64_t status
     cus = 256;
   s->grid = cfg.dd_grid > 0 ? cfg.dd_grid : cus * 32;
   s->hc = hub_cut(s->g, cfg.hybrid_hubs);
+  {  // two-hop size of every vertex, for cohort composition
+    const int64_t V = g->V;
+    std::vector<int32_t> all((size_t)V);
+    std::iota(all.begin(), all.end(), 0);
+    int32_t *d_all = nullptr;
+    u64 *d_key = nullptr;
+    hipError_t err = hipMalloc((void **)&d_all, sizeof(int32_t) * (size_t)std::max<int64_t>(V, 1));
+    if (err == hipSuccess) err = hipMalloc((void **)&d_key, sizeof(u64) * (size_t)std::max<int64_t>(V, 1));
+    if (err == hipSuccess) err = hipMemcpy(d_all, all.data(), sizeof(int32_t) * (size_t)V, hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = launch_two_hop(g->d_beg, g->d_adj, g->d_deg, g->d_cid, d_all, V, d_key, nullptr);
+    if (err == hipSuccess) {
+      s->h_key.resize((size_t)V);
+      err = hipMemcpy(s->h_key.data(), d_key, sizeof(u64) * (size_t)V, hipMemcpyDeviceToHost);
+    }
+    if (d_all) (void)hipFree(d_all);
+    if (d_key) (void)hipFree(d_key);
+    if (err != hipSuccess) {
+      delete s;
+      return fail(IBFS_ERR_HIP, "server key pass failed: %s", hipGetErrorString(err));
+    }
+  }
   s->ctx.resize(C);
   for (auto &c : s->ctx) {
     int rc = alloc_ctx(s, c);
