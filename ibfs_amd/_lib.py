@@ -49,7 +49,7 @@ class Config(C.Structure):
         ("ring_levels", C.c_int32), ("kernel_timing", C.c_int32), ("device", C.c_int32),
         ("work_model", C.c_int32), ("emit_mode", C.c_int32), ("dd_grid", C.c_int32),
         ("inspect_vpt", C.c_int32), ("hybrid_gamma", C.c_int32), ("hybrid_hubs", C.c_int32),
-        ("pack_sources", C.c_int32),
+        ("pack_sources", C.c_int32), ("split_cov", C.c_int32),
     ]
 
 
@@ -79,7 +79,7 @@ class Stats(C.Structure):
         ("expand_bu_ms", C.c_double), ("inspect_ms", C.c_double), ("emit_ms", C.c_double),
         ("n_td_launches", C.c_int32), ("n_bu_launches", C.c_int32),
         ("n_inspect_launches", C.c_int32), ("n_emit_launches", C.c_int32),
-        ("expand_hy_ms", C.c_double), ("n_hy_launches", C.c_int32), ("reserved", C.c_int32),
+        ("expand_hy_ms", C.c_double), ("n_hy_launches", C.c_int32), ("n_split_levels", C.c_int32),
     ]
 
 

@@ -41,6 +41,8 @@ inline int check_direction_config(const ibfs_config &c) {
   if (c.hybrid_gamma < 0 || c.hybrid_gamma > 4096)
     return fail(IBFS_ERR_ARG, "hybrid_gamma must be in [0, 4096] (got %d)", c.hybrid_gamma);
   if (c.hybrid_hubs < 0) return fail(IBFS_ERR_ARG, "hybrid_hubs must be >= 0");
+  if (c.split_cov < 0 || c.split_cov > 100)
+    return fail(IBFS_ERR_ARG, "split_cov must be in [0, 100] (got %d)", c.split_cov);
   return IBFS_OK;
 }
 
@@ -89,3 +91,9 @@ struct HubCut {
   int32_t deg = INT32_MAX, tie = -1, cut = 0, rho_q10 = 0;
 };
 HubCut hub_cut(const ibfs_graph_s *g, int64_t r);
+// Hubs the split rule of device-driven levels samples: the pulled hubs, or
+// the top 4,096 when no hub is pulled, at most the hub cache.
+inline int split_hubs(const ibfs_graph_s *g, const HubCut &hc) {
+  const int64_t r = hc.cut > 0 ? hc.cut : 4096;
+  return (int)(r < g->H ? r : g->H);
+}

@@ -435,6 +435,7 @@ int ibfs_config_default(ibfs_config *cfg) {
   // bottom-up candidate of a fast group scanning its whole list (R-MAT s22,
   // 256 lanes x 2 contexts: 10.3K -> 11.7K GTEPS)
   cfg->pack_sources = 1;
+  cfg->split_cov = 0;
   cfg->wg_size = 512;
   cfg->td_chunk = 2048;
   cfg->ring_levels = 32;

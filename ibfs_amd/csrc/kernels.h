@@ -41,17 +41,19 @@ enum { Q_TD_S = 0, Q_TD_M = 1, Q_TD_L = 2, Q_BU_S = 3, Q_BU_M = 4, Q_BU_L = 5 };
 // next level's direction here after each inspection, the expander reads it.
 struct Plan {
   int level;            // level to expand next (== level being inspected)
-  int dir;              // direction of `level`: 0 top-down, 1 bottom-up, 2 hybrid
+  int dir;              // direction of `level`: 0 top-down, 1 bottom-up, 2 hybrid, 3 split
   int done;             // 1 once a level produced no change
   int last;             // deepest level with a change (valid when done)
   unsigned int blocks_done;  // unused (kept for the layout the host copies)
   int policy, sw_level, alpha, beta;
   int gamma;            // hybrid cost factor (1/64 units; 0: never hybrid)
   int rho;              // hub entries per adjacency entry of the graph (1/1024 units)
-  int pad;
+  int split_cov;        // split levels: % of the top hubs a lane must hold to stay
+                        // bottom-up (0: no split levels)
   u64 nf_prev;          // frontier size of the level before `level`
   u64 V;
   int dirlog[256];      // direction per level, indexed level & 255
+  u64 split[MAX_W];     // split level (dir 3): lanes that go bottom-up
 };
 
 // Plan of a device-driven traversal before its level 0 (V: vertex count
@@ -64,6 +66,7 @@ inline void plan_init(Plan *p, const ibfs_config &c, int rho, u64 V) {
   p->beta = c.beta;
   p->gamma = c.hybrid_gamma;
   p->rho = rho;
+  p->split_cov = c.split_cov;
   p->nf_prev = ~0ull;
   p->V = V;
   p->dir = -1;
@@ -196,8 +199,12 @@ hipError_t launch_expand_dd(int W, const int64_t *beg, const int32_t *adj, const
                             u64 *ring, uint8_t *flags, int R, int64_t V, const LevelCtl *ctl_ring,
                             const Plan *plan, const u64 *hs, Queues q, int chunk_shift,
                             HubSplit hsplit, int grid, hipStream_t st);
-// Device-driven direction decision for plan->level + 1 (one thread).
-hipError_t launch_plan_decide(Plan *plan, const LevelCtl *ctl_ring, int R, hipStream_t st);
+// Device-driven direction decision for plan->level + 1; a bottom-up choice
+// becomes a split level when some active lanes hold fewer than
+// plan->split_cov % of the top `hubs` hubs in the hub cache hs (W words per
+// hub).
+hipError_t launch_plan_decide(Plan *plan, const LevelCtl *ctl_ring, int R, const u64 *hs, int W,
+                              int hubs, hipStream_t st);
 // Depth emission over the snapshot window [b, e] (ring slots L % R, each
 // Vt*W words).  level_masks[(L-b)*W + w] = lanes admitted at or before L.
 // mode 0: one vertex per thread, per-lane 4-byte stores; mode 1: LDS-tiled,

@@ -306,14 +306,56 @@ __global__ __launch_bounds__(NW * 64) void k_inspect(InspectArgs a) {
   }
 }
 
-// Device-driven decision after the inspection of plan->level (one thread).
-// A separate launch instead of a last-workgroup election inside the
-// inspector: on this multi-XCD part each workgroup's agent-scope fence
-// would write back and invalidate its L2 in the middle of the pass.
-__global__ void k_plan_decide(Plan *plan, const LevelCtl *ctl_ring, int R) {
-  if (threadIdx.x != 0 || plan->done) return;
-  const int L = plan->level;
-  plan_decide(plan, ctl_ring + L % R, L);
+// Device-driven decision after the inspection of plan->level.  A separate
+// launch instead of a last-workgroup election inside the inspector: on this
+// multi-XCD part each workgroup's agent-scope fence would write back and
+// invalidate its L2 in the middle of the pass.
+// Thread 0 applies the direction rule.  A bottom-up choice with split_cov > 0
+// then samples 64 of the top `hubs` hubs in the hub cache hs (S_L words, W per
+// hub; wave w takes word w): an active lane set in at least split_cov % of
+// the sample is dense and stays bottom-up, the other active lanes push
+// top-down.  The level splits (dir 3) when both kinds are active.  A lane
+// that holds few hubs has reached little of the graph: its candidates would
+// scan their whole lists without covering it, while its own frontier is small.
+__global__ __launch_bounds__(512) void k_plan_decide(Plan *plan, const LevelCtl *ctl_ring, int R,
+                                                     const u64 *hs, int W, int hubs) {
+  __shared__ int dir_s;
+  __shared__ u64 dense[MAX_W];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int L = plan->level, done = plan->done, cov = plan->split_cov;
+  __syncthreads();  // every wave has read the plan before thread 0 changes it
+  if (done) return;
+  if (t == 0) {
+    plan_decide(plan, ctl_ring + L % R, L);
+    dir_s = plan->done ? -1 : plan->dir;
+  }
+  __syncthreads();
+  if (dir_s != 1 || cov <= 0 || hubs <= 0 || R < 3) return;
+  const int ns = hubs < 64 ? hubs : 64, step = hubs / ns;
+  if (wv < W) {
+    const u64 x = lane < ns ? hs[(int64_t)lane * step * W + wv] : 0;
+    u64 d = 0;
+    for (int b = 0; b < 64; ++b) {
+      const u64 m = __ballot((x >> b) & 1);
+      if (__popcll(m) * 100 >= cov * ns) d |= 1ull << b;
+    }
+    if (lane == 0) dense[wv] = d;
+  }
+  __syncthreads();
+  if (t == 0) {
+    const LevelCtl *c = ctl_ring + L % R;
+    u64 dn = 0, sp = 0;
+    for (int w = 0; w < W; ++w) {
+      const u64 lw = atomic_read(&c->lam[w]);
+      dn |= lw & dense[w];
+      sp |= lw & ~dense[w];
+    }
+    if (dn && sp) {
+      for (int w = 0; w < W; ++w) plan->split[w] = dense[w];
+      plan->dir = 3;
+      plan->dirlog[(L + 1) & 255] = 3;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -323,16 +365,35 @@ __global__ void k_plan_decide(Plan *plan, const LevelCtl *ctl_ring, int R) {
 // Top-down levels flag nothing: their changes are scattered, and a byte store
 // per push costs a partial-line write once the line is evicted; the inspector
 // diffs every vertex after a top-down level instead.
-template <int W>
+// NEW (split levels): only the bits u gained at L-1 (S_{L-1} & ~S_{L-2}, pp
+// = S_{L-2} or null) are pushed, and u is skipped when none is in lam.
+template <int W, bool NEW = false>
+__device__ __forceinline__ bool td_bits(const u64 *__restrict__ prev, const u64 *__restrict__ pp,
+                                        const u64 (&lam)[W], int32_t u, u64 (&p)[W]) {
+  load_word<W>(prev + (int64_t)u * W, p);
+  u64 any = 0;
+  if (NEW && pp) {
+    u64 o[W];
+    load_word<W>(pp + (int64_t)u * W, o);
+#pragma unroll
+    for (int w = 0; w < W; ++w) p[w] &= ~o[w];
+  }
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    p[w] &= lam[w];
+    any |= p[w];
+  }
+  return !NEW || any != 0;
+}
+
+template <int W, bool NEW = false>
 __device__ __forceinline__ void td_push(const int64_t *__restrict__ beg,
                                         const int32_t *__restrict__ adj,
                                         const u64 *__restrict__ prev, u64 *cur,
                                         const u64 (&lam)[W], int32_t u, int64_t j0, int64_t j1,
-                                        int stride) {
+                                        int stride, const u64 *pp = nullptr) {
   u64 p[W];
-  load_word<W>(prev + (int64_t)u * W, p);
-#pragma unroll
-  for (int w = 0; w < W; ++w) p[w] &= lam[w];
+  if (!td_bits<W, NEW>(prev, pp, lam, u, p)) return;
   for (int64_t j = j0; j < j1; j += stride) {
     const int64_t v = adj[j];
     u64 c[W];
@@ -346,15 +407,14 @@ __device__ __forceinline__ void td_push(const int64_t *__restrict__ beg,
 // One thread per small frontier vertex: four neighbours per step, their
 // adjacency entries and words loaded before any atomic, so four dependent
 // load chains are in flight instead of one.
-template <int W>
+template <int W, bool NEW = false>
 __device__ __forceinline__ void td_push_small(const int64_t *__restrict__ beg,
                                               const int32_t *__restrict__ adj,
                                               const u64 *__restrict__ prev, u64 *cur,
-                                              const u64 (&lam)[W], int32_t u) {
+                                              const u64 (&lam)[W], int32_t u,
+                                              const u64 *pp = nullptr) {
   u64 p[W];
-  load_word<W>(prev + (int64_t)u * W, p);
-#pragma unroll
-  for (int w = 0; w < W; ++w) p[w] &= lam[w];
+  if (!td_bits<W, NEW>(prev, pp, lam, u, p)) return;
   int64_t j = beg[u];
   const int64_t e = beg[u + 1];
   for (; j + 4 <= e; j += 4) {
@@ -415,15 +475,16 @@ __device__ __forceinline__ void bu_commit(u64 *cur, uint8_t *flags, int64_t v, c
 // order (descending degree, hubs first) and stops as soon as its word
 // covers lam_{L-1} (bitwise early termination, expander.cu:258-359).
 // One candidate per thread, IBFS_BU_STEP neighbours per step.
-// HY (hybrid level): only the prefix of hub entries with slot < cut is read
-// (from the hub cache), and new bits go in with atomic ORs (the other
-// frontier vertices push into the same words concurrently); nothing is
-// flagged.
+// MODE 0: bottom-up level.  MODE 1 (hybrid level): only the prefix of hub
+// entries with slot < cut is read (from the hub cache).  MODE 2 (split
+// level): the whole list, for the dense lanes in lam.  In modes 1 and 2 new
+// bits go in with atomic ORs (frontier vertices push into the same words
+// concurrently) and nothing is flagged.
 __device__ __forceinline__ bool pulled(int32_t a, int cut) {
   return a < 0 && (a & 0x7fffffff) < cut;
 }
 
-template <int W, bool HY = false>
+template <int W, int MODE = 0>
 __device__ __forceinline__ void bu_thread(const int64_t *__restrict__ beg,
                                           const int32_t *__restrict__ tadj, const Gather &gs,
                                           u64 *cur, uint8_t *flags,
@@ -444,7 +505,7 @@ __device__ __forceinline__ void bu_thread(const int64_t *__restrict__ beg,
     bool nonhub = false;
 #pragma unroll
     for (int r = 0; r < IBFS_BU_STEP; ++r) {
-      if (HY && !pulled(an[r], cut)) {
+      if (MODE == 1 && !pulled(an[r], cut)) {
         nonhub = true;
 #pragma unroll
         for (int w = 0; w < W; ++w) sn[r][w] = 0;
@@ -459,11 +520,11 @@ __device__ __forceinline__ void bu_thread(const int64_t *__restrict__ beg,
     j += IBFS_BU_STEP;
     if (covers<W>(x, lam) || nonhub) break;  // pulled prefix ended
   }
-  bu_commit<W, HY>(cur, flags, v, x, x0);
+  bu_commit<W, MODE != 0>(cur, flags, v, x, x0);
 }
 
 // One candidate per wave: 64 neighbours per step, wave OR-reduction.
-template <int W, bool HY = false>
+template <int W, int MODE = 0>
 __device__ __forceinline__ void bu_wave(const int64_t *__restrict__ beg,
                                         const int32_t *__restrict__ tadj, const Gather &gs,
                                         u64 *cur, uint8_t *flags,
@@ -482,21 +543,21 @@ __device__ __forceinline__ void bu_wave(const int64_t *__restrict__ beg,
     bool nonhub = false;
     if (j < e) {
       const int32_t a = tadj[j];
-      if (HY && !pulled(a, cut)) nonhub = true;
+      if (MODE == 1 && !pulled(a, cut)) nonhub = true;
       else load_nbr<W>(gs, a, y);
     }
 #pragma unroll
     for (int w = 0; w < W; ++w) x[w] |= wave_or(y[w] & lam[w]);
     if (covers<W>(x, lam)) break;
-    if (HY && __ballot(nonhub)) break;  // pulled prefix ended
+    if (MODE == 1 && __ballot(nonhub)) break;  // pulled prefix ended
   }
-  if (lane == 0) bu_commit<W, HY>(cur, flags, v, x, x0);
+  if (lane == 0) bu_commit<W, MODE != 0>(cur, flags, v, x, x0);
 }
 
 // One candidate per workgroup: 256 neighbours per step, the four wave
 // partials combined through LDS.  Must be called by the whole workgroup.
-// nh: four ints of LDS (pulled prefix ended, per wave; used by HY).
-template <int W, bool HY = false>
+// nh: four ints of LDS (pulled prefix ended, per wave; used by MODE 1).
+template <int W, int MODE = 0>
 __device__ __forceinline__ void bu_block(const int64_t *__restrict__ beg,
                                          const int32_t *__restrict__ tadj, const Gather &gs,
                                          u64 *cur, uint8_t *flags,
@@ -504,9 +565,9 @@ __device__ __forceinline__ void bu_block(const int64_t *__restrict__ beg,
                                          int *nh, int cut = 0) {
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   u64 x[W], x0[W];
-  if (HY) {
+  if (MODE != 0) {
     // One read of the candidate's word for the whole workgroup: in a hybrid
-    // level pushes from other workgroups can change it between the waves'
+    // or split level pushes from other workgroups can change it between the waves'
     // own reads, and every wave must take the same covers/stop branches
     // around the barriers below.
     if (t == 0) {
@@ -533,25 +594,25 @@ __device__ __forceinline__ void bu_block(const int64_t *__restrict__ beg,
     bool nonhub = false;
     if (j < e) {
       const int32_t a = tadj[j];
-      if (HY && !pulled(a, cut)) nonhub = true;
+      if (MODE == 1 && !pulled(a, cut)) nonhub = true;
       else load_nbr<W>(gs, a, y);
     }
 #pragma unroll
     for (int w = 0; w < W; ++w) y[w] = wave_or(y[w] & lam[w]);
-    const bool wnh = HY && __ballot(nonhub) != 0;
+    const bool wnh = MODE == 1 && __ballot(nonhub) != 0;
     if (lane == 0) {
 #pragma unroll
       for (int w = 0; w < W; ++w) part[wv][w] = y[w];
-      if (HY) nh[wv] = wnh;
+      if (MODE == 1) nh[wv] = wnh;
     }
     __syncthreads();
 #pragma unroll
     for (int w = 0; w < W; ++w) x[w] |= part[0][w] | part[1][w] | part[2][w] | part[3][w];
-    const bool stop = HY && (nh[0] | nh[1] | nh[2] | nh[3]);
+    const bool stop = MODE == 1 && (nh[0] | nh[1] | nh[2] | nh[3]);
     __syncthreads();
     if (covers<W>(x, lam) || stop) break;
   }
-  if (t == 0) bu_commit<W, HY>(cur, flags, v, x, x0);
+  if (t == 0) bu_commit<W, MODE != 0>(cur, flags, v, x, x0);
 }
 
 // Host-driven top-down: workgroups [0, nbs) take small vertices one per
@@ -632,11 +693,11 @@ __device__ __forceinline__ void expand_hybrid(const int64_t *__restrict__ beg,
   const int64_t G = gridDim.x;
   const int64_t bs = ctl->q[Q_BU_S], bm = ctl->q[Q_BU_M], bl = ctl->q[Q_BU_L];
   for (int64_t i = (int64_t)blockIdx.x * BS + t; i < bs; i += G * BS)
-    bu_thread<W, true>(beg, tadj, gs, cur, nullptr, lam, q.q[Q_BU_S][i], hsp.cut);
+    bu_thread<W, 1>(beg, tadj, gs, cur, nullptr, lam, q.q[Q_BU_S][i], hsp.cut);
   for (int64_t i = (int64_t)blockIdx.x * 4 + wv; i < bm; i += G * 4)
-    bu_wave<W, true>(beg, tadj, gs, cur, nullptr, lam, q.q[Q_BU_M][i], lane, hsp.cut);
+    bu_wave<W, 1>(beg, tadj, gs, cur, nullptr, lam, q.q[Q_BU_M][i], lane, hsp.cut);
   for (int64_t i = blockIdx.x; i < bl; i += G)
-    bu_block<W, true>(beg, tadj, gs, cur, nullptr, lam, q.q[Q_BU_L][i], part, nh, hsp.cut);
+    bu_block<W, 1>(beg, tadj, gs, cur, nullptr, lam, q.q[Q_BU_L][i], part, nh, hsp.cut);
   const int64_t ts = ctl->q[Q_TD_S], tm = ctl->q[Q_TD_M], tl = ctl->q[Q_TD_L];
   for (int64_t i = (int64_t)blockIdx.x * BS + t; i < ts; i += G * BS) {
     const int32_t u = q.q[Q_TD_S][i];
@@ -713,6 +774,37 @@ __global__ __launch_bounds__(BS) void k_expand_dd(const int64_t *__restrict__ be
   } else if (dir == 2) {
     const Gather gs{prev, hs};
     expand_hybrid<W>(beg, adj, tadj, gs, cur, lam, c, q, cs, hsp, part, nh);
+  } else if (dir == 3) {
+    // split level: the dense lanes pull over whole lists, the sparse lanes
+    // push the bits their frontier vertices gained at L-1
+    const Gather gs{prev, hs};
+    const u64 *pp = L >= 2 ? ring + (int64_t)((L - 2) % R) * slot : nullptr;
+    u64 ld[W], ls[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      ld[w] = lam[w] & plan->split[w];
+      ls[w] = lam[w] & ~plan->split[w];
+    }
+    const int64_t bs = c->q[Q_BU_S], bm = c->q[Q_BU_M], bl = c->q[Q_BU_L];
+    for (int64_t i = (int64_t)blockIdx.x * BS + t; i < bs; i += G * BS)
+      bu_thread<W, 2>(beg, tadj, gs, cur, nullptr, ld, q.q[Q_BU_S][i]);
+    for (int64_t i = (int64_t)blockIdx.x * 4 + wv; i < bm; i += G * 4)
+      bu_wave<W, 2>(beg, tadj, gs, cur, nullptr, ld, q.q[Q_BU_M][i], lane);
+    for (int64_t i = blockIdx.x; i < bl; i += G)
+      bu_block<W, 2>(beg, tadj, gs, cur, nullptr, ld, q.q[Q_BU_L][i], part, nh);
+    const int64_t ns = c->q[Q_TD_S], nm = c->q[Q_TD_M], nl = c->q[Q_TD_L];
+    for (int64_t i = (int64_t)blockIdx.x * BS + t; i < ns; i += G * BS)
+      td_push_small<W, true>(beg, adj, prev, cur, ls, q.q[Q_TD_S][i], pp);
+    for (int64_t i = (int64_t)blockIdx.x * 4 + wv; i < nm; i += G * 4) {
+      const int32_t u = q.q[Q_TD_M][i];
+      td_push<W, true>(beg, adj, prev, cur, ls, u, beg[u] + lane, beg[u + 1], 64, pp);
+    }
+    for (int64_t i = blockIdx.x; i < nl; i += G) {
+      const int2 e = q.chunks[i];
+      int64_t lo, hi;
+      td_chunk_range(beg, e, cs, lo, hi);
+      td_push<W, true>(beg, adj, prev, cur, ls, e.x, lo + t, hi, BS, pp);
+    }
   } else {
     const int64_t ns = c->q[Q_BU_S], nm = c->q[Q_BU_M], nl = c->q[Q_BU_L];
     const Gather gs{prev, hs};
@@ -917,8 +1009,9 @@ hipError_t launch_expand_hy(int W, const int64_t *beg, const int32_t *adj, const
   return hipGetLastError();
 }
 
-hipError_t launch_plan_decide(Plan *plan, const LevelCtl *ctl_ring, int R, hipStream_t st) {
-  hipLaunchKernelGGL(k_plan_decide, dim3(1), dim3(64), 0, st, plan, ctl_ring, R);
+hipError_t launch_plan_decide(Plan *plan, const LevelCtl *ctl_ring, int R, const u64 *hs, int W,
+                              int hubs, hipStream_t st) {
+  hipLaunchKernelGGL(k_plan_decide, dim3(1), dim3(512), 0, st, plan, ctl_ring, R, hs, W, hubs);
   return hipGetLastError();
 }
 

@@ -84,13 +84,18 @@ typedef struct ibfs_config {
                           two-hop sizes (their lanes progress alike), 0 =
                           groups in the caller's order; outputs stay in the
                           caller's order either way                          */
+  int32_t split_cov;    /* batched runs and serving (device-driven levels): a
+                          bottom-up level lets the active lanes that hold
+                          fewer than split_cov % of the top hybrid_hubs hubs
+                          push top-down instead (their sparse frontiers would
+                          keep every candidate scanning); 0: never           */
 } ibfs_config;
 
 /* Per-level record of the last traversal (the build's monitoring log, in place
  * of ENABLE_MONITORING prints at traversal.cu:127-128). */
 typedef struct ibfs_level_info {
   int32_t level;       /* level L: the pass that produced status_L             */
-  int32_t direction;   /* 0 top-down, 1 bottom-up, 2 hybrid, -1 init          */
+  int32_t direction;   /* 0 top-down, 1 bottom-up, 2 hybrid, 3 split, -1 init */
   int64_t n_changed;   /* vertices whose status gained bits at L               */
   int64_t n_frontier;  /* changed vertices with degree > 0                     */
   int64_t m_frontier;  /* sum of degrees of the frontier                       */
@@ -114,7 +119,8 @@ typedef struct ibfs_stats {
   int32_t n_td_launches, n_bu_launches, n_inspect_launches, n_emit_launches;
   double expand_hy_ms;     /* hybrid expansions (timing=1)                     */
   int32_t n_hy_launches;
-  int32_t reserved;
+  int32_t n_split_levels;  /* batched runs: split levels (dense lanes pull,
+                              sparse lanes push; cfg.split_cov)               */
 } ibfs_stats;
 
 /* ---- errors / version ---------------------------------------------------- */

@@ -101,3 +101,20 @@ def test_hybrid_levels_with_pushes_into_workgroup_candidates():
             say(f"[gpu] chung-lu 400K hybrid cut={cut} run {rep}")
             np.testing.assert_array_equal(eng.run_groups(src), want)
             np.testing.assert_array_equal(b.run(src), want)
+
+
+def test_split_levels_chung_lu_400k():
+    """Split levels on a Chung-Lu graph (heavy-tailed degrees: workgroup-class
+    candidates pull the dense lanes while the sparse lanes' frontier pushes
+    into the same words), 512 sources in caller order, coverage 50 %, three
+    runs against the oracle."""
+    h = ibfs_amd.HostCsr.chung_lu(400_000, 2.2, 76.0, 20000.0, seed=5, threads=8)
+    beg, adj = h.arrays()
+    g = ibfs_amd.DeviceGraph(h)
+    src = h.gen_sources(512, seed=7)
+    want = oracle.bfs_many(beg, adj, src, threads=8)
+    b = ibfs_amd.Batch(g, contexts=2, burst=4, lane_bits=256, pack_sources=0, split_cov=50)
+    for rep in range(3):
+        say(f"[gpu] chung-lu 400K split run {rep}")
+        np.testing.assert_array_equal(b.run(src), want)
+    assert b.last_stats.n_split_levels > 0

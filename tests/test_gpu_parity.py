@@ -506,7 +506,7 @@ def test_hybrid_is_chosen_on_hub_heavy_levels(s16):
 
 def test_direction_config_checks(s16):
     for bad in (dict(policy=5), dict(hybrid_gamma=-1), dict(hybrid_gamma=5000),
-                dict(hybrid_hubs=-1)):
+                dict(hybrid_hubs=-1), dict(split_cov=-1), dict(split_cov=101)):
         with pytest.raises(ibfs_amd.IbfsError):
             ibfs_amd.Engine(s16["g"], **bad)
         with pytest.raises(ibfs_amd.IbfsError):
@@ -574,3 +574,34 @@ def test_batch_packing_keeps_caller_order(s16):
         np.testing.assert_array_equal(dsum, want_sum)
         np.testing.assert_array_equal(reach, want_reach)
         b.close()
+
+
+def test_split_levels_every_coverage(s16):
+    """Split levels (dense lanes pull over whole lists with atomic commits,
+    sparse lanes push the bits they gained one level before) at coverage
+    thresholds 1, 50 and 99 %, in caller order so that slow and fast sources
+    share groups, with rings of 4 and 32 levels, batched and served: every
+    depth array and closeness equals the oracle's, and split levels occur."""
+    V = s16["h"].vert_count
+    src = s16["h"].gen_sources(512, seed=41)
+    want = oracle.bfs_many(s16["beg"], s16["adj"], src, threads=8)
+    want_sum, want_reach = _close_ref(want)
+    splits = 0
+    for cov in (1, 50, 99):
+        for ring in (4, 32):
+            say(f"[gpu] split levels cov={cov} ring={ring}")
+            b = ibfs_amd.Batch(s16["g"], contexts=2, burst=4, lane_bits=256, pack_sources=0,
+                               split_cov=cov, ring_levels=ring)
+            np.testing.assert_array_equal(b.run(src), want, err_msg=f"cov={cov} ring={ring}")
+            splits += b.last_stats.n_split_levels
+            dsum, reach = b.closeness(src)
+            np.testing.assert_array_equal(dsum, want_sum)
+            np.testing.assert_array_equal(reach, want_reach)
+            b.close()
+        srv = ibfs_amd.Server(s16["g"], lane_bits=256, split_cov=cov)
+        queries = [src[:200], src[200:201], src[201:512]]
+        outs, _ = _serve(srv, queries, V, gap_s=0.002)
+        for q, o in zip(queries, outs):
+            np.testing.assert_array_equal(o, oracle.bfs_many(s16["beg"], s16["adj"], q, threads=8))
+        srv.close()
+    assert splits > 0

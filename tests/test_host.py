@@ -39,7 +39,7 @@ def test_abi_version_and_defaults():
     cfg = ibfs_amd.default_config()
     assert cfg.lane_bits == 128 and cfg.sml_shed == 64 and cfg.lrg_shed == 512
     assert cfg.policy == _lib.POLICY_ALPHA_BETA
-    assert cfg.hybrid_gamma == 16 and cfg.hybrid_hubs == 4096
+    assert cfg.hybrid_gamma == 16 and cfg.hybrid_hubs == 4096 and cfg.split_cov == 0
     assert cfg.ring_levels >= 4
 
 
