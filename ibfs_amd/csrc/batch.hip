@@ -208,7 +208,7 @@ int alloc_ctx(ibfs_batch_s *b, Ctx &c) {
                                    c.ctl, c.d_plan, c.hs, c.q, b->chunk_shift,
                                    HubSplit{b->hc.deg, b->hc.tie, b->hc.cut}, b->grid, c.st);
   hipError_t e2 = launch_inspect(b->W, dd_inspect_args(b, c), c.st);
-  if (e2 == hipSuccess) e2 = launch_plan_decide(c.d_plan, c.ctl, b->R, c.hs, b->W, split_hubs(b->g, b->hc), c.st);
+  if (e2 == hipSuccess) e2 = launch_plan_decide(c.d_plan, c.ctl, b->R, c.hs, b->W, split_hubs(b->g, b->hc, b->cfg), c.st);
   hipError_t e3 = hipStreamEndCapture(c.st, &c.graph);
   if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
     return fail(IBFS_ERR_HIP, "level graph capture failed: %s / %s / %s", hipGetErrorString(e1),
@@ -247,7 +247,7 @@ int start_group(ibfs_batch_s *b, Ctx &c, const int32_t *src, int64_t g0, int k, 
   HIP_TRY(launch_fill_u64(reinterpret_cast<u64 *>(c.ctl), (int64_t)b->R * CTL_WORDS, 0ull, c.st));
   HIP_TRY(launch_init_sources(W, c.d_src, k, 0, slot(b, c, 0), c.flags, c.st));
   HIP_TRY(launch_inspect(W, dd_inspect_args(b, c), c.st));
-  HIP_TRY(launch_plan_decide(c.d_plan, c.ctl, b->R, c.hs, W, split_hubs(b->g, b->hc), c.st));
+  HIP_TRY(launch_plan_decide(c.d_plan, c.ctl, b->R, c.hs, W, split_hubs(b->g, b->hc, b->cfg), c.st));
   c.g0 = g0;
   c.k = k;
   c.base = 0;

@@ -92,8 +92,10 @@ struct HubCut {
 };
 HubCut hub_cut(const ibfs_graph_s *g, int64_t r);
 // Hubs the split rule of device-driven levels samples: the pulled hubs, or
-// the top 4,096 when no hub is pulled, at most the hub cache.
-inline int split_hubs(const ibfs_graph_s *g, const HubCut &hc) {
+// the top 4,096 when no hub is pulled, at most the hub cache; 0 when split
+// levels are off (the decision kernel then runs as one wave).
+inline int split_hubs(const ibfs_graph_s *g, const HubCut &hc, const ibfs_config &c) {
+  if (c.split_cov <= 0) return 0;
   const int64_t r = hc.cut > 0 ? hc.cut : 4096;
   return (int)(r < g->H ? r : g->H);
 }

@@ -330,7 +330,7 @@ __global__ __launch_bounds__(512) void k_plan_decide(Plan *plan, const LevelCtl 
     dir_s = plan->done ? -1 : plan->dir;
   }
   __syncthreads();
-  if (dir_s != 1 || cov <= 0 || hubs <= 0 || R < 3) return;
+  if (dir_s != 1 || cov <= 0 || hubs <= 0 || R < 3) return;  // (hubs > 0: 64 * W threads)
   const int ns = hubs < 64 ? hubs : 64, step = hubs / ns;
   if (wv < W) {
     const u64 x = lane < ns ? hs[(int64_t)lane * step * W + wv] : 0;
@@ -1011,7 +1011,10 @@ hipError_t launch_expand_hy(int W, const int64_t *beg, const int32_t *adj, const
 
 hipError_t launch_plan_decide(Plan *plan, const LevelCtl *ctl_ring, int R, const u64 *hs, int W,
                               int hubs, hipStream_t st) {
-  hipLaunchKernelGGL(k_plan_decide, dim3(1), dim3(512), 0, st, plan, ctl_ring, R, hs, W, hubs);
+  // one wave without the split rule: a wider workgroup waits longer for a CU
+  // while another context's expansion fills the device
+  hipLaunchKernelGGL(k_plan_decide, dim3(1), dim3(hubs > 0 ? 64 * W : 64), 0, st, plan, ctl_ring,
+                     R, hs, W, hubs);
   return hipGetLastError();
 }
 

@@ -264,7 +264,7 @@ int alloc_ctx(ibfs_server_s *s, SCtx &c) {
                                      s->R, s->g->Vt, c.ctl, c.d_plan, c.hs, c.q, s->chunk_shift,
                                      HubSplit{s->hc.deg, s->hc.tie, s->hc.cut}, s->grid, c.st);
     hipError_t e2 = launch_inspect(Wc, plan_args(s, c), c.st);
-    if (e2 == hipSuccess) e2 = launch_plan_decide(c.d_plan, c.ctl, s->R, c.hs, Wc, split_hubs(s->g, s->hc), c.st);
+    if (e2 == hipSuccess) e2 = launch_plan_decide(c.d_plan, c.ctl, s->R, c.hs, Wc, split_hubs(s->g, s->hc, s->cfg), c.st);
     hipError_t e3 = hipStreamEndCapture(c.st, &c.graph[i]);
     if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
       return fail(IBFS_ERR_HIP, "cohort level capture failed: %s / %s / %s", hipGetErrorString(e1),
@@ -601,7 +601,7 @@ int start_cohort(ibfs_server_s *s, SCtx &c) {
   plan_init(c.h_plan, s->cfg, s->hc.rho_q10, (u64)s->g->V);
   HIP_CHK(hipMemcpyAsync(c.d_plan, c.h_plan, sizeof(Plan), hipMemcpyHostToDevice, c.st));
   HIP_CHK(launch_inspect(W, plan_args(s, c), c.st));
-  HIP_CHK(launch_plan_decide(c.d_plan, c.ctl, s->R, c.hs, W, split_hubs(s->g, s->hc), c.st));
+  HIP_CHK(launch_plan_decide(c.d_plan, c.ctl, s->R, c.hs, W, split_hubs(s->g, s->hc, s->cfg), c.st));
   c.coh = 1;
   c.dlevel = 1;
   c.base = 0;
