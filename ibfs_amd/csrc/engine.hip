@@ -30,6 +30,7 @@ struct ibfs_engine_s {
   int hy_grid = 8192;  // workgroups of a hybrid expansion
   HubCut hc;           // hubs a hybrid level pulls through
   hipStream_t st = nullptr;
+  hipEvent_t ev_wait = nullptr;  // recorded after each host copy the loop waits for
   u64 *ring = nullptr;
   LevelCtl *d_ctl = nullptr;
   LevelCtl *h_ctl = nullptr;  // pinned
@@ -50,18 +51,23 @@ struct ibfs_engine_s {
   std::vector<ibfs_level_info> log;
   std::vector<hipEvent_t> ev;  // event pool
   size_t ev_used = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> emit_ev;  // emission timings of a traversal
 };
 
 namespace {
 
 size_t masks_words(const ibfs_engine_s *e) { return (size_t)(2 + (e->R + 1)) * e->W; }
 
-// Waits for the stream by polling it.  The host-driven level loop waits on
-// the control-block copy after every inspection; a blocking wait's wake-up
-// cost tens of microseconds per level, more than the sparse levels' kernels.
-hipError_t poll_stream(hipStream_t st) {
+// Waits for the work enqueued so far by polling an event recorded behind it.
+// The host-driven level loop waits on the control-block copy after every
+// inspection; a blocking wait's wake-up costs tens of microseconds per level,
+// more than the sparse levels' kernels, and an event query reads one
+// completion signal where a stream query takes the stream's lock.
+hipError_t poll_stream(hipStream_t st, hipEvent_t ev) {
+  const hipError_t r = hipEventRecord(ev, st);
+  if (r != hipSuccess) return r;
   for (;;) {
-    const hipError_t q = hipStreamQuery(st);
+    const hipError_t q = hipEventQuery(ev);
     if (q != hipErrorNotReady) return q;
   }
 }
@@ -118,6 +124,8 @@ void engine_release(ibfs_engine_s *e) {
   dfree(e->flags);
   for (auto x : e->ev) (void)hipEventDestroy(x);
   e->ev.clear();
+  if (e->ev_wait) (void)hipEventDestroy(e->ev_wait);
+  e->ev_wait = nullptr;
   if (e->st) (void)hipStreamDestroy(e->st);
   e->st = nullptr;
 }
@@ -189,10 +197,7 @@ int emit_window(ibfs_engine_s *e, int b, int en, int k, ibfs_stats *stats, u64 *
   if (e->cfg.kernel_timing) {
     e1 = next_event(e);
     HIP_TRY(hipEventRecord(e1, e->st));
-    HIP_TRY(hipEventSynchronize(e1));
-    float ms = 0;
-    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-    if (stats) stats->emit_ms += ms;
+    e->emit_ev.emplace_back(e0, e1);  // read after the traversal's last wait
   }
   if (stats) stats->n_emit_launches++;
   return IBFS_OK;
@@ -207,6 +212,7 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
   while (e->W * 64 < k && e->W < e->Wmax) e->W *= 2;
   const int W = e->W;
   e->ev_used = 0;
+  e->emit_ev.clear();
   e->log.clear();
 
   // lanes, masks, sources (pinned staging -> device)
@@ -243,7 +249,7 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
                          e->st));
   if (timing) { ti1 = next_event(e); HIP_TRY(hipEventRecord(ti1, e->st)); }
   HIP_TRY(hipMemcpyAsync(e->h_ctl, &e->d_ctl[0], sizeof(LevelCtl), hipMemcpyDeviceToHost, e->st));
-  HIP_TRY(poll_stream(e->st));
+  HIP_TRY(poll_stream(e->st, e->ev_wait));
   auto record = [&](int L, int dir, const LevelCtl &c, float exp_ms, float insp_ms) {
     ibfs_level_info li{};
     li.level = L;
@@ -329,7 +335,7 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
     if (timing) { y1 = next_event(e); HIP_TRY(hipEventRecord(y1, e->st)); }
     HIP_TRY(hipMemcpyAsync(e->h_ctl, &e->d_ctl[L % e->R], sizeof(LevelCtl),
                            hipMemcpyDeviceToHost, e->st));
-    HIP_TRY(poll_stream(e->st));
+    HIP_TRY(poll_stream(e->st, e->ev_wait));
     float ems = 0, ims = 0;
     if (timing) {
       HIP_TRY(hipEventElapsedTime(&ems, x0, x1));
@@ -362,20 +368,26 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
     if (stats) stats->emit_windows++;
     counted = stats != nullptr;
   }
+  // one wait for the group: its end event, the edge count and its copy
   hipEvent_t t_end = next_event(e);
   HIP_TRY(hipEventRecord(t_end, e->st));
-  HIP_TRY(hipEventSynchronize(t_end));
   if (stats) {
-    float ms = 0;
-    HIP_TRY(hipEventElapsedTime(&ms, t_begin, t_end));
-    stats->total_ms += ms;
-    stats->levels = std::max(stats->levels, last + 1);
     if (!counted) {  // no final window (the last flush ended at last + 1)
       HIP_TRY(hipMemsetAsync(e->d_count, 0, sizeof(u64), e->st));
       HIP_TRY(launch_count_edges(W, slot(e, last), g->d_deg, Vt, e->d_masks, e->d_count, e->st));
     }
     HIP_TRY(hipMemcpyAsync(e->h_count, e->d_count, sizeof(u64), hipMemcpyDeviceToHost, e->st));
-    HIP_TRY(poll_stream(e->st));
+  }
+  HIP_TRY(poll_stream(e->st, e->ev_wait));
+  if (stats) {
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, t_begin, t_end));
+    stats->total_ms += ms;
+    for (const auto &x : e->emit_ev) {
+      HIP_TRY(hipEventElapsedTime(&ms, x.first, x.second));
+      stats->emit_ms += ms;
+    }
+    stats->levels = std::max(stats->levels, last + 1);
     stats->traversed_edges += (int64_t)*e->h_count;
   }
   return IBFS_OK;
@@ -750,6 +762,7 @@ int ibfs_engine_create(ibfs_graph_t g, const ibfs_config *cfg_in, ibfs_engine_t 
   while (R > 4 && (size_t)R * slot_bytes > budget) --R;
   e->R = R;
   hipError_t err = hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking);
+  if (err == hipSuccess) err = hipEventCreateWithFlags(&e->ev_wait, hipEventDisableTiming);
   auto A = [&](void **p, size_t bytes) {
     if (err == hipSuccess) err = hipMalloc(p, bytes ? bytes : 1);
   };
