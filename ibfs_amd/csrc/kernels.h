@@ -83,7 +83,8 @@ struct HubSplit {
   int32_t cut;  // hub slots a hybrid level pulls through: [0, cut)
 };
 
-// Direction of level Ln (0 top-down, 1 bottom-up, 2 hybrid) from the
+// Direction of level Ln (0 top-down, 1 bottom-up, 2 hybrid; the device plan
+// may turn a bottom-up choice into a split level, 3) from the
 // counters of level Ln-1, shared by the host-driven drivers and the
 // device-driven plan.  alpha/beta: top-down -> bottom-up when m_f * alpha >
 // m_u, back when n_f * beta < V on a shrinking frontier.  A top-down choice

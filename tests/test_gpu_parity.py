@@ -605,3 +605,25 @@ def test_split_levels_every_coverage(s16):
             np.testing.assert_array_equal(o, oracle.bfs_many(s16["beg"], s16["adj"], q, threads=8))
         srv.close()
     assert splits > 0
+
+
+@pytest.mark.parametrize("opts", [dict(), dict(policy=ibfs_amd.POLICY_FIXED, sw_level=2),
+                                  dict(hybrid_gamma=0), dict(hybrid_gamma=64, hybrid_hubs=64)])
+def test_device_plan_chooses_the_host_directions(s16, opts):
+    """One group through the host-driven engine and through the batched
+    launcher's device-driven plan: both apply the same direction rule to the
+    same counters, so the levels and the number of top-down, bottom-up and
+    hybrid levels agree (and no split level occurs with split levels off)."""
+    say(f"[gpu] device plan directions {opts}")
+    src = s16["src"][:128]
+    eng = ibfs_amd.Engine(s16["g"], lane_bits=128, **opts)
+    eng.traversal(src)
+    levels = eng.last_stats.levels
+    dirs = [li["direction"] for li in eng.level_log() if 1 <= li["level"] < levels]
+    b = ibfs_amd.Batch(s16["g"], contexts=1, burst=3, lane_bits=128, pack_sources=0, **opts)
+    b.run(src)
+    st = b.last_stats
+    assert st.levels == levels
+    assert (st.n_td_launches, st.n_bu_launches, st.n_hy_launches, st.n_split_levels) == \
+        (dirs.count(0), dirs.count(1), dirs.count(2), 0)
+    b.close()
