@@ -290,13 +290,15 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
     }
     if (L > 1 << 20) return fail(IBFS_ERR_STATE, "level limit exceeded");
     const int dir = decide(e, L, c, dir_prev, nf_prev);
+    const bool tdf = dir == 0 && flag_top_down(c.m_front, (u64)g->V);
     // lam of level L-1 is the first member of its control block
     const u64 *lamp = reinterpret_cast<const u64 *>(e->d_ctl + (L - 1) % e->R);
     hipEvent_t x0 = nullptr, x1 = nullptr, y1 = nullptr;
     if (timing) { x0 = next_event(e); HIP_TRY(hipEventRecord(x0, e->st)); }
     if (dir == 0) {
       HIP_TRY(launch_expand_td(W, g->d_beg, g->d_adj, slot(e, L - 1), slot(e, L), lamp, e->q,
-                               c.q[Q_TD_S], c.q[Q_TD_M], c.q[Q_TD_L], e->chunk_shift, e->st));
+                               c.q[Q_TD_S], c.q[Q_TD_M], c.q[Q_TD_L], e->chunk_shift,
+                               tdf ? e->flags : nullptr, e->st));
       if (stats) stats->n_td_launches++;
     } else if (dir == 2) {
       HIP_TRY(launch_expand_hy(W, g->d_beg, g->d_adj, g->d_tadj, slot(e, L - 1), e->hs,
@@ -330,7 +332,7 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
     }
     HIP_TRY(launch_inspect(W, inspect_args(e, slot(e, L - 1), slot(e, L), slot(e, L + 1), lamp,
                                            &e->d_ctl[L % e->R], &e->d_ctl[(L + 1) % e->R],
-                                           dir != 1 ? 1 : 0),
+                                           (dir == 1 || tdf) ? 0 : 1),
                            e->st));
     if (timing) { y1 = next_event(e); HIP_TRY(hipEventRecord(y1, e->st)); }
     HIP_TRY(hipMemcpyAsync(e->h_ctl, &e->d_ctl[L % e->R], sizeof(LevelCtl),

@@ -50,6 +50,7 @@ struct Plan {
   int rho;              // hub entries per adjacency entry of the graph (1/1024 units)
   int split_cov;        // split levels: % of the top hubs a lane must hold to stay
                         // bottom-up (0: no split levels)
+  int td_flag;          // top-down level `level` flags what it changes (flag_top_down)
   u64 nf_prev;          // frontier size of the level before `level`
   u64 V;
   int dirlog[256];      // direction per level, indexed level & 255
@@ -112,6 +113,12 @@ __host__ __device__ inline int choose_direction(int policy, int sw_level, int al
   }
   return dir;
 }
+
+// A sparse top-down level (pushed edges m_f under V/8) flags every vertex
+// whose word it changes, one byte store per changed vertex, so that the
+// inspection after it diffs only those (a dense one would pay a partial-line
+// write per flagged byte, and its inspection diffs every vertex instead).
+__host__ __device__ inline bool flag_top_down(u64 mf, u64 V) { return mf * 8 < V; }
 
 // Output slot of one lane for the depth emission.
 struct LaneOut {
@@ -180,9 +187,10 @@ hipError_t launch_init_sources(int W, const int32_t *src, int k, int lane0, u64 
 hipError_t launch_admit(int W, const int32_t *src, const int32_t *lane, int n, u64 *S,
                         uint8_t *flags, hipStream_t st);
 hipError_t launch_inspect(int W, const InspectArgs &a, hipStream_t st);
+// flags (or null): set for every vertex whose word changes (flag_top_down)
 hipError_t launch_expand_td(int W, const int64_t *beg, const int32_t *adj, const u64 *prev,
                             u64 *cur, const u64 *lam_prev, Queues q, int64_t ns, int64_t nm,
-                            int64_t nl, int chunk_shift, hipStream_t st);
+                            int64_t nl, int chunk_shift, uint8_t *flags, hipStream_t st);
 // tadj: traversal adjacency (neighbours by descending degree, hub entries
 // encoded as 0x80000000 | hub slot); hs: hub cache mirroring prev.
 hipError_t launch_expand_bu(int W, const int64_t *beg, const int32_t *tadj, const u64 *prev,

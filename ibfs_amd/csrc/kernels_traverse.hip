@@ -64,6 +64,7 @@ __device__ void plan_decide(Plan *p, const LevelCtl *c, int L) {
                                    p->V, Ln, dir_prev, p->nf_prev, nf, mf, mc, mfn);
   p->nf_prev = nf;
   p->dir = dir;
+  p->td_flag = dir == 0 && flag_top_down(mf, p->V);
   p->dirlog[Ln & 255] = dir;
   p->level = Ln;
 }
@@ -72,9 +73,10 @@ __device__ void plan_decide(Plan *p, const LevelCtl *c, int L) {
 // Inspector of level L: one pass over the traversal vertices.  Copies S_L
 // (cur) into S_{L+1} (next, the slot the expanders of L+1 update), refreshes
 // the hub cache from S_L, diffs S_L against S_{L-1} and accumulates the lanes
-// that advanced (lam of L).  After a bottom-up level only the vertices the
-// expander (or the source admission) flagged are diffed (prev is read only
-// for them); after a top-down level and at level 0 every vertex is.  Flags
+// that advanced (lam of L).  After a bottom-up or a sparse top-down level only
+// the vertices the expander (or the source admission) flagged are diffed
+// (prev is read only for them); after other levels and at level 0 every
+// vertex is.  Flags
 // are cleared as they are read.  It compacts, per degree class, both the top-down
 // frontier of L+1 (changed, degree > 0) and the bottom-up candidates
 // (degree > 0 and missing a lane of lam_prev).  Replaces
@@ -110,7 +112,7 @@ __global__ __launch_bounds__(NW * 64) void k_inspect(InspectArgs a) {
   if (a.plan) {
     if (a.plan->done) return;
     const int L = a.plan->level;
-    full = (L == 0 || a.plan->dir != 1);
+    full = (L == 0 || !(a.plan->dir == 1 || (a.plan->dir == 0 && a.plan->td_flag)));
     const int64_t slot = a.V * W;
     prev = L > 0 ? a.ring + (int64_t)((L - 1) % a.R) * slot : nullptr;
     cur = a.ring + (int64_t)(L % a.R) * slot;
@@ -362,9 +364,10 @@ __global__ __launch_bounds__(512) void k_plan_decide(Plan *plan, const LevelCtl 
 // Top-down: a frontier vertex u pushes S_{L-1}[u] & lam_{L-1} into S_L of its
 // neighbours with 64-bit atomic ORs, skipping neighbours that already hold
 // the bits (expander.cu:100-252).
-// Top-down levels flag nothing: their changes are scattered, and a byte store
-// per push costs a partial-line write once the line is evicted; the inspector
-// diffs every vertex after a top-down level instead.
+// A dense top-down level flags nothing: its changes are scattered, and a byte
+// store per changed vertex costs a partial-line write once the line is
+// evicted; the inspector diffs every vertex after it instead.  A sparse one
+// (flag_top_down) flags the vertices it changes.
 // NEW (split levels): only the bits u gained at L-1 (S_{L-1} & ~S_{L-2}, pp
 // = S_{L-2} or null) are pushed, and u is skipped when none is in lam.
 template <int W, bool NEW = false>
@@ -391,16 +394,22 @@ __device__ __forceinline__ void td_push(const int64_t *__restrict__ beg,
                                         const int32_t *__restrict__ adj,
                                         const u64 *__restrict__ prev, u64 *cur,
                                         const u64 (&lam)[W], int32_t u, int64_t j0, int64_t j1,
-                                        int stride, const u64 *pp = nullptr) {
+                                        int stride, const u64 *pp = nullptr,
+                                        uint8_t *flags = nullptr) {
   u64 p[W];
   if (!td_bits<W, NEW>(prev, pp, lam, u, p)) return;
   for (int64_t j = j0; j < j1; j += stride) {
     const int64_t v = adj[j];
     u64 c[W];
     load_word<W>(cur + v * W, c);
+    bool ch = false;
 #pragma unroll
     for (int w = 0; w < W; ++w)
-      if (p[w] & ~c[w]) atomicOr(&cur[v * W + w], p[w]);
+      if (p[w] & ~c[w]) {
+        atomicOr(&cur[v * W + w], p[w]);
+        ch = true;
+      }
+    if (flags && ch) flags[v] = 1;
   }
 }
 
@@ -412,7 +421,7 @@ __device__ __forceinline__ void td_push_small(const int64_t *__restrict__ beg,
                                               const int32_t *__restrict__ adj,
                                               const u64 *__restrict__ prev, u64 *cur,
                                               const u64 (&lam)[W], int32_t u,
-                                              const u64 *pp = nullptr) {
+                                              const u64 *pp = nullptr, uint8_t *flags = nullptr) {
   u64 p[W];
   if (!td_bits<W, NEW>(prev, pp, lam, u, p)) return;
   int64_t j = beg[u];
@@ -425,18 +434,29 @@ __device__ __forceinline__ void td_push_small(const int64_t *__restrict__ beg,
 #pragma unroll
     for (int r = 0; r < 4; ++r) load_word<W>(cur + v[r] * W, c[r]);
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int r = 0; r < 4; ++r) {
+      bool ch = false;
 #pragma unroll
       for (int w = 0; w < W; ++w)
-        if (p[w] & ~c[r][w]) atomicOr(&cur[v[r] * W + w], p[w]);
+        if (p[w] & ~c[r][w]) {
+          atomicOr(&cur[v[r] * W + w], p[w]);
+          ch = true;
+        }
+      if (flags && ch) flags[v[r]] = 1;
+    }
   }
   for (; j < e; ++j) {
     const int64_t v = adj[j];
     u64 c[W];
     load_word<W>(cur + v * W, c);
+    bool ch = false;
 #pragma unroll
     for (int w = 0; w < W; ++w)
-      if (p[w] & ~c[w]) atomicOr(&cur[v * W + w], p[w]);
+      if (p[w] & ~c[w]) {
+        atomicOr(&cur[v * W + w], p[w]);
+        ch = true;
+      }
+    if (flags && ch) flags[v] = 1;
   }
 }
 
@@ -624,7 +644,7 @@ __global__ __launch_bounds__(BS) void k_expand_td(const int64_t *__restrict__ be
                                                   const u64 *__restrict__ prev, u64 *cur,
                                                   const u64 *__restrict__ lam_prev, Queues q,
                                                   int64_t ns, int64_t nm, int64_t nl, int64_t nbs,
-                                                  int64_t nbm, int cs) {
+                                                  int64_t nbm, int cs, uint8_t *flags) {
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int64_t b = blockIdx.x;
   u64 lam[W];
@@ -633,19 +653,19 @@ __global__ __launch_bounds__(BS) void k_expand_td(const int64_t *__restrict__ be
   if (b < nbs) {
     const int64_t i = b * BS + t;
     if (i >= ns) return;
-    td_push_small<W>(beg, adj, prev, cur, lam, q.q[Q_TD_S][i]);
+    td_push_small<W>(beg, adj, prev, cur, lam, q.q[Q_TD_S][i], nullptr, flags);
   } else if (b < nbs + nbm) {
     const int64_t i = (b - nbs) * 4 + wv;
     if (i >= nm) return;
     const int32_t u = q.q[Q_TD_M][i];
-    td_push<W>(beg, adj, prev, cur, lam, u, beg[u] + lane, beg[u + 1], 64);
+    td_push<W>(beg, adj, prev, cur, lam, u, beg[u] + lane, beg[u + 1], 64, nullptr, flags);
   } else {
     const int64_t i = b - nbs - nbm;
     if (i >= nl) return;
     const int2 e = q.chunks[i];
     int64_t lo, hi;
     td_chunk_range(beg, e, cs, lo, hi);
-    td_push<W>(beg, adj, prev, cur, lam, e.x, lo + t, hi, BS);
+    td_push<W>(beg, adj, prev, cur, lam, e.x, lo + t, hi, BS, nullptr, flags);
   }
 }
 
@@ -758,18 +778,19 @@ __global__ __launch_bounds__(BS) void k_expand_dd(const int64_t *__restrict__ be
 #pragma unroll
   for (int w = 0; w < W; ++w) lam[w] = c->lam[w];
   if (dir == 0) {
+    uint8_t *tf = plan->td_flag ? flags : nullptr;
     const int64_t ns = c->q[Q_TD_S], nm = c->q[Q_TD_M], nl = c->q[Q_TD_L];
     for (int64_t i = (int64_t)blockIdx.x * BS + t; i < ns; i += G * BS)
-      td_push_small<W>(beg, adj, prev, cur, lam, q.q[Q_TD_S][i]);
+      td_push_small<W>(beg, adj, prev, cur, lam, q.q[Q_TD_S][i], nullptr, tf);
     for (int64_t i = (int64_t)blockIdx.x * 4 + wv; i < nm; i += G * 4) {
       const int32_t u = q.q[Q_TD_M][i];
-      td_push<W>(beg, adj, prev, cur, lam, u, beg[u] + lane, beg[u + 1], 64);
+      td_push<W>(beg, adj, prev, cur, lam, u, beg[u] + lane, beg[u + 1], 64, nullptr, tf);
     }
     for (int64_t i = blockIdx.x; i < nl; i += G) {
       const int2 e = q.chunks[i];
       int64_t lo, hi;
       td_chunk_range(beg, e, cs, lo, hi);
-      td_push<W>(beg, adj, prev, cur, lam, e.x, lo + t, hi, BS);
+      td_push<W>(beg, adj, prev, cur, lam, e.x, lo + t, hi, BS, nullptr, tf);
     }
   } else if (dir == 2) {
     const Gather gs{prev, hs};
@@ -964,14 +985,14 @@ hipError_t launch_inspect(int W, const InspectArgs &a, hipStream_t st) {
 
 hipError_t launch_expand_td(int W, const int64_t *beg, const int32_t *adj, const u64 *prev,
                             u64 *cur, const u64 *lam_prev, Queues q, int64_t ns, int64_t nm,
-                            int64_t nl, int chunk_shift, hipStream_t st) {
+                            int64_t nl, int chunk_shift, uint8_t *flags, hipStream_t st) {
   const int64_t nbs = (ns + BS - 1) / BS, nbm = (nm + 3) / 4;
   const int64_t nb = nbs + nbm + nl;
   if (nb == 0) return hipSuccess;
   if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
   dim3 g((unsigned)nb), blk(BS);
   IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_expand_td<WW>, g, blk, 0, st, beg, adj, prev, cur,
-                                        lam_prev, q, ns, nm, nl, nbs, nbm, chunk_shift));
+                                        lam_prev, q, ns, nm, nl, nbs, nbm, chunk_shift, flags));
   return hipGetLastError();
 }
 

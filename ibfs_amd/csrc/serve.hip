@@ -476,7 +476,8 @@ int process_level(ibfs_server_s *s, SCtx &c) {
   const u64 *lamp = reinterpret_cast<const u64 *>(c.ctl + L % s->R);
   if (dir == 0)
     HIP_CHK(launch_expand_td(W, s->g->d_beg, s->g->d_adj, slot(s, c, L), slot(s, c, Ln), lamp,
-                             c.q, lc.q[Q_TD_S], lc.q[Q_TD_M], lc.q[Q_TD_L], s->chunk_shift, c.st));
+                             c.q, lc.q[Q_TD_S], lc.q[Q_TD_M], lc.q[Q_TD_L], s->chunk_shift,
+                             nullptr, c.st));
   else if (dir == 2)
     HIP_CHK(launch_expand_hy(W, s->g->d_beg, s->g->d_adj, s->g->d_tadj, slot(s, c, L), c.hs,
                              slot(s, c, Ln), c.ctl + L % s->R, c.q, s->chunk_shift,
