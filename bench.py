@@ -302,7 +302,7 @@ def main() -> int:
     if args.inspect_vpt is not None:
         tune["inspect_vpt"] = args.inspect_vpt
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
+        print(f"bench.py measures one GPU: libibfs has no multi-GPU path (asked for "
               f"--gpus {args.gpus}); source groups would shard across GPUs with no exchange.",
               file=sys.stderr)
         return 2
