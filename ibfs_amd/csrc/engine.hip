@@ -280,6 +280,37 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
     record(0, -1, c0, 0.f, insp0);
   }
 
+  // A level's event timings and log entry are settled after the next level's
+  // kernels are enqueued, so the host does that bookkeeping while they run
+  // instead of between the control-block wait and the next launch.
+  struct Pending {
+    bool on = false;
+    int L = 0, dir = 0;
+    hipEvent_t x0 = nullptr, x1 = nullptr, y0 = nullptr, y1 = nullptr;
+    LevelCtl c{};
+    int64_t bu_work = 0, bu_prefix = 0;
+  } pend;
+  auto settle = [&]() -> int {
+    if (!pend.on) return IBFS_OK;
+    pend.on = false;
+    float ems = 0, ims = 0;
+    if (timing) {
+      HIP_TRY(hipEventElapsedTime(&ems, pend.x0, pend.x1));
+      HIP_TRY(hipEventElapsedTime(&ims, pend.y0, pend.y1));
+      if (stats) {
+        (pend.dir == 0 ? stats->expand_td_ms
+                       : pend.dir == 2 ? stats->expand_hy_ms : stats->expand_bu_ms) += ems;
+        stats->inspect_ms += ims;
+      }
+    }
+    record(pend.L, pend.dir, pend.c, ems, ims);
+    if (e->cfg.work_model && pend.dir == 1) {
+      e->log.back().bu_work = pend.bu_work;
+      e->log.back().bu_prefix = pend.bu_prefix;
+    }
+    return IBFS_OK;
+  };
+
   int base = 0, last = 0, dir_prev = -1;
   u64 nf_prev = ~0ull;
   for (int L = 1;; ++L) {
@@ -337,28 +368,32 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
     if (timing) { y1 = next_event(e); HIP_TRY(hipEventRecord(y1, e->st)); }
     HIP_TRY(hipMemcpyAsync(e->h_ctl, &e->d_ctl[L % e->R], sizeof(LevelCtl),
                            hipMemcpyDeviceToHost, e->st));
+    {
+      const int rc = settle();  // level L-1, while level L runs
+      if (rc) return rc;
+    }
     HIP_TRY(poll_stream(e->st, e->ev_wait));
-    float ems = 0, ims = 0;
-    if (timing) {
-      HIP_TRY(hipEventElapsedTime(&ems, x0, x1));
-      HIP_TRY(hipEventElapsedTime(&ims, y0, y1));
-      if (stats) {
-        (dir == 0 ? stats->expand_td_ms : dir == 2 ? stats->expand_hy_ms : stats->expand_bu_ms) += ems;
-        stats->inspect_ms += ims;
-      }
-    }
     if (stats) stats->n_inspect_launches++;
-    record(L, dir, *e->h_ctl, ems, ims);
-    if (e->cfg.work_model && dir == 1) {
-      e->log.back().bu_work = (int64_t)e->h_count[1];
-      e->log.back().bu_prefix = (int64_t)e->h_count[2];
-    }
+    pend.on = true;
+    pend.L = L;
+    pend.dir = dir;
+    pend.x0 = x0;
+    pend.x1 = x1;
+    pend.y0 = y0;
+    pend.y1 = y1;
+    pend.c = *e->h_ctl;
+    pend.bu_work = (int64_t)e->h_count[1];
+    pend.bu_prefix = (int64_t)e->h_count[2];
     dir_prev = dir;
     nf_prev = c.n_front;
-    if (e->h_ctl->n_changed == 0) {
+    if (pend.c.n_changed == 0) {
       last = L - 1;
       break;
     }
+  }
+  {
+    const int rc = settle();
+    if (rc) return rc;
   }
   // the final window also counts the TEPS numerator (over S_last, which its
   // pass reads anyway)
