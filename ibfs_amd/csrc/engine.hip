@@ -34,6 +34,8 @@ struct ibfs_engine_s {
   u64 *ring = nullptr;
   LevelCtl *d_ctl = nullptr;
   LevelCtl *h_ctl = nullptr;  // pinned
+  CtlPost *post = nullptr;    // coherent pinned: control blocks posted by k_post_ctl
+  u64 post_seq = 0;
   Queues q{};
   u64 *d_masks = nullptr;     // [0,W) lanes_all, [W,2W) sel, [2W, 2W+(R+1)W) level masks
   u64 *h_masks = nullptr;     // pinned mirror
@@ -70,6 +72,43 @@ hipError_t poll_stream(hipStream_t st, hipEvent_t ev) {
     const hipError_t q = hipEventQuery(ev);
     if (q != hipErrorNotReady) return q;
   }
+}
+
+// Waits for the control block posted with tag seq (a spin on the tag; every
+// 1,024 reads the stream is queried, so an error, or an idle stream without
+// the tag, ends the wait with an error), then copies it to e->h_ctl.
+int wait_post(ibfs_engine_s *e, u64 seq) {
+  for (unsigned i = 1;; ++i) {
+    if (__atomic_load_n(&e->post->tag, __ATOMIC_ACQUIRE) == seq) break;
+    if ((i & 1023) == 0) {
+      const hipError_t q = hipStreamQuery(e->st);
+      if (q == hipErrorNotReady) continue;
+      if (q != hipSuccess) return fail(IBFS_ERR_HIP, "stream error: %s", hipGetErrorString(q));
+      if (__atomic_load_n(&e->post->tag, __ATOMIC_ACQUIRE) == seq) break;
+      return fail(IBFS_ERR_STATE, "control block post did not arrive");
+    }
+  }
+  std::memcpy(e->h_ctl, &e->post->c, sizeof(LevelCtl));
+  return IBFS_OK;
+}
+
+// Elapsed time between two events the device has passed.  A posted control
+// block can reach the host before the runtime has seen the completion of the
+// events enqueued ahead of it, so each is queried until it reports completion.
+hipError_t elapsed_ms(float *ms, hipEvent_t a, hipEvent_t b) {
+  for (hipEvent_t x : {a, b}) {
+    hipError_t q;
+    while ((q = hipEventQuery(x)) == hipErrorNotReady) {
+    }
+    if (q != hipSuccess) return q;
+  }
+  return hipEventElapsedTime(ms, a, b);
+}
+
+int post_and_wait(ibfs_engine_s *e, const LevelCtl *d) {
+  const u64 seq = ++e->post_seq;
+  HIP_TRY(launch_post_ctl(d, e->post, seq, e->st));
+  return wait_post(e, seq);
 }
 
 hipEvent_t next_event(ibfs_engine_s *e) {
@@ -109,6 +148,7 @@ void engine_release(ibfs_engine_s *e) {
   dfree(e->ring);
   dfree(e->d_ctl);
   hfree(e->h_ctl);
+  hfree(e->post);
   for (int i = 0; i < 6; ++i) dfree(e->q.q[i]);
   dfree(e->q.chunks);
   dfree(e->d_masks);
@@ -248,8 +288,10 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
                                          &e->d_ctl[0], &e->d_ctl[1 % e->R], 1),
                          e->st));
   if (timing) { ti1 = next_event(e); HIP_TRY(hipEventRecord(ti1, e->st)); }
-  HIP_TRY(hipMemcpyAsync(e->h_ctl, &e->d_ctl[0], sizeof(LevelCtl), hipMemcpyDeviceToHost, e->st));
-  HIP_TRY(poll_stream(e->st, e->ev_wait));
+  {
+    const int rc = post_and_wait(e, &e->d_ctl[0]);
+    if (rc) return rc;
+  }
   auto record = [&](int L, int dir, const LevelCtl &c, float exp_ms, float insp_ms) {
     ibfs_level_info li{};
     li.level = L;
@@ -270,7 +312,7 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
   };
   float insp0 = 0;
   if (timing) {
-    HIP_TRY(hipEventElapsedTime(&insp0, ti0, ti1));
+    HIP_TRY(elapsed_ms(&insp0, ti0, ti1));
     if (stats) stats->inspect_ms += insp0;
   }
   if (stats) stats->n_inspect_launches++;
@@ -295,8 +337,8 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
     pend.on = false;
     float ems = 0, ims = 0;
     if (timing) {
-      HIP_TRY(hipEventElapsedTime(&ems, pend.x0, pend.x1));
-      HIP_TRY(hipEventElapsedTime(&ims, pend.y0, pend.y1));
+      HIP_TRY(elapsed_ms(&ems, pend.x0, pend.x1));
+      HIP_TRY(elapsed_ms(&ims, pend.y0, pend.y1));
       if (stats) {
         (pend.dir == 0 ? stats->expand_td_ms
                        : pend.dir == 2 ? stats->expand_hy_ms : stats->expand_bu_ms) += ems;
@@ -366,13 +408,16 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
                                            (dir == 1 || tdf) ? 0 : 1),
                            e->st));
     if (timing) { y1 = next_event(e); HIP_TRY(hipEventRecord(y1, e->st)); }
-    HIP_TRY(hipMemcpyAsync(e->h_ctl, &e->d_ctl[L % e->R], sizeof(LevelCtl),
-                           hipMemcpyDeviceToHost, e->st));
+    const u64 seq = ++e->post_seq;
+    HIP_TRY(launch_post_ctl(&e->d_ctl[L % e->R], e->post, seq, e->st));
     {
       const int rc = settle();  // level L-1, while level L runs
       if (rc) return rc;
     }
-    HIP_TRY(poll_stream(e->st, e->ev_wait));
+    {
+      const int rc = wait_post(e, seq);
+      if (rc) return rc;
+    }
     if (stats) stats->n_inspect_launches++;
     pend.on = true;
     pend.L = L;
@@ -418,10 +463,10 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
   HIP_TRY(poll_stream(e->st, e->ev_wait));
   if (stats) {
     float ms = 0;
-    HIP_TRY(hipEventElapsedTime(&ms, t_begin, t_end));
+    HIP_TRY(elapsed_ms(&ms, t_begin, t_end));
     stats->total_ms += ms;
     for (const auto &x : e->emit_ev) {
-      HIP_TRY(hipEventElapsedTime(&ms, x.first, x.second));
+      HIP_TRY(elapsed_ms(&ms, x.first, x.second));
       stats->emit_ms += ms;
     }
     stats->levels = std::max(stats->levels, last + 1);
@@ -809,6 +854,8 @@ int ibfs_engine_create(ibfs_graph_t g, const ibfs_config *cfg_in, ibfs_engine_t 
   A((void **)&e->ring, (size_t)R * slot_bytes);
   A((void **)&e->d_ctl, sizeof(LevelCtl) * (size_t)R);
   H((void **)&e->h_ctl, sizeof(LevelCtl));
+  if (err == hipSuccess) err = hipHostMalloc((void **)&e->post, sizeof(CtlPost), hipHostMallocCoherent);
+  if (err == hipSuccess) std::memset(e->post, 0, sizeof(CtlPost));
   for (int i = 0; i < 6; ++i)
     if (i != Q_TD_L) A((void **)&e->q.q[i], sizeof(int32_t) * (size_t)Vt);
   const size_t nchunks = (size_t)Vt + (size_t)(g->E >> e->chunk_shift) + 2;

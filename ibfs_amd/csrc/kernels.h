@@ -179,6 +179,14 @@ struct InspectArgs {
 
 // Launchers (all asynchronous on `st`).  W = status words (u64) per vertex.
 hipError_t launch_fill_u64(u64 *p, int64_t n, u64 value, hipStream_t st);
+// A control block posted to coherent pinned host memory: the block's words,
+// then `tag` with a system-scope release, so a host that reads the tag with
+// an acquire load sees the block without a copy command or an event.
+struct CtlPost {
+  LevelCtl c;
+  u64 tag;
+};
+hipError_t launch_post_ctl(const LevelCtl *src, CtlPost *dst, u64 tag, hipStream_t st);
 hipError_t launch_fill_i32(int32_t *p, int64_t n, int32_t value, hipStream_t st);
 // S0[src[i]] |= bit lane0+i, i < k (src in traversal ids; -1 entries skipped).
 hipError_t launch_init_sources(int W, const int32_t *src, int k, int lane0, u64 *S0,

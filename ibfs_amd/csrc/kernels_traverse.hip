@@ -15,6 +15,18 @@ __global__ void k_fill_u64(u64 *p, int64_t n, u64 value) {
     p[i] = value;
 }
 
+// One wave: thread t < CTL_WORDS copies word t of the block (vector stores),
+// makes it visible system-wide, and after the barrier thread 0 releases the tag.
+__global__ void k_post_ctl(const LevelCtl *src, CtlPost *dst, u64 tag) {
+  const int t = threadIdx.x;
+  if (t < (int)CTL_WORDS) {
+    reinterpret_cast<u64 *>(&dst->c)[t] = atomic_read(reinterpret_cast<const u64 *>(src) + t);
+    __threadfence_system();
+  }
+  __syncthreads();
+  if (t == 0) __hip_atomic_store(&dst->tag, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t value) {
   for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
     p[i] = value;
@@ -1027,6 +1039,12 @@ hipError_t launch_expand_hy(int W, const int64_t *beg, const int32_t *adj, const
   const Gather gs{prev, hs};
   IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_expand_hy<WW>, g, blk, 0, st, beg, adj, tadj, gs, cur,
                                         ctl_prev, q, chunk_shift, hsplit));
+  return hipGetLastError();
+}
+
+hipError_t launch_post_ctl(const LevelCtl *src, CtlPost *dst, u64 tag, hipStream_t st) {
+  static_assert(CTL_WORDS <= 64, "one wave posts the control block");
+  hipLaunchKernelGGL(k_post_ctl, dim3(1), dim3(64), 0, st, src, dst, tag);
   return hipGetLastError();
 }
 
