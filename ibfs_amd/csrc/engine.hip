@@ -60,11 +60,11 @@ namespace {
 
 size_t masks_words(const ibfs_engine_s *e) { return (size_t)(2 + (e->R + 1)) * e->W; }
 
-// Waits for the work enqueued so far by polling an event recorded behind it.
-// The host-driven level loop waits on the control-block copy after every
-// inspection; a blocking wait's wake-up costs tens of microseconds per level,
-// more than the sparse levels' kernels, and an event query reads one
-// completion signal where a stream query takes the stream's lock.
+// Waits for the work enqueued so far by polling an event recorded behind it
+// (the end of a group: its edge count and timing events).  A blocking wait's
+// wake-up costs tens of microseconds, and an event query reads one completion
+// signal where a stream query takes the stream's lock.  The level loop waits
+// on posted control blocks instead (wait_post).
 hipError_t poll_stream(hipStream_t st, hipEvent_t ev) {
   const hipError_t r = hipEventRecord(ev, st);
   if (r != hipSuccess) return r;
