@@ -627,3 +627,38 @@ def test_device_plan_chooses_the_host_directions(s16, opts):
     assert (st.n_td_launches, st.n_bu_launches, st.n_hy_launches, st.n_split_levels) == \
         (dirs.count(0), dirs.count(1), dirs.count(2), 0)
     b.close()
+
+
+def test_engines_on_threads_share_a_graph(s16):
+    """One engine per host thread on a shared graph (the supported threading
+    model): three engines of 64, 128 and 256 lanes with kernel timing run
+    their groups concurrently, twice each; every depth array equals the
+    oracle's and each engine's traversed edges agree across its runs."""
+    import threading
+    say("[gpu] engines on threads")
+    pool = s16["h"].gen_sources(900, seed=51)
+    jobs = [(64, pool[:150]), (128, pool[150:450]), (256, pool[450:900])]
+    want = [oracle.bfs_many(s16["beg"], s16["adj"], src, threads=8) for _, src in jobs]
+    engines = [ibfs_amd.Engine(s16["g"], lane_bits=lb, kernel_timing=1) for lb, _ in jobs]
+    got = [[None, None] for _ in jobs]
+    edges = [[0, 0] for _ in jobs]
+    errors = []
+
+    def work(i):
+        try:
+            for rep in range(2):
+                got[i][rep] = engines[i].run_groups(jobs[i][1])
+                edges[i][rep] = engines[i].last_stats.traversed_edges
+        except Exception as exc:  # reported on the main thread
+            errors.append(exc)
+
+    threads = [threading.Thread(target=work, args=(i,)) for i in range(len(jobs))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not errors, errors
+    for i in range(len(jobs)):
+        for rep in range(2):
+            np.testing.assert_array_equal(got[i][rep], want[i], err_msg=f"engine {i} run {rep}")
+        assert edges[i][0] == edges[i][1] > 0
