@@ -81,31 +81,44 @@ def alg_bytes_per_kernel(log_levels, V, Vt, W, k, hub_frac=0.0):
 
 
 def plain_leg(g, src, K, steps, warmup, tune, emit_mode, out):
-    eng = ibfs_amd.Engine(g, lane_bits=K, kernel_timing=1, emit_mode=emit_mode, **tune)
-    for i in range(warmup):
-        eng.run_groups(src, out=out)
-    kern = {"expand_td": 0.0, "expand_bu": 0.0, "expand_hy": 0.0, "inspect": 0.0, "emit": 0.0}
-    launches = dict.fromkeys(kern, 0)
-    edges = 0
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        eng.run_groups(src, out=out)
-        st = eng.last_stats
-        edges += st.traversed_edges
-        kern["expand_td"] += st.expand_td_ms
-        kern["expand_bu"] += st.expand_bu_ms
-        kern["expand_hy"] += st.expand_hy_ms
-        kern["inspect"] += st.inspect_ms
-        kern["emit"] += st.emit_ms
-        launches["expand_td"] += st.n_td_launches
-        launches["expand_bu"] += st.n_bu_launches
-        launches["expand_hy"] += st.n_hy_launches
-        launches["inspect"] += st.n_inspect_launches
-        launches["emit"] += st.n_emit_launches
-    wall = time.perf_counter() - t0
-    return dict(gteps=edges / wall / 1e9, ms_per_step=wall * 1e3 / steps, edges_per_step=edges // steps,
-                kern_ms={k: v / steps for k, v in kern.items()},
-                launches={k: v // steps for k, v in launches.items()}, cfg=eng.cfg)
+    """Two timed regions of `steps` steps each on the same engine settings:
+    the first without kernel events (the headline rate), the second with a
+    HIP event pair around every kernel on the engine's stream (per-kernel
+    durations for the roofline; the events add gaps between kernels)."""
+    res = {}
+    for timing in (0, 1):
+        eng = ibfs_amd.Engine(g, lane_bits=K, kernel_timing=timing, emit_mode=emit_mode, **tune)
+        for i in range(warmup):
+            eng.run_groups(src, out=out)
+        kern = {"expand_td": 0.0, "expand_bu": 0.0, "expand_hy": 0.0, "inspect": 0.0, "emit": 0.0}
+        launches = dict.fromkeys(kern, 0)
+        edges = 0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            eng.run_groups(src, out=out)
+            st = eng.last_stats
+            edges += st.traversed_edges
+            kern["expand_td"] += st.expand_td_ms
+            kern["expand_bu"] += st.expand_bu_ms
+            kern["expand_hy"] += st.expand_hy_ms
+            kern["inspect"] += st.inspect_ms
+            kern["emit"] += st.emit_ms
+            launches["expand_td"] += st.n_td_launches
+            launches["expand_bu"] += st.n_bu_launches
+            launches["expand_hy"] += st.n_hy_launches
+            launches["inspect"] += st.n_inspect_launches
+            launches["emit"] += st.n_emit_launches
+        wall = time.perf_counter() - t0
+        res[timing] = dict(gteps=edges / wall / 1e9, ms_per_step=wall * 1e3 / steps,
+                           edges_per_step=edges // steps,
+                           kern_ms={k: v / steps for k, v in kern.items()},
+                           launches={k: v // steps for k, v in launches.items()}, cfg=eng.cfg)
+        eng.close()
+    out0 = res[0]
+    out0["kern_ms"] = res[1]["kern_ms"]
+    out0["launches"] = res[1]["launches"]
+    out0["events_ms_per_step"] = res[1]["ms_per_step"]
+    return out0
 
 
 def roofline_leg(g, src, K, V, tune, kern_ms, launches, steps):
@@ -155,15 +168,32 @@ def measured_traffic(workload, K, kernel):
     return None, None
 
 
-# batched-leg settings found by ibfs_autotune_batch on this workload (R-MAT s22,
-# the 1,024 GroupBy-ordered sources): 256 lanes, 2 contexts, bursts of 4,
-# lrg_shed 4096, beta 96, top-down hub chunks of 8,192 edges
-BATCH_TUNED = {"lrg_shed": 4096, "beta": 96, "td_chunk": 8192}
+# Batched-leg settings: the "best" record of the batched autotuner's report
+# for this workload (ibfs_amd.autotune_batch on the R-MAT s22 graph with the
+# 1,024 GroupBy-ordered sources, saved with save_to=); the library defaults
+# with 256 lanes, 2 contexts and bursts of 4 when no report is present.
+TUNED_DIR = ROOT / "ibfs_amd" / "tuned"
 
 
-def batch_leg(g, src, K, contexts, burst, steps, warmup, tune, emit_mode, out):
-    opts = dict(BATCH_TUNED, **tune)
-    b = ibfs_amd.Batch(g, contexts=contexts, burst=burst, lane_bits=K, emit_mode=emit_mode, **opts)
+def batch_settings(workload):
+    path = TUNED_DIR / f"batch_rmat_{workload}.json"
+    if not path.exists():
+        return {"lane_bits": 256}, 2, 4, None
+    rep = json.loads(path.read_text())
+    best = dict(rep["best"])
+    contexts = int(best.pop("contexts", 2))
+    burst = int(best.pop("burst", 4))
+    return best, contexts, burst, path.name
+
+
+def batch_leg(g, src, steps, warmup, tune, emit_mode, out, workload, contexts=None, burst=None):
+    opts, c0, b0, source = batch_settings(workload)
+    opts = dict(opts, **tune)
+    if emit_mode is not None:
+        opts["emit_mode"] = emit_mode
+    contexts = contexts or c0
+    burst = burst or b0
+    b = ibfs_amd.Batch(g, contexts=contexts, burst=burst, **opts)
     for _ in range(warmup):
         b.run(src, out=out)
     edges = 0
@@ -172,8 +202,13 @@ def batch_leg(g, src, K, contexts, burst, steps, warmup, tune, emit_mode, out):
         b.run(src, out=out)
         edges += b.last_stats.traversed_edges
     wall = time.perf_counter() - t0
-    return dict(gteps=edges / wall / 1e9, ms_per_step=wall * 1e3 / steps, lane_bits=K,
-                contexts=contexts, burst=burst, config=opts)
+    st = b.last_stats
+    b.close()
+    return dict(gteps=edges / wall / 1e9, ms_per_step=wall * 1e3 / steps,
+                lane_bits=opts.get("lane_bits", 256), contexts=contexts, burst=burst,
+                config=opts, tuned_from=source,
+                levels_per_direction={"td": st.n_td_launches, "bu": st.n_bu_launches,
+                                      "hybrid": st.n_hy_launches})
 
 
 def edges_of(g, sources, chunk=256):
@@ -184,16 +219,22 @@ def edges_of(g, sources, chunk=256):
     for i in range(0, len(sources), chunk):
         b.run(sources[i:i + chunk], out=buf)
         total += b.last_stats.traversed_edges
+    b.close()
+    buf.free()
     return total
 
 
-def served_leg(graphs, rates, n_queries, seed, K, admit_window=None):
+def served_leg(graphs, rates, n_queries, seed, K, admit_window=None, check=True, threads=16):
     """Poisson arrivals at each rate; every query names one graph and carries
-    a log-uniform 1..256 source set.  Latency = done - submit."""
+    a log-uniform 1..256 source set.  Outputs come from a device row pool per
+    graph, recycled on release (one allocation before the arrivals).  Latency
+    = completion - scheduled arrival, so any wait of the arrival thread for
+    pool space counts.  Per rate, 2 rows of each of the first 3 queries on
+    each graph are copied out at completion and compared with the oracle."""
     rng = np.random.default_rng(seed)
     servers = [ibfs_amd.Server(gr["g"], lane_bits=K, status_bytes=gr["status_bytes"],
                                admit_window=admit_window) for gr in graphs]
-    checkers = [ibfs_amd.Batch(gr["g"], contexts=1, burst=8, lane_bits=256) for gr in graphs]
+    pools = [ibfs_amd.RowPool(gr["g"].vert_count, gr["pool_rows"]) for gr in graphs]
     results = []
     for lam in rates:
         gi = rng.integers(0, len(graphs), size=n_queries)
@@ -201,71 +242,128 @@ def served_leg(graphs, rates, n_queries, seed, K, admit_window=None):
         qs = [rng.choice(graphs[i]["pool"], size=int(n)).astype(np.int32) for i, n in zip(gi, sizes)]
         gaps = rng.exponential(1.0 / lam, size=n_queries)
         ids = [None] * n_queries
-        # outputs allocated before the arrivals start: no allocation or free
-        # inside the measured window
-        bufs = [ibfs_amd.DeviceBuffer(4 * len(q) * graphs[i]["g"].vert_count) for q, i in zip(qs, gi)]
+        views = [None] * n_queries
+        t_sched = [0.0] * n_queries
         lat = [None] * n_queries
+        samples = {}  # query -> (graph, rows of its sources, host copies)
+        want_sample = {k: 0 for k in range(len(graphs))}
+        origin = time.perf_counter()
 
         def producer():
             t_next = time.perf_counter()
             for i in range(n_queries):
                 t_next += gaps[i]
+                t_sched[i] = t_next
                 d = t_next - time.perf_counter()
                 if d > 0:
                     time.sleep(d)
-                ids[i] = servers[gi[i]].submit(qs[i], bufs[i])
+                views[i] = pools[gi[i]].alloc(len(qs[i]))
+                ids[i] = servers[gi[i]].submit(qs[i], views[i])
 
         def consumer():
             for i in range(n_queries):
                 while ids[i] is None:
                     time.sleep(0.0002)
                 info = servers[gi[i]].wait(ids[i], timeout_ms=120000)
-                lat[i] = info
-                servers[gi[i]].release(ids[i])
+                t_done = time.perf_counter()
+                lat[i] = (info, t_done)
+                k = gi[i]
+                if check and want_sample[k] < 3:
+                    want_sample[k] += 1
+                    V = graphs[k]["g"].vert_count
+                    rows = list(range(min(2, len(qs[i]))))
+                    samples[i] = (k, rows, [views[i].to_host(count=V, offset=r * V) for r in rows])
+                servers[k].release(ids[i])
+                pools[k].free(views[i])
+                views[i] = None
 
+        waits0 = [p.waits for p in pools]
         tp, tc = threading.Thread(target=producer), threading.Thread(target=consumer)
         tp.start()
         tc.start()
         tp.join()
         tc.join()
-        # untimed check: served depth arrays equal the batched form's for the
-        # first queries of the rate
+        waits = sum(p.waits for p in pools) - sum(waits0)
+        lats = np.array([(lat[i][1] - t_sched[i]) * 1e3 for i in range(n_queries)])
+        t_first = min(t_sched) - origin
+        t_last = max(x[1] for x in lat) - origin
+        span_s = t_last - t_first
         verified = 0
-        for i in range(min(3, n_queries)):
-            gr = graphs[gi[i]]
-            want = checkers[gi[i]].run(qs[i])
-            got = bufs[i].to_host(count=len(qs[i]) * gr["g"].vert_count).reshape(want.shape)
-            if not np.array_equal(got, want):
-                raise RuntimeError(f"served query {i} differs from the batched form")
-            verified += 1
-        for b in bufs:
-            b.free()
-        t_first = min(x["submit_ms"] for x in lat)
-        t_last = max(x["done_ms"] for x in lat)
-        lats = np.array([x["latency_ms"] for x in lat])
+        if check:  # untimed: the sampled rows against the oracle
+            import oracle
+            for k, gr in enumerate(graphs):
+                sel = [(i, r) for i, (kk, rows, _) in samples.items() if kk == k for r in rows]
+                if not sel:
+                    continue
+                want = oracle.bfs_many(gr["beg"], gr["adj"],
+                                       np.array([qs[i][r] for i, r in sel], dtype=np.int32),
+                                       threads=threads)
+                for (i, r), w in zip(sel, want):
+                    got = samples[i][2][samples[i][1].index(r)]
+                    if not np.array_equal(got, w):
+                        raise RuntimeError(f"served query {i} row {r} ({gr['name']}) differs "
+                                           f"from the oracle")
+                    verified += 1
         edges = 0
         for k, gr in enumerate(graphs):
             sel = [q for q, i in zip(qs, gi) if i == k]
             if sel:
                 edges += edges_of(gr["g"], np.concatenate(sel))
-        span_s = (t_last - t_first) / 1e3
         results.append({"lambda_qps": lam, "queries": n_queries,
                         "sources": int(sum(len(q) for q in qs)),
                         "p50_ms": round(float(np.percentile(lats, 50)), 3),
                         "p99_ms": round(float(np.percentile(lats, 99)), 3),
                         "mean_ms": round(float(lats.mean()), 3),
                         "gteps": round(edges / span_s / 1e9, 3), "span_ms": round(span_s * 1e3, 2),
-                        "verified_queries": verified})
+                        "pool_waits": waits, "verified_rows": verified})
         log(f"served lambda={lam}/s: p50 {results[-1]['p50_ms']} ms p99 {results[-1]['p99_ms']} ms "
-            f"{results[-1]['gteps']} GTEPS over {results[-1]['span_ms']} ms")
+            f"{results[-1]['gteps']} GTEPS over {results[-1]['span_ms']} ms "
+            f"({verified} rows checked, {waits} pool waits)")
     info = {"graphs": [gr["name"] for gr in graphs], "lane_bits": K,
             "cohort_lanes": (f"narrowest of 64..{K} holding the waiting sources"
                              if not admit_window else K),
             "admit_window": 0 if admit_window is None else admit_window,
-            "contexts": [s.contexts for s in servers], "rates": results}
-    for s in servers:
-        s.close()
+            "contexts": [s.contexts for s in servers],
+            "output_pool_rows": [gr["pool_rows"] for gr in graphs],
+            "latency": "completion - scheduled arrival", "rates": results}
+    for s_ in servers:
+        s_.close()
+    for p_ in pools:
+        p_.close()
     return info
+
+
+def host_cpu_info():
+    """Host cores: those this process may run on, the machine's count, the
+    cgroup CPU quota if one is set, and the lscpu model name."""
+    info = {"affinity_cpus": len(os.sched_getaffinity(0)), "nproc": os.cpu_count()}
+    try:
+        q = Path("/sys/fs/cgroup/cpu.max").read_text().split()
+        if q and q[0] != "max":
+            info["cgroup_cpu_quota"] = round(int(q[0]) / int(q[1]), 2)
+    except (OSError, ValueError, IndexError):
+        pass
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                info["model"] = line.split(":", 1)[1].strip()
+            elif line.startswith("Socket(s):"):
+                info["sockets"] = int(line.split(":", 1)[1])
+    except Exception:
+        pass
+    return info
+
+
+def verify_sample(out, rows, want, V, what):
+    """Rows `rows` of a device depth buffer against the oracle's arrays."""
+    for r, w in zip(rows, want):
+        got = out.to_host(count=V, offset=int(r) * V)
+        if not np.array_equal(got, w):
+            log(f"PARITY FAILURE: {what} row {r} differs from the CPU oracle")
+            return False
+    return True
 
 
 def main() -> int:
@@ -278,18 +376,25 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--emit-mode", type=int, default=0)
+    ap.add_argument("--emit-mode", type=int, default=None)
+    ap.add_argument("--tune", action="store_true",
+                    help="run the batched autotuner on this workload, save its report under "
+                         "ibfs_amd/tuned/ (the batched leg's settings) and exit")
+    ap.add_argument("--tune-out", default=None, help="report path for --tune")
     ap.add_argument("--hubs", type=int, default=None)
     ap.add_argument("--alpha", type=int, default=None)
     ap.add_argument("--beta", type=int, default=None)
     ap.add_argument("--inspect-vpt", type=int, default=None)
-    ap.add_argument("--batch-contexts", type=int, default=2)
-    ap.add_argument("--batch-lanes", type=int, default=256)
-    ap.add_argument("--burst", type=int, default=4)
+    ap.add_argument("--batch-contexts", type=int, default=None)
+    ap.add_argument("--burst", type=int, default=None)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-queries", type=int, default=120)
+    ap.add_argument("--no-plain", action="store_true",
+                    help="batched and served legs only (profiling runs)")
+    ap.add_argument("--serve-queries", type=int, default=1000)
     ap.add_argument("--serve-rates", default="200,800,2000,4000")
+    ap.add_argument("--serve-s22-rates", default="2000,4000",
+                    help="rates of the R-MAT-only served line (empty: none)")
     ap.add_argument("--admit-window", type=int, default=None)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device 4 times (HBM counter yardstick)")
@@ -324,8 +429,11 @@ def main() -> int:
         log(f"copy calibration: {nb >> 20} MiB in {dt * 1e3:.3f} ms")
         a.free()
         b.free()
+    cpu = host_cpu_info()
+    # host threads: the cores this process may use, at most the 16-core
+    # share of one GPU on these machines
+    threads = max(1, min(16, cpu["affinity_cpus"]))
     wl = WORKLOADS[args.workload]
-    threads = min(16, os.cpu_count() or 1)
     log(f"generating R-MAT scale {wl['scale']} ({threads} threads)")
     t0 = time.perf_counter()
     h = ibfs_amd.HostCsr.rmat(wl["scale"], 16, 0.57, 0.19, 0.19, seed=1, permute=True,
@@ -340,67 +448,95 @@ def main() -> int:
     groupby_ms = (time.perf_counter() - t0) * 1e3
     K = args.lane_bits
     n = len(src)
+    if args.tune:
+        path = Path(args.tune_out) if args.tune_out else TUNED_DIR / f"batch_rmat_{args.workload}.json"
+        path.parent.mkdir(parents=True, exist_ok=True)
+        log(f"batched autotuner on the {n} sources")
+        t0 = time.perf_counter()
+        best, c, b, rep = ibfs_amd.autotune_batch(g, src, contexts=2, burst=4, passes=2, reps=2)
+        rep["workload"] = f"rmat_s{wl['scale']}_ef16_{n}src_groupby"
+        rep["seconds"] = round(time.perf_counter() - t0, 1)
+        path.write_text(json.dumps(rep, indent=1) + "\n")
+        log(f"best {rep['best']} at {rep['best_gteps']} GTEPS; report in {path}")
+        return 0
+    emit_mode = 0 if args.emit_mode is None else args.emit_mode
     out = ibfs_amd.DeviceBuffer(4 * n * V)
-
-    plain = plain_leg(g, src, K, args.steps, args.warmup, tune, args.emit_mode, out)
-    log(f"plain: {plain['ms_per_step']:.2f} ms/step, {plain['gteps']:.1f} GTEPS")
-    roof, kernels, levels = roofline_leg(g, src, K, V, tune, plain["kern_ms"], plain["launches"],
-                                         args.steps)
-    roof["traffic"], roof["traffic_source"] = measured_traffic(args.workload, K, roof["kernel"])
-    if roof["traffic"]:
-        roof["alg_bytes_per_launch"] = round(kernels[roof["kernel"]]["alg_GB_per_step"] * 1e9
-                                             / max(1, kernels[roof["kernel"]]["launches_per_step"]))
-    log(f"dominant kernel {roof['kernel']}: {roof['achieved']:.0f} GB/s algorithmic")
-
-    batched = None
-    if not args.no_batch:
-        batched = batch_leg(g, src, args.batch_lanes, args.batch_contexts, args.burst, args.steps,
-                            args.warmup, tune, args.emit_mode, out)
-        batched["gteps"] = round(batched["gteps"], 3)
-        batched["vs_plain"] = round(batched["gteps"] / plain["gteps"], 3)
-        log(f"batched: {batched['ms_per_step']:.2f} ms/step, {batched['gteps']} GTEPS "
-            f"({batched['vs_plain']}x plain)")
-
-    cpu = None
-    verified = 0
+    # the sample of sources checked against the oracle (and timed on the
+    # CPU): spread over every group
+    stride = max(1, n // max(1, min(args.cpu_sample, n)))
+    sample_rows = np.arange(0, n, stride)[:args.cpu_sample]
+    sample = src[sample_rows]
+    beg = adj = want = None
+    cpu_rec = None
     if not args.no_cpu:
-        import oracle  # CPU baseline leg only
+        import oracle  # CPU baseline and checker only
         beg, adj = h.arrays()
-        sample = src[:min(args.cpu_sample, n)]
         cth = args.cpu_threads or threads
         log(f"cpu baseline: {len(sample)} sources on {cth} threads")
         t0 = time.perf_counter()
         want = oracle.bfs_many(beg, adj, sample, threads=cth)
         cpu_s = time.perf_counter() - t0
         cpu_edges = sum(oracle.traversed_edges(beg, d) for d in want)
-        cpu = {"value": round(cpu_edges / cpu_s / 1e9, 4), "unit": "GTEPS", "cores": cth,
-               "kind": "port",
-               "sample": f"textbook BFS of the first {len(sample)} of the {n} GroupBy-ordered "
-                         f"sources on the same graph, {cpu_s:.1f} s"}
-        # the same BFS on one thread, over the first two sources of the sample
-        one = sample[:2]
+        cpu_rec = {"value": round(cpu_edges / cpu_s / 1e9, 4), "unit": "GTEPS", "cores": cth,
+                   "kind": "port",
+                   "sample": f"textbook BFS of {len(sample)} of the {n} GroupBy-ordered sources "
+                             f"(every {stride}th) on the same graph, {cpu_s:.1f} s",
+                   "host": cpu}
+        one = sample[:8]
         t0 = time.perf_counter()
         want1 = oracle.bfs_many(beg, adj, one, threads=1)
         one_s = time.perf_counter() - t0
         one_edges = sum(oracle.traversed_edges(beg, d) for d in want1)
-        cpu["single_thread"] = {"value": round(one_edges / one_s / 1e9, 4), "unit": "GTEPS",
-                                "cores": 1, "sample": f"first {len(one)} sources, {one_s:.1f} s"}
-        # untimed check: the device depth arrays of the sample equal the CPU ones
-        got = out.to_host(count=len(sample) * V).reshape(len(sample), V)
-        if not np.array_equal(got, want):
-            log("PARITY FAILURE: device depths differ from the CPU oracle")
-            return 4
-        verified = len(sample)
-        log(f"cpu {cpu['value']} GTEPS; {verified} device depth arrays verified bit-exact")
-    del out
+        cpu_rec["single_thread"] = {"value": round(one_edges / one_s / 1e9, 4), "unit": "GTEPS",
+                                    "cores": 1, "sample": f"{len(one)} sources, {one_s:.1f} s"}
+        log(f"cpu {cpu_rec['value']} GTEPS on {cth} threads, {cpu_rec['single_thread']['value']} "
+            f"on one")
 
-    served = None
+    plain = roof = kernels = levels = None
+    verified = {}
+    if not args.no_plain:
+        plain = plain_leg(g, src, K, args.steps, args.warmup, tune, emit_mode, out)
+        log(f"plain: {plain['ms_per_step']:.2f} ms/step, {plain['gteps']:.1f} GTEPS "
+            f"({plain['events_ms_per_step']:.2f} ms/step with kernel events)")
+        if want is not None:  # untimed: the plain leg's own output
+            if not verify_sample(out, sample_rows, want, V, "plain"):
+                return 4
+            verified["plain"] = len(sample_rows)
+        roof, kernels, levels = roofline_leg(g, src, K, V, tune, plain["kern_ms"],
+                                             plain["launches"], args.steps)
+        roof["traffic"], roof["traffic_source"] = measured_traffic(args.workload, K, roof["kernel"])
+        if roof["traffic"]:
+            roof["alg_bytes_per_launch"] = round(kernels[roof["kernel"]]["alg_GB_per_step"] * 1e9
+                                                 / max(1, kernels[roof["kernel"]]["launches_per_step"]))
+        log(f"dominant kernel {roof['kernel']}: {roof['achieved']:.0f} GB/s algorithmic")
+
+    batched = None
+    if not args.no_batch:
+        out_b = ibfs_amd.DeviceBuffer(4 * n * V)
+        batched = batch_leg(g, src, args.steps, args.warmup, tune, args.emit_mode, out_b,
+                            args.workload, args.batch_contexts, args.burst)
+        batched["gteps"] = round(batched["gteps"], 3)
+        if plain:
+            batched["vs_plain"] = round(batched["gteps"] / plain["gteps"], 3)
+        log(f"batched: {batched['ms_per_step']:.2f} ms/step, {batched['gteps']} GTEPS "
+            f"({batched.get('vs_plain')}x plain)")
+        if want is not None:
+            if not verify_sample(out_b, sample_rows, want, V, "batched"):
+                return 4
+            verified["batched"] = len(sample_rows)
+        out_b.free()
+    if verified:
+        log(f"device depth arrays verified bit-exact against the oracle: {verified}")
+    out.free()
+
+    served = served_s22 = None
     if not args.no_serve:
         log("generating Chung-Lu graph (3.07M vertices, gamma 2.2, mean 76, cap 33000)")
         t0 = time.perf_counter()
         hc = ibfs_amd.HostCsr.chung_lu(3_070_000, 2.2, 76.0, 33000.0, seed=1, threads=threads)
         gc = ibfs_amd.DeviceGraph(hc, device=0)
         log(f"Chung-Lu V={hc.vert_count} E={hc.edge_count} in {time.perf_counter() - t0:.1f}s")
+
         # status budget of one traversal context per graph (32 snapshots of
         # 256-lane words per vertex with an edge): with admission at cohort
         # starts, one context per graph gave the largest cohorts and the best
@@ -408,24 +544,37 @@ def main() -> int:
         # 2,000 queries/s; 1 + 1: 11 ms and 10.3K)
         def one_context(gr):
             return 32 * gr.traversal_vertices * 32 + 4096
+        check = not args.no_cpu
+        if check and beg is None:
+            beg, adj = h.arrays()
+        cbeg, cadj = hc.arrays() if check else (None, None)
+        # output rows per graph: room for the queries in flight at the highest
+        # rate (about 60 GB of depth arrays each)
         graphs = [
             {"name": f"rmat_s{wl['scale']}", "g": g, "pool": h.gen_sources(4096, seed=3),
-             "status_bytes": one_context(g)},
+             "status_bytes": one_context(g), "beg": beg, "adj": adj,
+             "pool_rows": int(60e9 // (4 * V))},
             {"name": "chung_lu_3.07M", "g": gc, "pool": hc.gen_sources(4096, seed=3),
-             "status_bytes": one_context(gc)},
+             "status_bytes": one_context(gc), "beg": cbeg, "adj": cadj,
+             "pool_rows": int(60e9 // (4 * hc.vert_count))},
         ]
         rates = [float(x) for x in args.serve_rates.split(",") if x]
         served = served_leg(graphs, rates, args.serve_queries, seed=3, K=256,
-                            admit_window=args.admit_window)
+                            admit_window=args.admit_window, check=check, threads=threads)
+        rates1 = [float(x) for x in args.serve_s22_rates.split(",") if x]
+        if rates1:
+            log("served, R-MAT s22 alone")
+            served_s22 = served_leg(graphs[:1], rates1, args.serve_queries, seed=4, K=256,
+                                    admit_window=args.admit_window, check=check, threads=threads)
 
     res = {
         "metric": METRIC,
-        "value": round(plain["gteps"], 3),
+        "value": round(plain["gteps"], 3) if plain else None,
         "unit": "GTEPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(plain["ms_per_step"], 3),
+        "ms_per_step": round(plain["ms_per_step"], 3) if plain else None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -436,15 +585,19 @@ def main() -> int:
                    "V_with_edges": g.traversal_vertices,
                    "sources": n, "lane_bits": K, "groups": (n + K - 1) // K,
                    "graph_seed": 1, "source_seed": 2, "permuted_ids": True,
-                   "policy": f"alpha_beta({plain['cfg'].alpha},{plain['cfg'].beta})",
+                   "policy": (f"alpha_beta({plain['cfg'].alpha},{plain['cfg'].beta})"
+                              if plain else None),
                    "depth_out": "device int32 [sources][V]"},
         "roofline": roof,
-        "cpu_baseline": cpu,
-        "graph500_gteps": round(plain["gteps"] / 2, 3),
+        "cpu_baseline": cpu_rec,
+        "graph500_gteps": round(plain["gteps"] / 2, 3) if plain else None,
         "groupby_ms": round(groupby_ms, 3),
+        "plain_with_kernel_events_ms_per_step": (round(plain["events_ms_per_step"], 3)
+                                                  if plain else None),
         "kernels": kernels,
         "batched": batched,
         "served": served,
+        "served_rmat_only": served_s22,
         "levels_per_group": levels,
         "verified_sources": verified,
     }

@@ -18,8 +18,10 @@ from ._lib import (IbfsError, OUT_DEVICE, OUT_HOST, POLICY_ALPHA_BETA, POLICY_BU
                    lib)
 
 __all__ = [
-    "IbfsError", "HostCsr", "DeviceGraph", "Engine", "DeviceBuffer", "graph", "gpu_ibfs",
-    "default_config", "device_count", "POLICY_FIXED", "POLICY_ALPHA_BETA", "POLICY_TD_ONLY",
+    "IbfsError", "HostCsr", "DeviceGraph", "Engine", "Batch", "Server", "DeviceBuffer",
+    "DeviceView", "RowPool", "RowRanges", "graph", "gpu_ibfs",
+    "default_config", "device_count", "set_wait_timeout", "get_wait_timeout", "POLICY_FIXED",
+    "POLICY_ALPHA_BETA", "POLICY_TD_ONLY",
     "POLICY_BU_ONLY", "POLICY_HYBRID",
 ]
 
@@ -36,6 +38,17 @@ def device_count() -> int:
     n = C.c_int(0)
     check(lib().ibfs_device_count(C.byref(n)))
     return n.value
+
+
+def set_wait_timeout(seconds: float) -> None:
+    """Limit of every host wait of the library on device work (process-wide)."""
+    check(lib().ibfs_set_wait_timeout(float(seconds)))
+
+
+def get_wait_timeout() -> float:
+    x = C.c_double()
+    check(lib().ibfs_get_wait_timeout(C.byref(x)))
+    return x.value
 
 
 def default_config(**overrides) -> Config:
@@ -137,6 +150,21 @@ class HostCsr:
             pass
 
 
+def _copy_out(ptr, nbytes, dtype, count, offset, out):
+    item = np.dtype(dtype).itemsize
+    n = (nbytes // item - offset) if count is None else count
+    if offset < 0 or n < 0 or (offset + n) * item > nbytes:
+        raise ValueError("range outside the device memory")
+    if out is None:
+        out = np.empty(n, dtype=dtype)
+    else:
+        if out.dtype != np.dtype(dtype) or not out.flags.c_contiguous or out.size < n:
+            raise ValueError("out must be C-contiguous, of the dtype and large enough")
+        out = out.reshape(-1)[:n]
+    check(lib().ibfs_memcpy_d2h(out.ctypes.data, C.c_void_p(ptr + offset * item), n * item))
+    return out
+
+
 class DeviceBuffer:
     """Raw device allocation (for device-resident depth outputs)."""
 
@@ -146,16 +174,12 @@ class DeviceBuffer:
         self.ptr = p.value
         self.nbytes = nbytes
 
-    def to_host(self, dtype=np.int32, count: Optional[int] = None, offset: int = 0) -> np.ndarray:
-        """Copies `count` elements starting at element `offset` to the host."""
-        item = np.dtype(dtype).itemsize
-        n = (self.nbytes // item - offset) if count is None else count
-        if offset < 0 or n < 0 or (offset + n) * item > self.nbytes:
-            raise ValueError("range outside the buffer")
-        out = np.empty(n, dtype=dtype)
-        check(lib().ibfs_memcpy_d2h(out.ctypes.data, C.c_void_p(self.ptr + offset * item),
-                                    out.nbytes))
-        return out
+    def to_host(self, dtype=np.int32, count: Optional[int] = None, offset: int = 0,
+                out: Optional[np.ndarray] = None) -> np.ndarray:
+        """Copies `count` elements starting at element `offset` to the host,
+        into `out` (a C-contiguous array of at least `count` elements of
+        dtype; a view of its first `count` is returned) or a new array."""
+        return _copy_out(self.ptr, self.nbytes, dtype, count, offset, out)
 
     def copy_from(self, other: "DeviceBuffer", nbytes: Optional[int] = None) -> None:
         """Device-to-device copy of nbytes (default: all of `other`)."""
@@ -174,6 +198,103 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+
+
+class DeviceView:
+    """A sub-range of a DeviceBuffer (same to_host interface).  It holds a
+    reference to its buffer, so the memory stays allocated while the view
+    is alive."""
+
+    def __init__(self, buf: DeviceBuffer, offset_bytes: int, nbytes: int):
+        if offset_bytes < 0 or nbytes < 0 or offset_bytes + nbytes > buf.nbytes:
+            raise ValueError("view outside the buffer")
+        self.buf = buf
+        self.offset = offset_bytes
+        self.ptr = buf.ptr + offset_bytes
+        self.nbytes = nbytes
+
+    def to_host(self, dtype=np.int32, count: Optional[int] = None, offset: int = 0,
+                out: Optional[np.ndarray] = None) -> np.ndarray:
+        return _copy_out(self.ptr, self.nbytes, dtype, count, offset, out)
+
+
+class RowRanges:
+    """First-fit allocator of consecutive rows in [0, rows), with coalescing
+    of freed ranges (the bookkeeping of RowPool; no device memory)."""
+
+    def __init__(self, rows: int):
+        if rows < 1:
+            raise ValueError("need rows >= 1")
+        self.rows = rows
+        self.free_ranges = [(0, rows)]  # sorted (first row, count), coalesced
+
+    def take(self, n: int) -> Optional[int]:
+        if n < 1 or n > self.rows:
+            raise ValueError(f"cannot hold {n} rows in {self.rows}")
+        for i, (r0, cnt) in enumerate(self.free_ranges):
+            if cnt >= n:
+                if cnt == n:
+                    self.free_ranges.pop(i)
+                else:
+                    self.free_ranges[i] = (r0 + n, cnt - n)
+                return r0
+        return None
+
+    def fits(self, n: int) -> bool:
+        return any(cnt >= n for _, cnt in self.free_ranges)
+
+    def give(self, r0: int, n: int) -> None:
+        if r0 < 0 or n < 1 or r0 + n > self.rows:
+            raise ValueError("range outside the pool")
+        for a, c in self.free_ranges:
+            if r0 < a + c and a < r0 + n:
+                raise ValueError(f"rows {r0}..{r0 + n - 1} are already free")
+        merged = []
+        for a, c in sorted(self.free_ranges + [(r0, n)]):
+            if merged and merged[-1][0] + merged[-1][1] == a:
+                merged[-1] = (merged[-1][0], merged[-1][1] + c)
+            else:
+                merged.append((a, c))
+        self.free_ranges = merged
+
+
+class RowPool:
+    """Device arena of depth rows (int32[V] each) for query outputs that are
+    recycled: alloc(n) returns a DeviceView of n consecutive rows (first fit,
+    waiting up to `timeout` seconds for space; None after a timeout), free()
+    returns them.  Thread-safe.  One allocation at construction: nothing is
+    allocated or freed per query."""
+
+    def __init__(self, vert_count: int, rows: int, device: int = 0):
+        import threading
+        if rows < 1 or vert_count < 1:
+            raise ValueError("need rows >= 1 and vert_count >= 1")
+        self.row_bytes = 4 * vert_count
+        self.rows = rows
+        self.buf = DeviceBuffer(self.row_bytes * rows, device)
+        self._ranges = RowRanges(rows)
+        self._cv = threading.Condition()
+        self.waits = 0            # alloc calls that had to wait for space
+
+    def alloc(self, n: int, timeout: Optional[float] = None) -> Optional[DeviceView]:
+        with self._cv:
+            r0 = self._ranges.take(n)
+            if r0 is None:
+                self.waits += 1
+                if not self._cv.wait_for(lambda: self._ranges.fits(n), timeout=timeout):
+                    return None
+                r0 = self._ranges.take(n)
+        return DeviceView(self.buf, r0 * self.row_bytes, n * self.row_bytes)
+
+    def free(self, view: DeviceView) -> None:
+        if view.buf is not self.buf:
+            raise ValueError("view of another pool")
+        with self._cv:
+            self._ranges.give(view.offset // self.row_bytes, view.nbytes // self.row_bytes)
+            self._cv.notify_all()
+
+    def close(self) -> None:
+        self.buf.free()
 
 
 class DeviceGraph:
@@ -425,13 +546,16 @@ class Server:
     are written to the given device buffer ([n][V] int32)."""
 
     def __init__(self, g: DeviceGraph, status_bytes: int = 0, cfg: Optional[Config] = None,
-                 admit_window: Optional[int] = None, **overrides):
+                 admit_window: Optional[int] = None, host_levels: Optional[bool] = None,
+                 **overrides):
         if cfg is None:
             overrides.setdefault("lane_bits", 256)
             cfg = default_config(device=g.device, **overrides)
         h = C.c_void_p()
         check(lib().ibfs_server_create(g._h, C.byref(cfg), status_bytes, C.byref(h)))
         self._h = h
+        if host_levels is not None:
+            check(lib().ibfs_server_set_levels(h, int(bool(host_levels))))
         if admit_window is not None:
             check(lib().ibfs_server_set_admission(h, admit_window))
         self.graph = g
@@ -439,19 +563,36 @@ class Server:
         c, l = C.c_int32(), C.c_int32()
         check(lib().ibfs_server_contexts(h, C.byref(c), C.byref(l)))
         self.contexts, self.lanes = c.value, l.value
+        # output buffer of every query not yet released: the worker's kernels
+        # write into it until the query is done, so it must stay allocated
+        self._outs: dict[int, object] = {}
 
     def set_admission(self, window_levels: int) -> None:
         """Sources join a running context only within window_levels levels of
-        its cohort's start (0: only at cohort starts, cohorts run device-driven)."""
+        its cohort's start (0: only at cohort starts)."""
         check(lib().ibfs_server_set_admission(self._h, window_levels))
 
-    def submit(self, sources, out: DeviceBuffer) -> int:
+    def set_levels(self, host_levels: bool) -> None:
+        """False (default): device-driven cohort levels; True: host-driven
+        levels with lanes freed and reused one by one."""
+        check(lib().ibfs_server_set_levels(self._h, int(bool(host_levels))))
+
+    def counters(self) -> dict:
+        lv, ad, inf = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().ibfs_server_counters(self._h, C.byref(lv), C.byref(ad), C.byref(inf)))
+        return {"levels": lv.value, "admitted": ad.value, "admitted_in_flight": inf.value}
+
+    def submit(self, sources, out) -> int:
+        """Queues a query; its depth arrays ([n][V] int32) go to `out` (a
+        DeviceBuffer or DeviceView), which the server keeps referenced until
+        release() or close()."""
         s = np.ascontiguousarray(sources, dtype=np.int32)
         if out.nbytes < 4 * len(s) * self.graph.vert_count:
             raise ValueError("device buffer too small")
         qid = C.c_int64()
         check(lib().ibfs_server_submit(self._h, _i32p(s), len(s), C.c_void_p(out.ptr),
                                        C.byref(qid)))
+        self._outs[qid.value] = out
         return qid.value
 
     def wait(self, qid: int, timeout_ms: float = -1.0) -> dict:
@@ -463,11 +604,14 @@ class Server:
 
     def release(self, qid: int) -> None:
         check(lib().ibfs_server_release(self._h, qid))
+        self._outs.pop(qid, None)
 
     def close(self) -> None:
         if self._h:
+            # destroy joins the worker, so no kernel writes an output afterwards
             lib().ibfs_server_destroy(self._h)
             self._h = C.c_void_p()
+        self._outs.clear()
 
     def __del__(self):
         try:

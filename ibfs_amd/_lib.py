@@ -93,6 +93,8 @@ SIGNATURES = {
     "ibfs_last_error": (C.c_int, [C.c_char_p, C.c_size_t]),
     "ibfs_config_default": (C.c_int, [C.POINTER(Config)]),
     "ibfs_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "ibfs_set_wait_timeout": (C.c_int, [C.c_double]),
+    "ibfs_get_wait_timeout": (C.c_int, [C.POINTER(C.c_double)]),
     "ibfs_device_alloc": (C.c_int, [C.c_int, C.c_size_t, C.POINTER(P)]),
     "ibfs_device_free": (C.c_int, [P]),
     "ibfs_memcpy_d2h": (C.c_int, [P, P, C.c_size_t]),
@@ -135,6 +137,8 @@ SIGNATURES = {
     "ibfs_server_create": (C.c_int, [P, C.POINTER(Config), C.c_int64, C.POINTER(P)]),
     "ibfs_server_contexts": (C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "ibfs_server_set_admission": (C.c_int, [P, C.c_int32]),
+    "ibfs_server_set_levels": (C.c_int, [P, C.c_int32]),
+    "ibfs_server_counters": (C.c_int, [P, I64P, I64P, I64P]),
     "ibfs_server_submit": (C.c_int, [P, I32P, C.c_int32, P, C.POINTER(C.c_int64)]),
     "ibfs_server_wait": (C.c_int, [P, C.c_int64, C.c_double, C.POINTER(QueryInfo)]),
     "ibfs_server_release": (C.c_int, [P, C.c_int64]),

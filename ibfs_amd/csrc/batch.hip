@@ -404,6 +404,10 @@ static int batch_run_impl(ibfs_batch_s *b, const int32_t *src, int64_t n, int32_
   // lanes of a group progress alike (results are per lane and unchanged)
   b->order.resize((size_t)n);
   std::iota(b->order.begin(), b->order.end(), (int64_t)0);
+  // set-up work goes on the first context's stream and is waited for there
+  // (no device-wide synchronisation: other engines, batches and servers on
+  // the device keep running); the contexts start after it
+  hipStream_t st0 = b->ctx[0].st;
   if (b->cfg.pack_sources && n > b->K) {
     if (b->key_cap < n) {
       dfree(b->d_keysrc);
@@ -413,23 +417,27 @@ static int batch_run_impl(ibfs_batch_s *b, const int32_t *src, int64_t n, int32_
       HIP_TRY(hipMalloc((void **)&b->d_keys, sizeof(u64) * (size_t)n));
       b->key_cap = n;
     }
-    HIP_TRY(hipMemcpy(b->d_keysrc, src, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(b->d_keysrc, src, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice,
+                           st0));
     HIP_TRY(launch_two_hop(b->g->d_beg, b->g->d_adj, b->g->d_deg, b->g->d_cid, b->d_keysrc, n,
-                           b->d_keys, nullptr));
+                           b->d_keys, st0));
     b->h_keys.resize((size_t)n);
-    HIP_TRY(hipMemcpy(b->h_keys.data(), b->d_keys, sizeof(u64) * (size_t)n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpyAsync(b->h_keys.data(), b->d_keys, sizeof(u64) * (size_t)n,
+                           hipMemcpyDeviceToHost, st0));
+    HIP_TRY(hipStreamSynchronize(st0));
     std::stable_sort(b->order.begin(), b->order.end(),
                      [&](int64_t x, int64_t y) { return b->h_keys[x] > b->h_keys[y]; });
   }
   b->psrc.resize((size_t)n);
   for (int64_t p = 0; p < n; ++p) b->psrc[p] = src[b->order[p]];
-  HIP_TRY(hipMemset(b->d_edges, 0, sizeof(u64)));
-  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemsetAsync(b->d_edges, 0, sizeof(u64), st0));
+  HIP_TRY(hipStreamSynchronize(st0));
   const auto t0 = std::chrono::steady_clock::now();
   const int64_t groups = (n + b->K - 1) / b->K;
   int64_t next = 0, finished = 0;
   int rc = IBFS_OK;
   for (auto &c : b->ctx) c.state = IDLE;
+  WaitClock clk;  // restarted whenever a context makes progress
   while (finished < groups && rc == IBFS_OK) {
     bool progressed = false;
     for (auto &c : b->ctx) {
@@ -476,10 +484,21 @@ static int batch_run_impl(ibfs_batch_s *b, const int32_t *src, int64_t n, int32_
       }
       if (rc) break;
     }
-    if (!progressed) std::this_thread::yield();
+    if (progressed) {
+      clk.restart();
+    } else {
+      if (clk.expired()) {
+        rc = fail(IBFS_ERR_STATE, "no batch context progressed within the wait limit (%.1f s)",
+                  wait_timeout_s());
+        break;
+      }
+      std::this_thread::yield();
+    }
   }
   if (rc) {
-    for (auto &c : b->ctx) (void)hipStreamSynchronize(c.st);
+    // a context that exceeded the wait limit is not waited for again
+    if (rc != IBFS_ERR_STATE)
+      for (auto &c : b->ctx) (void)hipStreamSynchronize(c.st);
     return rc;
   }
   for (auto &c : b->ctx) HIP_TRY(hipStreamSynchronize(c.st));
@@ -521,8 +540,10 @@ int ibfs_batch_closeness(ibfs_batch_t b, const int32_t *src, int64_t n, int64_t 
     HIP_TRY(hipMalloc((void **)&b->d_reach, sizeof(u64) * (size_t)n));
     b->sum_cap = n;
   }
-  HIP_TRY(hipMemset(b->d_sum, 0, sizeof(u64) * (size_t)n));
-  HIP_TRY(hipMemset(b->d_reach, 0, sizeof(u64) * (size_t)n));
+  // cleared on the first context's stream, which the run synchronises
+  // before any context starts
+  HIP_TRY(hipMemsetAsync(b->d_sum, 0, sizeof(u64) * (size_t)n, b->ctx[0].st));
+  HIP_TRY(hipMemsetAsync(b->d_reach, 0, sizeof(u64) * (size_t)n, b->ctx[0].st));
   b->closeness = true;
   int rc = batch_run_impl(b, src, n, nullptr, IBFS_OUT_DEVICE, stats);
   b->closeness = false;

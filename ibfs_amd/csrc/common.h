@@ -49,6 +49,24 @@ inline int check_direction_config(const ibfs_config &c) {
 // Parallel helper: runs fn(t) on nthreads std::threads.
 void parallel_for_threads(int nthreads, void (*fn)(int, void *), void *ctx);
 
+// Limit of every host wait of the library on device work (seconds,
+// process-wide, ibfs_set_wait_timeout; default 120).  A wait that exceeds it
+// returns IBFS_ERR_STATE instead of spinning on a stream that never drains.
+double wait_timeout_s();
+
+// Deadline of one wait, started at construction.
+class WaitClock {
+ public:
+  WaitClock();
+  // true once the limit has passed (reads the clock every 16 calls)
+  bool expired();
+  void restart();
+
+ private:
+  double t0_;
+  unsigned n_ = 0;
+};
+
 }  // namespace ibfs
 
 struct ibfs_hostcsr_s {

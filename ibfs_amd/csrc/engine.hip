@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -65,12 +66,18 @@ size_t masks_words(const ibfs_engine_s *e) { return (size_t)(2 + (e->R + 1)) * e
 // wake-up costs tens of microseconds, and an event query reads one completion
 // signal where a stream query takes the stream's lock.  The level loop waits
 // on posted control blocks instead (wait_post).
-hipError_t poll_stream(hipStream_t st, hipEvent_t ev) {
+// Past the wait limit (ibfs_set_wait_timeout) it fails with IBFS_ERR_STATE.
+int poll_stream(hipStream_t st, hipEvent_t ev) {
   const hipError_t r = hipEventRecord(ev, st);
-  if (r != hipSuccess) return r;
+  if (r != hipSuccess) return fail(IBFS_ERR_HIP, "hipEventRecord: %s", hipGetErrorString(r));
+  WaitClock clk;
   for (;;) {
     const hipError_t q = hipEventQuery(ev);
-    if (q != hipErrorNotReady) return q;
+    if (q == hipSuccess) return IBFS_OK;
+    if (q != hipErrorNotReady) return fail(IBFS_ERR_HIP, "stream error: %s", hipGetErrorString(q));
+    if (clk.expired())
+      return fail(IBFS_ERR_STATE, "group did not finish within the wait limit (%.1f s)",
+                  wait_timeout_s());
   }
 }
 
@@ -78,11 +85,17 @@ hipError_t poll_stream(hipStream_t st, hipEvent_t ev) {
 // 1,024 reads the stream is queried, so an error, or an idle stream without
 // the tag, ends the wait with an error), then copies it to e->h_ctl.
 int wait_post(ibfs_engine_s *e, u64 seq) {
+  WaitClock clk;
   for (unsigned i = 1;; ++i) {
     if (__atomic_load_n(&e->post->tag, __ATOMIC_ACQUIRE) == seq) break;
     if ((i & 1023) == 0) {
       const hipError_t q = hipStreamQuery(e->st);
-      if (q == hipErrorNotReady) continue;
+      if (q == hipErrorNotReady) {
+        if (clk.expired())
+          return fail(IBFS_ERR_STATE, "level did not finish within the wait limit (%.1f s)",
+                      wait_timeout_s());
+        continue;
+      }
       if (q != hipSuccess) return fail(IBFS_ERR_HIP, "stream error: %s", hipGetErrorString(q));
       if (__atomic_load_n(&e->post->tag, __ATOMIC_ACQUIRE) == seq) break;
       return fail(IBFS_ERR_STATE, "control block post did not arrive");
@@ -95,10 +108,13 @@ int wait_post(ibfs_engine_s *e, u64 seq) {
 // Elapsed time between two events the device has passed.  A posted control
 // block can reach the host before the runtime has seen the completion of the
 // events enqueued ahead of it, so each is queried until it reports completion.
+// Past the wait limit it returns hipErrorNotReady (reported by HIP_TRY).
 hipError_t elapsed_ms(float *ms, hipEvent_t a, hipEvent_t b) {
   for (hipEvent_t x : {a, b}) {
     hipError_t q;
+    WaitClock clk;
     while ((q = hipEventQuery(x)) == hipErrorNotReady) {
+      if (clk.expired()) return hipErrorNotReady;
     }
     if (q != hipSuccess) return q;
   }
@@ -126,6 +142,29 @@ int check_device(int dev) {
     return fail(IBFS_ERR_NODEVICE, "no HIP device available");
   if (dev < 0 || dev >= n) return fail(IBFS_ERR_ARG, "device %d out of range [0,%d)", dev, n);
   if (hipSetDevice(dev) != hipSuccess) return fail(IBFS_ERR_HIP, "hipSetDevice(%d) failed", dev);
+  return IBFS_OK;
+}
+
+// One non-blocking stream per device for the library's synchronous copies
+// (created on first use, kept until the process ends).
+int copy_stream(int dev, hipStream_t *out) {
+  static std::mutex mu;
+  static std::vector<hipStream_t> streams;
+  std::lock_guard<std::mutex> lk(mu);
+  if (dev < 0) return fail(IBFS_ERR_ARG, "bad device %d", dev);
+  if ((size_t)dev >= streams.size()) streams.resize((size_t)dev + 1, nullptr);
+  if (!streams[(size_t)dev]) {
+    int cur = 0;
+    HIP_TRY(hipGetDevice(&cur));
+    HIP_TRY(hipSetDevice(dev));
+    const hipError_t r = hipStreamCreateWithFlags(&streams[(size_t)dev], hipStreamNonBlocking);
+    (void)hipSetDevice(cur);
+    if (r != hipSuccess) {
+      streams[(size_t)dev] = nullptr;
+      return fail(IBFS_ERR_HIP, "copy stream: %s", hipGetErrorString(r));
+    }
+  }
+  *out = streams[(size_t)dev];
   return IBFS_OK;
 }
 
@@ -460,7 +499,7 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
     }
     HIP_TRY(hipMemcpyAsync(e->h_count, e->d_count, sizeof(u64), hipMemcpyDeviceToHost, e->st));
   }
-  HIP_TRY(poll_stream(e->st, e->ev_wait));
+  if (int rc = poll_stream(e->st, e->ev_wait)) return rc;
   if (stats) {
     float ms = 0;
     HIP_TRY(elapsed_ms(&ms, t_begin, t_end));
@@ -573,9 +612,17 @@ int ibfs_memcpy_h2d(void *dst, const void *src, size_t bytes) {
 
 int ibfs_memcpy_d2d(void *dst, const void *src, size_t bytes) {
   if (bytes && (!dst || !src)) return fail(IBFS_ERR_ARG, "null device pointer");
-  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
-  // a device-to-device hipMemcpy may return before the copy has finished
-  HIP_TRY(hipDeviceSynchronize());
+  if (!bytes) return IBFS_OK;
+  // on a library stream of the destination's device, waiting for that
+  // stream only: engines, batches and servers on the device keep running
+  hipPointerAttribute_t pa{};
+  HIP_TRY(hipPointerGetAttributes(&pa, dst));
+  int dev = pa.device;
+  if (dev < 0) HIP_TRY(hipGetDevice(&dev));
+  hipStream_t st = nullptr;
+  if (int rc = copy_stream(dev, &st)) return rc;
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipStreamSynchronize(st));
   return IBFS_OK;
 }
 

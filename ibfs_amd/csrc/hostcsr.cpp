@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cctype>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <sys/stat.h>
@@ -24,6 +25,27 @@ void set_error(const char *fmt, ...) {
 }
 
 const std::string &last_error() { return g_last_error; }
+
+static std::atomic<double> g_wait_timeout_s{120.0};
+
+double wait_timeout_s() { return g_wait_timeout_s.load(std::memory_order_relaxed); }
+
+static double steady_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+WaitClock::WaitClock() : t0_(steady_s()) {}
+
+bool WaitClock::expired() {
+  if ((++n_ & 15u) != 0) return false;
+  return steady_s() - t0_ > wait_timeout_s();
+}
+
+void WaitClock::restart() {
+  t0_ = steady_s();
+  n_ = 0;
+}
 
 int fail(int code, const char *fmt, ...) {
   char buf[1024];
@@ -146,6 +168,19 @@ using namespace ibfs;
 extern "C" {
 
 int ibfs_abi_version(void) { return IBFS_ABI_VERSION; }
+
+int ibfs_set_wait_timeout(double seconds) {
+  if (!(seconds > 0) || seconds > 1e7)
+    return fail(IBFS_ERR_ARG, "wait timeout must be in (0, 1e7] seconds (got %g)", seconds);
+  ibfs::g_wait_timeout_s.store(seconds, std::memory_order_relaxed);
+  return IBFS_OK;
+}
+
+int ibfs_get_wait_timeout(double *seconds) {
+  if (!seconds) return fail(IBFS_ERR_ARG, "null argument");
+  *seconds = wait_timeout_s();
+  return IBFS_OK;
+}
 
 int ibfs_last_error(char *buf, size_t n) {
   if (!buf || n == 0) return IBFS_ERR_ARG;

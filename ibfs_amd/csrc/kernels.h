@@ -57,6 +57,17 @@ struct Plan {
   u64 split[MAX_W];     // split level (dir 3): lanes that go bottom-up
 };
 
+// Sources admitted into a running device-driven traversal at level `level`
+// (traversal ids, -1: isolated source without a status word; lanes clean so
+// far).  Written by the host with a stream-ordered copy before the level's
+// expansion is launched; entries of other levels are ignored.
+struct AdmitList {
+  int level;
+  int n;
+  int32_t src[MAX_LANES];
+  int32_t lane[MAX_LANES];
+};
+
 // Plan of a device-driven traversal before its level 0 (V: vertex count
 // of the beta rule; rho: the graph's hub-entry fraction, x1024).
 inline void plan_init(Plan *p, const ibfs_config &c, int rho, u64 V) {
@@ -194,6 +205,10 @@ hipError_t launch_init_sources(int W, const int32_t *src, int k, int lane0, u64 
 // S[src[i]] |= bit lane[i], i < n (traversal ids; -1 entries skipped).
 hipError_t launch_admit(int W, const int32_t *src, const int32_t *lane, int n, u64 *S,
                         uint8_t *flags, hipStream_t st);
+// Device-driven admission between the expansion and the inspection of
+// plan->level (k_admit_dd): no effect unless al->level == plan->level.
+hipError_t launch_admit_dd(int W, const AdmitList *al, const Plan *plan, u64 *ring, int R,
+                           int64_t V, LevelCtl *ctl_ring, uint8_t *flags, hipStream_t st);
 hipError_t launch_inspect(int W, const InspectArgs &a, hipStream_t st);
 // flags (or null): set for every vertex whose word changes (flag_top_down)
 hipError_t launch_expand_td(int W, const int64_t *beg, const int32_t *adj, const u64 *prev,

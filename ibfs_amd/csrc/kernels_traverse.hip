@@ -55,6 +55,39 @@ __global__ void k_admit(const int32_t *src, const int32_t *lane, int n, u64 *S, 
   flags[src[i]] = 1;
 }
 
+// Admission into a device-driven traversal between the expansion and the
+// inspection of level L = plan->level: when the posted list is tagged for L,
+// lane lane[i] (clean in every snapshot so far) gets its bit at src[i] in
+// S_L, the vertex is flagged, and the lanes join lam_{L-1} so that the
+// inspection of L counts a vertex that misses one of them as a bottom-up
+// candidate of L+1 (lam_{L-1} is not read again after that).  A lane whose
+// source is isolated gains nothing in any status word and stays out of
+// lam.  One workgroup; nothing to do when the plan is done or the list is
+// for another level.
+template <int W>
+__global__ void k_admit_dd(const AdmitList *al, const Plan *plan, u64 *ring, int R, int64_t V,
+                           LevelCtl *ctl_ring, uint8_t *flags) {
+  const int L = plan->level;
+  if (plan->done || L < 1 || al->level != L) return;
+  const int n = al->n;
+  u64 *S = ring + (int64_t)(L % R) * V * W;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int32_t v = al->src[i];
+    const int k = al->lane[i];
+    if (v >= 0) {
+      atomicOr(&S[(int64_t)v * W + (k >> 6)], 1ull << (k & 63));
+      flags[v] = 1;
+    }
+  }
+  if (threadIdx.x < W) {
+    u64 m = 0;
+    for (int i = 0; i < n; ++i)
+      if (al->src[i] >= 0 && (al->lane[i] >> 6) == (int)threadIdx.x)
+        m |= 1ull << (al->lane[i] & 63);
+    if (m) atomicOr(&ctl_ring[(L - 1) % R].lam[threadIdx.x], m);
+  }
+}
+
 // Direction of level L+1 from the counters of level L (same rule as the
 // host-driven driver): fixed switch level, or alpha/beta.
 __device__ void plan_decide(Plan *p, const LevelCtl *c, int L) {
@@ -927,6 +960,13 @@ hipError_t launch_init_sources(int W, const int32_t *src, int k, int lane0, u64 
   dim3 g(grid_for(k, BS, 1 << 20)), blk(BS);
   IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_init_sources<WW>, g, blk, 0, st, src, k, lane0, S0,
                                         flags));
+  return hipGetLastError();
+}
+
+hipError_t launch_admit_dd(int W, const AdmitList *al, const Plan *plan, u64 *ring, int R,
+                           int64_t V, LevelCtl *ctl_ring, uint8_t *flags, hipStream_t st) {
+  IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_admit_dd<WW>, dim3(1), dim3(BS), 0, st, al, plan, ring,
+                                        R, V, ctl_ring, flags));
   return hipGetLastError();
 }
 

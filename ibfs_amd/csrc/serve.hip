@@ -103,6 +103,7 @@ struct SCtx {
   // graph (expand, inspect, direction decision) replayed in bursts.
   Plan *d_plan = nullptr, *h_plan = nullptr;
   u64 *d_all = nullptr, *h_all = nullptr;  // lanes of the cohort (lam of level 0)
+  AdmitList *d_al = nullptr, *h_al = nullptr;  // sources joining a running cohort
   // one captured level graph per status-word width (1, 2, 4, 8 words), so a
   // cohort runs at the narrowest width that holds its sources
   hipGraph_t graph[4] = {};
@@ -110,6 +111,7 @@ struct SCtx {
   int W = 0, K = 0;  // current width: words per vertex and lanes (<= the server's)
   int coh = 0;     // 0: host-driven levels, 1: cohort levels on the device, 2: cohort emitting
   int dlevel = 0;  // cohort level the device expands next (host view)
+  WaitClock clk;   // since the context's last completed event (wait limit)
 };
 
 template <class T>
@@ -140,11 +142,18 @@ struct ibfs_server_s {
   std::unordered_map<int64_t, Query> queries;
   std::deque<PendingSrc> incoming;  // submitted, not yet seen by the worker
   std::deque<PendingSrc> pending;   // worker-owned FIFO of sources to admit
-  int64_t levels_issued = 0, lanes_admitted = 0;
+  std::atomic<int64_t> levels_issued{0}, lanes_admitted{0};
   // sources join a running context only within this many levels of its
   // cohort's start, so that the lanes of a context stay close in phase (set
   // from any thread, read by the worker)
   std::atomic<int> admit_window{0};
+  // 1: host-driven levels (every level boundary on the host; lanes finish,
+  // are emitted and reused one by one); 0: device-driven cohorts whose
+  // levels run as captured graphs, joined by waiting sources at the level
+  // boundaries inside the admission window (set from any thread)
+  std::atomic<int> host_levels{0};
+  // sources that joined a running cohort (after its first level)
+  std::atomic<int64_t> lanes_admitted_in_flight{0};
   int grid = 2048;  // workgroups of the device-driven expander
   HubCut hc;        // hubs a hybrid level pulls through
   std::vector<u64> h_key;  // two-hop size of every vertex (cohort composition)
@@ -176,6 +185,8 @@ void release_ctx(SCtx &c) {
   hfree(c.h_plan);
   dfree(c.d_all);
   hfree(c.h_all);
+  dfree(c.d_al);
+  hfree(c.h_al);
   dfree(c.ring);
   dfree(c.ctl);
   hfree(c.h_ctl);
@@ -247,6 +258,8 @@ int alloc_ctx(ibfs_server_s *s, SCtx &c) {
   H((void **)&c.h_plan, sizeof(Plan));
   A((void **)&c.d_all, sizeof(u64) * (size_t)s->W);
   H((void **)&c.h_all, sizeof(u64) * (size_t)s->W);
+  A((void **)&c.d_al, sizeof(AdmitList));
+  H((void **)&c.h_al, sizeof(AdmitList));
   if (err == hipSuccess) err = hipEventCreateWithFlags(&c.ev, hipEventDisableTiming);
   if (err != hipSuccess)
     return fail(err == hipErrorOutOfMemory ? IBFS_ERR_NOMEM : IBFS_ERR_HIP,
@@ -516,6 +529,7 @@ int process_level(ibfs_server_s *s, SCtx &c) {
     if (fresh || Ln - c.cohort <= s->admit_window) {
       rc = admit(s, c, Ln, A.data(), &n_adm);
       if (fresh && n_adm > 0) c.cohort = Ln;
+      if (!fresh) s->lanes_admitted_in_flight += n_adm;
     }
   }
   if (rc) return rc;
@@ -528,10 +542,52 @@ int process_level(ibfs_server_s *s, SCtx &c) {
   return inspect_level(s, c, Ln, filter.data(), keep.data(), nand.data(), dir != 1 ? 1 : 0);
 }
 
-// Launches up to kCohortBurst level graphs of the cohort (bounded so that no
-// snapshot of the emission window is overwritten; flushes the window first
-// when it is full), then a copy of the plan and the context's event.
-int cohort_burst(ibfs_server_s *s, SCtx &c) {
+// Waiting sources take the clean lanes of a running cohort at level L
+// (caller holds the lock): the admission list for the device, the lane
+// table, host bookkeeping.  *n_adm = sources admitted.
+int admit_into_cohort(ibfs_server_s *s, SCtx &c, int L, int *n_adm) {
+  int n = 0;
+  for (int k = 0; k < c.K && !s->pending.empty(); ++k) {
+    if (c.lanes[k].state != CLEAN) continue;
+    const PendingSrc p = s->pending.front();
+    s->pending.pop_front();
+    Query &q = s->queries[p.qid];
+    c.lanes[k].state = ACTIVE;
+    c.lanes[k].qid = p.qid;
+    c.lanes[k].admit = L;
+    c.h_al->src[n] = s->g->h_cid[q.src[p.idx]];
+    c.h_al->lane[n] = k;
+    c.h_lanes[k].out = q.out + (size_t)p.idx * (size_t)s->g->V;
+    c.h_lanes[k].admit = L;
+    c.h_lanes[k].src = q.src[p.idx];
+    if (q.t_admit < 0) q.t_admit = now_ms();
+    ++n;
+  }
+  *n_adm = n;
+  if (n == 0) return IBFS_OK;
+  c.h_al->level = L;
+  c.h_al->n = n;
+  s->lanes_admitted += n;
+  s->lanes_admitted_in_flight += n;
+  HIP_CHK(hipMemcpyAsync(c.d_al, c.h_al, offsetof(AdmitList, src) + sizeof(int32_t) * (size_t)n,
+                         hipMemcpyHostToDevice, c.st));
+  HIP_CHK(hipMemcpyAsync(reinterpret_cast<char *>(c.d_al) + offsetof(AdmitList, lane),
+                         reinterpret_cast<char *>(c.h_al) + offsetof(AdmitList, lane),
+                         sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, c.st));
+  HIP_CHK(hipMemcpyAsync(c.d_lanes, c.h_lanes, sizeof(LaneOut) * (size_t)c.K,
+                         hipMemcpyHostToDevice, c.st));
+  return IBFS_OK;
+}
+
+// Launches the cohort's next levels, then a copy of the plan and the
+// context's event.  Inside the admission window (levels 1..admit_window of
+// the cohort) one level at a time, each joined by the sources waiting at its
+// boundary: expansion, admission, inspection and decision launched
+// directly; after the window up to kCohortBurst captured level graphs
+// (bounded so that no snapshot of the emission window is overwritten; the
+// window is flushed first when it is full).  locked: the caller holds the
+// server lock.
+int cohort_burst(ibfs_server_s *s, SCtx &c, bool locked) {
   int B = std::min(kCohortBurst, s->R - 1 - (c.dlevel - c.base));
   if (B < 1) {
     std::vector<int> act;
@@ -543,7 +599,30 @@ int cohort_burst(ibfs_server_s *s, SCtx &c) {
     B = std::min(kCohortBurst, s->R - 1 - (c.dlevel - c.base));
   }
   hipGraphExec_t x = c.exec[__builtin_ctz((unsigned)c.W)];
-  for (int i = 0; i < B; ++i) HIP_CHK(hipGraphLaunch(x, c.st));
+  const int win = s->admit_window;
+  if (win > 0 && c.dlevel <= win) {
+    int n = 0;
+    if (locked) {
+      if (int rc = admit_into_cohort(s, c, c.dlevel, &n)) return rc;
+    } else {
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (int rc = admit_into_cohort(s, c, c.dlevel, &n)) return rc;
+    }
+    if (n > 0) {
+      const HubSplit hsp{s->hc.deg, s->hc.tie, s->hc.cut};
+      HIP_CHK(launch_expand_dd(c.W, s->g->d_beg, s->g->d_adj, s->g->d_tadj, c.ring, c.flags, s->R,
+                               s->g->Vt, c.ctl, c.d_plan, c.hs, c.q, s->chunk_shift, hsp, s->grid,
+                               c.st));
+      HIP_CHK(launch_admit_dd(c.W, c.d_al, c.d_plan, c.ring, s->R, s->g->Vt, c.ctl, c.flags, c.st));
+      HIP_CHK(launch_inspect(c.W, plan_args(s, c), c.st));
+      HIP_CHK(launch_plan_decide(c.d_plan, c.ctl, s->R, c.hs, c.W,
+                                 split_hubs(s->g, s->hc, s->cfg), c.st));
+    } else {
+      HIP_CHK(hipGraphLaunch(x, c.st));
+    }
+  } else {
+    for (int i = 0; i < B; ++i) HIP_CHK(hipGraphLaunch(x, c.st));
+  }
   HIP_CHK(hipMemcpyAsync(c.h_plan, c.d_plan, offsetof(Plan, dirlog), hipMemcpyDeviceToHost, c.st));
   HIP_CHK(hipEventRecord(c.ev, c.st));
   ++s->levels_issued;
@@ -607,7 +686,7 @@ int start_cohort(ibfs_server_s *s, SCtx &c) {
   c.dlevel = 1;
   c.base = 0;
   c.running = true;
-  return cohort_burst(s, c);
+  return cohort_burst(s, c, true);
 }
 
 // A cohort's event completed: more levels, the emission of all its lanes once
@@ -627,7 +706,7 @@ int cohort_step(ibfs_server_s *s, SCtx &c) {
     }
     c.dlevel = c.h_plan->level;
     if (c.dlevel > s->g->Vt + 2) return fail(IBFS_ERR_STATE, "cohort exceeded V+2 levels");
-    return cohort_burst(s, c);
+    return cohort_burst(s, c, false);
   }
   std::lock_guard<std::mutex> lk(s->mu);
   for (auto &l : c.lanes) {
@@ -667,14 +746,24 @@ void worker_main(ibfs_server_s *s) {
         {
           std::lock_guard<std::mutex> lk(s->mu);
           have = !s->pending.empty();
-          if (have) rc = s->admit_window == 0 ? start_cohort(s, c) : start_ctx(s, c);
+          if (have) {
+            c.clk.restart();
+            rc = s->host_levels ? start_ctx(s, c) : start_cohort(s, c);
+          }
         }
         progressed |= have;
       } else {
         hipError_t q = hipEventQuery(c.ev);
-        if (q == hipErrorNotReady) continue;
-        if (q != hipSuccess) rc = fail(IBFS_ERR_HIP, "server event: %s", hipGetErrorString(q));
-        else rc = c.coh ? cohort_step(s, c) : process_level(s, c);
+        if (q == hipErrorNotReady) {
+          if (!c.clk.expired()) continue;
+          rc = fail(IBFS_ERR_STATE, "server context did not progress within the wait limit (%.1f s)",
+                    wait_timeout_s());
+        } else if (q != hipSuccess) {
+          rc = fail(IBFS_ERR_HIP, "server event: %s", hipGetErrorString(q));
+        } else {
+          c.clk.restart();
+          rc = c.coh ? cohort_step(s, c) : process_level(s, c);
+        }
         progressed = true;
       }
       if (rc) {
@@ -778,6 +867,24 @@ int ibfs_server_set_admission(ibfs_server_t s, int32_t window_levels) {
   if (window_levels < 0) return fail(IBFS_ERR_ARG, "window_levels must be >= 0");
   std::lock_guard<std::mutex> lk(s->mu);
   s->admit_window = window_levels;
+  return IBFS_OK;
+}
+
+int ibfs_server_set_levels(ibfs_server_t s, int32_t host_levels) {
+  if (!s) return fail(IBFS_ERR_ARG, "null server");
+  if (host_levels != 0 && host_levels != 1) return fail(IBFS_ERR_ARG, "host_levels must be 0 or 1");
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->host_levels = host_levels;
+  return IBFS_OK;
+}
+
+int ibfs_server_counters(ibfs_server_t s, int64_t *levels, int64_t *admitted,
+                         int64_t *admitted_in_flight) {
+  if (!s) return fail(IBFS_ERR_ARG, "null server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (levels) *levels = s->levels_issued;
+  if (admitted) *admitted = s->lanes_admitted;
+  if (admitted_in_flight) *admitted_in_flight = s->lanes_admitted_in_flight;
   return IBFS_OK;
 }
 

@@ -130,6 +130,14 @@ int ibfs_abi_version(void);
 int ibfs_last_error(char *buf, size_t n);
 int ibfs_config_default(ibfs_config *cfg);
 int ibfs_device_count(int *n);
+/* Limit, in seconds, of every host wait of the library on device work
+ * (process-wide; default 120).  A wait that exceeds it fails with
+ * IBFS_ERR_STATE and a message naming the wait, instead of spinning on a
+ * stream that never drains; the object it happened on should then be
+ * destroyed.  The reference's host loop spins on a mapped flag with no limit
+ * (traversal.cu:101-130). */
+int ibfs_set_wait_timeout(double seconds);
+int ibfs_get_wait_timeout(double *seconds);
 
 /* ---- device memory helpers (callers without a GPU framework) ------------- */
 int ibfs_device_alloc(int device, size_t bytes, void **dptr);
@@ -292,6 +300,20 @@ int ibfs_server_create(ibfs_graph_t g, const ibfs_config *cfg, int64_t status_by
  * large value admits at every level boundary.  Default 0.  No reference
  * counterpart (serving). */
 int ibfs_server_set_admission(ibfs_server_t s, int32_t window_levels);
+/* Level driver of the server's contexts.  host_levels = 0 (default): a
+ * cohort's levels run on the device as captured level graphs (direction
+ * decided on the device, one host poll per burst of levels); inside the
+ * admission window the host takes every level boundary and waiting sources
+ * join the running cohort there (a device-side admission list, applied
+ * between the level's expansion and inspection); all lanes of a cohort are
+ * emitted when it ends.  host_levels = 1: every level boundary on the host;
+ * lanes whose BFS ended are emitted and freed one by one and reused two
+ * levels later.  No reference counterpart (serving). */
+int ibfs_server_set_levels(ibfs_server_t s, int32_t host_levels);
+/* Counters since creation: level bursts/levels issued, sources admitted, and
+ * sources that joined a cohort already running (after its first level). */
+int ibfs_server_counters(ibfs_server_t s, int64_t *levels, int64_t *admitted,
+                         int64_t *admitted_in_flight);
 int ibfs_server_contexts(ibfs_server_t s, int32_t *contexts, int32_t *lanes);
 int ibfs_server_submit(ibfs_server_t s, const int32_t *src, int32_t n,
                        int32_t *depth_out, int64_t *id);

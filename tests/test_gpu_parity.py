@@ -310,16 +310,18 @@ def test_server_mixed_equals_alone_s16(s16, lane_bits):
     alone.close()
 
 
+@pytest.mark.parametrize("host_levels", [True, False])
 @pytest.mark.parametrize("ring,window", [(8, 1), (9, 0), (32, 1), (8, 1 << 20), (32, 3)])
-def test_server_ring_flush_and_recycling(ring, window):
-    say(f"[gpu] server ring={ring} admit_window={window}")
+def test_server_ring_flush_and_recycling(ring, window, host_levels):
+    say(f"[gpu] server ring={ring} admit_window={window} host_levels={host_levels}")
     z = load_golden("bfs_classes1500.npz")
     g = ibfs_amd.DeviceGraph(ibfs_amd.HostCsr.from_arrays(z["beg_pos"], z["adj"]))
     V = len(z["beg_pos"]) - 1
     rng = np.random.default_rng(ring)
     queries = [rng.integers(0, V, size=int(n)).astype(np.int32) for n in (1, 7, 64, 3, 100, 2, 40)]
     # 64 lanes < 217 sources
-    srv = ibfs_amd.Server(g, lane_bits=64, ring_levels=ring, admit_window=window)
+    srv = ibfs_amd.Server(g, lane_bits=64, ring_levels=ring, admit_window=window,
+                          host_levels=host_levels)
     outs, infos = _serve(srv, queries, V, gap_s=0.001)
     for q, o in zip(queries, outs):
         np.testing.assert_array_equal(o, z["depth"][q])
@@ -541,8 +543,9 @@ def test_server_cohort_widths_and_window_changes(s16):
     pool = s16["h"].gen_sources(600, seed=21)
     srv = ibfs_amd.Server(s16["g"], lane_bits=256, ring_levels=8)
     off = 0
-    for window in (0, 3, 0, 1, 0):
+    for window, host in ((0, False), (3, True), (0, False), (1, True), (2, False), (0, True)):
         srv.set_admission(window)
+        srv.set_levels(host)
         sizes = (1, 100, 200) if window == 0 else (5, 40)
         queries = []
         for n in sizes:
@@ -551,6 +554,64 @@ def test_server_cohort_widths_and_window_changes(s16):
         outs, _ = _serve(srv, queries, V, gap_s=0.002)
         for q, o in zip(queries, outs):
             np.testing.assert_array_equal(o, oracle.bfs_many(s16["beg"], s16["adj"], q, threads=8))
+    srv.close()
+
+
+@pytest.mark.parametrize("lane_bits", [64, 256])
+def test_server_sources_join_running_device_cohorts(s16, lane_bits):
+    """Device-driven cohorts with an admission window: queries that arrive
+    while a cohort runs its first levels join it at a level boundary (the
+    device-side admission list between that level's expansion and
+    inspection); their depths are relative to their own admission level and
+    equal the oracle's, and the same queries alone give the same arrays."""
+    say(f"[gpu] server in-flight admission lanes={lane_bits}")
+    V = s16["h"].vert_count
+    pool = s16["h"].gen_sources(3000, seed=31)
+    rng = np.random.default_rng(lane_bits)
+    queries = [rng.choice(pool, size=int(n), replace=False).astype(np.int32)
+               for n in (1, 3, 2, 9, 1, 17, 4, 2, 30, 5, 1, 12)]
+    srv = ibfs_amd.Server(s16["g"], lane_bits=lane_bits, admit_window=3, ring_levels=16)
+    outs, infos = _serve(srv, queries, V, gap_s=0.0002)
+    for q, o in zip(queries, outs):
+        np.testing.assert_array_equal(o, oracle.bfs_many(s16["beg"], s16["adj"], q, threads=8))
+    cnt = srv.counters()
+    assert cnt["admitted"] == sum(len(q) for q in queries)
+    srv.close()
+    alone = ibfs_amd.Server(s16["g"], lane_bits=lane_bits, admit_window=3, ring_levels=16)
+    o2, _ = _serve(alone, [queries[5]], V)
+    np.testing.assert_array_equal(o2[0], outs[5])
+    alone.close()
+
+
+def test_server_in_flight_admission_on_a_long_path():
+    """A path of 4,000 vertices: a cohort started by a source at one end runs
+    for thousands of levels, so the queries submitted after it has started
+    all join it in flight (unbounded admission window) at different levels,
+    across many ring flushes (16 snapshots).  Depths are |i - s| exactly."""
+    say("[gpu] server in-flight admission, path graph")
+    V = 4000
+    edges = [(i, i + 1) for i in range(V - 1)]
+    beg, adj = symmetric_csr_from_edges(V, edges)
+    g = ibfs_amd.DeviceGraph(ibfs_amd.HostCsr.from_arrays(beg, adj))
+    srv = ibfs_amd.Server(g, lane_bits=64, admit_window=1 << 20, ring_levels=16)
+    first = np.array([0], dtype=np.int32)
+    b0 = ibfs_amd.DeviceBuffer(4 * V)
+    q0 = srv.submit(first, b0)
+    import time
+    t0 = time.perf_counter()
+    while srv.counters()["admitted"] < 1:
+        assert time.perf_counter() - t0 < 30, "first query never admitted"
+        time.sleep(0.0001)
+    rng = np.random.default_rng(9)
+    queries = [rng.integers(0, V, size=int(n)).astype(np.int32) for n in (1, 5, 2, 9, 3)]
+    outs, _ = _serve(srv, queries, V, gap_s=0.001)
+    srv.wait(q0, timeout_ms=60000)
+    idx = np.arange(V)
+    np.testing.assert_array_equal(b0.to_host(), idx)
+    for q, o in zip(queries, outs):
+        np.testing.assert_array_equal(o, np.abs(idx[None, :] - q[:, None]))
+    cnt = srv.counters()
+    assert cnt["admitted_in_flight"] == sum(len(q) for q in queries), cnt
     srv.close()
 
 
@@ -662,3 +723,30 @@ def test_engines_on_threads_share_a_graph(s16):
         for rep in range(2):
             np.testing.assert_array_equal(got[i][rep], want[i], err_msg=f"engine {i} run {rep}")
         assert edges[i][0] == edges[i][1] > 0
+
+
+def test_wait_limit_ends_a_wait_and_the_engine_recovers():
+    """A host wait that exceeds the library's wait limit returns
+    IBFS_ERR_STATE instead of spinning; every kernel still runs to its end
+    (destroying the engine drains its stream), and with the limit restored a
+    new engine gives the oracle's depths.  The condition is forced from the
+    host with a 50-microsecond limit on a traversal of several milliseconds."""
+    say("[gpu] wait limit")
+    h = ibfs_amd.HostCsr.rmat(20, 16, seed=3)
+    beg, adj = h.arrays()
+    g = ibfs_amd.DeviceGraph(h)
+    src = h.gen_sources(512, seed=4)
+    old = ibfs_amd.get_wait_timeout()
+    eng = ibfs_amd.Engine(g, lane_bits=512)
+    try:
+        ibfs_amd.set_wait_timeout(50e-6)
+        with pytest.raises(ibfs_amd.IbfsError) as ei:
+            eng.traversal(src)
+        assert ei.value.code == ibfs_amd._lib.IBFS_ERR_STATE
+        assert "wait limit" in str(ei.value)
+    finally:
+        ibfs_amd.set_wait_timeout(old)
+        eng.close()
+    eng = ibfs_amd.Engine(g, lane_bits=512)
+    got = eng.traversal(src[:16])
+    np.testing.assert_array_equal(got, oracle.bfs_many(beg, adj, src[:16], threads=8))

@@ -205,3 +205,57 @@ def test_cli_usage_and_errors(tmp_path):
     r = subprocess.run([str(cli), str(tmp_path / "nope"), str(tmp_path / "nope2"), "128", "2",
                         "128"], capture_output=True, text=True)
     assert r.returncode != 0 and "cannot open" in r.stdout
+
+
+def test_wait_timeout_setting():
+    """The process-wide limit of the library's host waits on device work."""
+    old = ibfs_amd.get_wait_timeout()
+    assert old == 120.0
+    try:
+        ibfs_amd.set_wait_timeout(2.5)
+        assert ibfs_amd.get_wait_timeout() == 2.5
+        for bad in (0.0, -1.0, float("nan"), 1e9):
+            with pytest.raises(ibfs_amd.IbfsError) as ei:
+                ibfs_amd.set_wait_timeout(bad)
+            assert ei.value.code == _lib.IBFS_ERR_ARG
+        assert ibfs_amd.get_wait_timeout() == 2.5
+        assert _lib.lib().ibfs_get_wait_timeout(None) == _lib.IBFS_ERR_ARG
+    finally:
+        ibfs_amd.set_wait_timeout(old)
+
+
+def test_row_ranges_first_fit_and_coalescing():
+    """Bookkeeping of the recycled output rows of served queries."""
+    r = ibfs_amd.RowRanges(16)
+    a, b, c = r.take(4), r.take(8), r.take(4)
+    assert (a, b, c) == (0, 4, 12) and r.take(1) is None and not r.fits(1)
+    r.give(b, 8)
+    assert r.free_ranges == [(4, 8)] and r.take(9) is None
+    d = r.take(3)
+    assert d == 4 and r.free_ranges == [(7, 5)]
+    r.give(a, 4)
+    r.give(c, 4)
+    assert r.free_ranges == [(0, 4), (7, 9)]
+    r.give(d, 3)  # closes both gaps
+    assert r.free_ranges == [(0, 16)]
+    with pytest.raises(ValueError):
+        r.give(2, 3)  # already free
+    with pytest.raises(ValueError):
+        r.give(15, 2)  # outside
+    with pytest.raises(ValueError):
+        r.take(17)
+    # random alloc/free sequences never hand out overlapping rows
+    rng = np.random.default_rng(5)
+    r = ibfs_amd.RowRanges(64)
+    held = []
+    for _ in range(2000):
+        if held and (rng.random() < 0.5 or not r.fits(1)):
+            r0, n = held.pop(rng.integers(len(held)))
+            r.give(r0, n)
+        else:
+            n = int(rng.integers(1, 9))
+            r0 = r.take(n)
+            if r0 is not None:
+                assert all(r0 + n <= x or x + m <= r0 for x, m in held)
+                held.append((r0, n))
+        assert sum(m for _, m in held) + sum(c for _, c in r.free_ranges) == 64
