@@ -113,7 +113,7 @@ int tune(ibfs_graph_t g, const int32_t *src, int64_t n, const ibfs_config *start
       {"sml_shed", &ibfs_config::sml_shed, nullptr, {8, 16, 32, 64}},
       {"lrg_shed", &ibfs_config::lrg_shed, nullptr, {128, 256, 512, 1024, 4096}},
       {"td_chunk", &ibfs_config::td_chunk, nullptr, {1024, 2048, 4096, 8192}},
-      {"emit_mode", &ibfs_config::emit_mode, nullptr, {0, 1, 2}},
+      {"emit_mode", &ibfs_config::emit_mode, nullptr, {0, 1, 2, 3}},
       {"inspect_vpt", &ibfs_config::inspect_vpt, nullptr, {0, 1, 2, 4, 8}},
       {"wg_size", &ibfs_config::wg_size, nullptr, {256, 512, 1024}},
       {"hybrid_gamma", &ibfs_config::hybrid_gamma, nullptr, {0, 8, 16, 24, 32}},

@@ -328,7 +328,7 @@ int ibfs_batch_create(ibfs_graph_t g, const ibfs_config *cfg_in, int32_t context
     return fail(IBFS_ERR_ARG, "td_chunk must be a power of two >= 256");
   if (cfg.ring_levels < 4 || cfg.ring_levels > 255)
     return fail(IBFS_ERR_ARG, "ring_levels must be in [4, 255]");
-  if (cfg.emit_mode < 0 || cfg.emit_mode > 2) return fail(IBFS_ERR_ARG, "bad emit_mode");
+  if (cfg.emit_mode < 0 || cfg.emit_mode > 3) return fail(IBFS_ERR_ARG, "bad emit_mode");
   if (!valid_inspect_vpt(cfg.inspect_vpt)) return fail(IBFS_ERR_ARG, "bad inspect_vpt");
   if (!valid_wg_size(cfg.wg_size)) return fail(IBFS_ERR_ARG, "wg_size must be 256, 512 or 1024");
   if (cfg.device != g->device) return fail(IBFS_ERR_ARG, "config device differs from the graph's");

@@ -857,8 +857,8 @@ int ibfs_engine_create(ibfs_graph_t g, const ibfs_config *cfg_in, ibfs_engine_t 
   if (int rc = check_direction_config(cfg)) return rc;
   if (cfg.td_chunk < 256 || (cfg.td_chunk & (cfg.td_chunk - 1)))
     return fail(IBFS_ERR_ARG, "td_chunk must be a power of two >= 256 (got %d)", cfg.td_chunk);
-  if (cfg.emit_mode < 0 || cfg.emit_mode > 2)
-    return fail(IBFS_ERR_ARG, "emit_mode must be 0, 1 or 2 (got %d)", cfg.emit_mode);
+  if (cfg.emit_mode < 0 || cfg.emit_mode > 3)
+    return fail(IBFS_ERR_ARG, "emit_mode must be 0, 1, 2 or 3 (got %d)", cfg.emit_mode);
   if (cfg.ring_levels < 4 || cfg.ring_levels > 255)
     return fail(IBFS_ERR_ARG, "ring_levels must be in [4, 255] (got %d)", cfg.ring_levels);
   if (!valid_inspect_vpt(cfg.inspect_vpt))

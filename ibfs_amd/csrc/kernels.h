@@ -241,7 +241,8 @@ hipError_t launch_plan_decide(Plan *plan, const LevelCtl *ctl_ring, int R, const
 // Vt*W words).  level_masks[(L-b)*W + w] = lanes admitted at or before L.
 // mode 0: one vertex per thread, per-lane 4-byte stores; mode 1: LDS-tiled,
 // 16-byte stores when vec4 (all lane outputs 16-byte aligned, V % 4 == 0);
-// mode 2: as 0 with non-temporal stores.
+// mode 2: as 0 with non-temporal stores; mode 3: a wave per status word of
+// 256 vertices, 16-byte stores (1-KB pieces of each lane's row).
 // edges_out (final window only, or null): += sum over v of deg(v) * |S_e[v] &
 // sel|, the TEPS numerator of the selected lanes, counted during the pass.
 hipError_t launch_emit(int W, int mode, int vec4, const u64 *ring, int R, EmitSpace sp, int b,

@@ -217,6 +217,134 @@ __global__ __launch_bounds__(BS) void k_emit_tiled(const u64 *__restrict__ ring,
   }
 }
 
+// Wide-store depth emission: same values as k_emit.  A wave owns one status
+// word w (64 lanes) of 256 consecutive vertices, four per thread, so its
+// per-vertex state is NS+2 words of those four vertices and every store
+// instruction writes 16 bytes per thread: one lane's 256 depths as a single
+// 1-KB contiguous piece of its row (k_emit writes 256-byte pieces, one
+// vertex per thread).  The waves of a workgroup take consecutive (chunk,
+// word) units, so the words of one vertex are read by neighbouring waves
+// from the same lines.  A lane row that is not 16-byte aligned, a chunk
+// that ends past V, and a piece holding an entry a continuation window must
+// not touch fall back to 4-byte stores.
+template <int NS>
+__device__ __forceinline__ int32_t lane_depth4(const u64 (&c)[NS][4], const u64 (&fin)[4],
+                                               const u64 (&bs)[4], int i, int bit, int off,
+                                               bool iso, bool is_src) {
+  int cnt = 0;
+#pragma unroll
+  for (int j = 0; j < NS; ++j) cnt |= (int)((c[j][i] >> bit) & 1ull) << j;
+  const bool set = iso ? is_src : (bool)((fin[i] >> bit) & 1ull);
+  const bool at_base = iso ? is_src : (bool)((bs[i] >> bit) & 1ull);
+  if (off <= 0) return set ? cnt : -1;
+  return (set && !at_base) ? off + cnt : SKIP;
+}
+
+template <int W, int NS>
+__global__ __launch_bounds__(BS) void k_emit_wide(const u64 *__restrict__ ring, int R,
+                                                  EmitSpace sp, int b, int e,
+                                                  const LaneOut *__restrict__ lanes,
+                                                  const u64 *__restrict__ level_masks,
+                                                  const u64 *__restrict__ sel, int k_lanes,
+                                                  u64 *edges_out) {
+  __shared__ int32_t *s_out[MAX_LANES];
+  __shared__ int32_t s_off[MAX_LANES];
+  __shared__ int32_t s_src[MAX_LANES];
+  __shared__ u64 red[BS / 64];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  for (int i = t; i < k_lanes; i += BS) {
+    const bool on = (sel[i >> 6] >> (i & 63)) & 1ull;
+    s_out[i] = on ? lanes[i].out : nullptr;
+    s_off[i] = b - lanes[i].admit;
+    s_src[i] = lanes[i].src;
+  }
+  __syncthreads();
+  const int64_t V = sp.V, slot = sp.Vt * W;
+  const bool v4 = (V & 3) == 0;
+  const int64_t units = ((V + 255) >> 8) * W;
+  u64 acc = 0;
+  for (int64_t u = (int64_t)blockIdx.x * (BS / 64) + wv; u < units;
+       u += (int64_t)gridDim.x * (BS / 64)) {
+    const int64_t chunk = u / W;
+    const int w = (int)(u % W);
+    const int64_t v0 = chunk * 256 + 4 * lane;
+    int32_t ci[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) ci[i] = (v0 + i < V) ? sp.cid[v0 + i] : -1;
+    u64 c[NS][4], fin[4], bs[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      fin[i] = bs[i] = 0;
+#pragma unroll
+      for (int j = 0; j < NS; ++j) c[j][i] = 0;
+    }
+    const u64 *Sw = ring + w;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (ci[i] >= 0) {
+        fin[i] = Sw[(int64_t)(e % R) * slot + (int64_t)ci[i] * W];
+        bs[i] = Sw[(int64_t)(b % R) * slot + (int64_t)ci[i] * W];
+      }
+    }
+    for (int L = b; L <= e; ++L) {
+      const u64 lm = level_masks[(int64_t)(L - b) * W + w];
+      const u64 *S = Sw + (int64_t)(L % R) * slot;
+      u64 x[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) x[i] = ci[i] >= 0 ? S[(int64_t)ci[i] * W] : ~0ull;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        u64 carry = ~x[i] & lm;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+          const u64 nc = c[j][i] & carry;
+          c[j][i] ^= carry;
+          carry = nc;
+        }
+      }
+    }
+    if (edges_out) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (ci[i] >= 0) acc += (u64)__popcll(fin[i] & sel[w]) * (u64)sp.deg[ci[i]];
+    }
+    for (int bit = 0; bit < 64; ++bit) {
+      const int k = w * 64 + bit;
+      if (k >= k_lanes) break;
+      int32_t *out = s_out[k];
+      if (!out) continue;
+      const int off = s_off[k];
+      const int32_t src = s_src[k];
+      int32_t d[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        d[i] = lane_depth4<NS>(c, fin, bs, i, bit, off, ci[i] < 0, src == (int32_t)(v0 + i));
+      gint32 *o = (gint32 *)out;
+      const bool full = v4 && ((reinterpret_cast<uintptr_t>(out) & 15) == 0) && v0 + 3 < V &&
+                        d[0] != SKIP && d[1] != SKIP && d[2] != SKIP && d[3] != SKIP;
+      if (full) {
+        typedef int v4i __attribute__((ext_vector_type(4)));
+        v4i q = {d[0], d[1], d[2], d[3]};
+        *reinterpret_cast<__attribute__((address_space(1))) v4i *>(o + v0) = q;
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          if (v0 + i < V && d[i] != SKIP) o[v0 + i] = d[i];
+      }
+    }
+  }
+  if (edges_out) {  // uniform: one atomic per workgroup
+    acc = wave_sum(acc);
+    if (lane == 0) red[wv] = acc;
+    __syncthreads();
+    if (t == 0) {
+      u64 s = 0;
+      for (int i = 0; i < BS / 64; ++i) s += red[i];
+      if (s) atomicAdd(edges_out, s);
+    }
+  }
+}
+
 // Closeness reduction over the snapshot window [b, e]: per lane k, the sum
 // over vertices of depth_k(v) and the number of vertices reached, for the
 // vertices whose bit arrived in the window, without writing depth arrays.
@@ -423,6 +551,11 @@ static void emit_ns(int mode, int vec4, const u64 *ring, int R, EmitSpace sp, in
       hipLaunchKernelGGL(k_count_edges<W>, gc, blk, 0, st, ring + (int64_t)(e % R) * sp.Vt * W,
                          sp.deg, sp.Vt, sel, edges_out);
     }
+  } else if (mode == 3) {
+    const int64_t units = ((sp.V + 255) >> 8) * W;
+    dim3 g(grid_for(units, BS / 64, IBFS_EMIT_GRID_CAP)), blk(BS);
+    hipLaunchKernelGGL((k_emit_wide<W, NS>), g, blk, 0, st, ring, R, sp, b, e, lanes, masks, sel,
+                       k, edges_out);
   } else {
     dim3 g(grid_for(sp.V, BS, IBFS_EMIT_GRID_CAP)), blk(BS);
     hipLaunchKernelGGL((k_emit<W, NS>), g, blk, 0, st, ring, R, sp, b, e, lanes, masks, sel, k,

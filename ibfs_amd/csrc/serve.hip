@@ -111,6 +111,7 @@ struct SCtx {
   int W = 0, K = 0;  // current width: words per vertex and lanes (<= the server's)
   int coh = 0;     // 0: host-driven levels, 1: cohort levels on the device, 2: cohort emitting
   int dlevel = 0;  // cohort level the device expands next (host view)
+  int last_admit = 0;  // latest level at which sources joined the cohort
   WaitClock clk;   // since the context's last completed event (wait limit)
 };
 
@@ -567,6 +568,7 @@ int admit_into_cohort(ibfs_server_s *s, SCtx &c, int L, int *n_adm) {
   if (n == 0) return IBFS_OK;
   c.h_al->level = L;
   c.h_al->n = n;
+  c.last_admit = L;
   s->lanes_admitted += n;
   s->lanes_admitted_in_flight += n;
   HIP_CHK(hipMemcpyAsync(c.d_al, c.h_al, offsetof(AdmitList, src) + sizeof(int32_t) * (size_t)n,
@@ -684,6 +686,7 @@ int start_cohort(ibfs_server_s *s, SCtx &c) {
   HIP_CHK(launch_plan_decide(c.d_plan, c.ctl, s->R, c.hs, W, split_hubs(s->g, s->hc, s->cfg), c.st));
   c.coh = 1;
   c.dlevel = 1;
+  c.last_admit = 0;
   c.base = 0;
   c.running = true;
   return cohort_burst(s, c, true);
@@ -705,7 +708,9 @@ int cohort_step(ibfs_server_s *s, SCtx &c) {
       return IBFS_OK;
     }
     c.dlevel = c.h_plan->level;
-    if (c.dlevel > s->g->Vt + 2) return fail(IBFS_ERR_STATE, "cohort exceeded V+2 levels");
+    // a BFS admitted at level a ends by level a + Vt + 1
+    if (c.dlevel > c.last_admit + s->g->Vt + 2)
+      return fail(IBFS_ERR_STATE, "cohort exceeded V+2 levels after its last admission");
     return cohort_burst(s, c, false);
   }
   std::lock_guard<std::mutex> lk(s->mu);
@@ -795,7 +800,7 @@ int ibfs_server_create(ibfs_graph_t g, const ibfs_config *cfg_in, int64_t status
     return fail(IBFS_ERR_ARG, "serving needs ring_levels in [8, 255]");
   if (cfg.td_chunk < 256 || (cfg.td_chunk & (cfg.td_chunk - 1)))
     return fail(IBFS_ERR_ARG, "td_chunk must be a power of two >= 256");
-  if (cfg.emit_mode < 0 || cfg.emit_mode > 2) return fail(IBFS_ERR_ARG, "bad emit_mode");
+  if (cfg.emit_mode < 0 || cfg.emit_mode > 3) return fail(IBFS_ERR_ARG, "bad emit_mode");
   if (!valid_inspect_vpt(cfg.inspect_vpt)) return fail(IBFS_ERR_ARG, "bad inspect_vpt");
   if (!valid_wg_size(cfg.wg_size)) return fail(IBFS_ERR_ARG, "wg_size must be 256, 512 or 1024");
   if (int rc = check_direction_config(cfg)) return rc;

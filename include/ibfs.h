@@ -69,8 +69,10 @@ typedef struct ibfs_config {
   int32_t device;      /* HIP device ordinal                                   */
   int32_t work_model;  /* 1: measure each bottom-up level's early-termination
                           prefix into the level log (instrumentation)        */
-  int32_t emit_mode;   /* depth emission: 0 per-thread stores, 1 LDS-tiled,
-                          2 per-thread non-temporal stores                   */
+  int32_t emit_mode;   /* depth emission: 0 one vertex per thread, 4-byte
+                          stores; 1 LDS-tiled; 2 as 0 with non-temporal
+                          stores; 3 one status word of 256 vertices per wave,
+                          1-KB pieces of each lane's row per store           */
   int32_t dd_grid;     /* workgroups of the device-driven expander (0: auto)  */
   int32_t inspect_vpt; /* inspector vertices per lane and tile: 1, 2, 4, 8 or
                           16, capped at 16 words per lane (0: by size)       */

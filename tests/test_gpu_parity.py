@@ -170,7 +170,7 @@ def test_ring_wrap_long_path(ring):
 
 
 @pytest.mark.parametrize("hubs", [0, 3, 1000, None])
-@pytest.mark.parametrize("emit_mode", [0, 1, 2])
+@pytest.mark.parametrize("emit_mode", [0, 1, 2, 3])
 def test_hub_cache_and_emission_variants(hubs, emit_mode):
     say(f"[gpu] hubs={hubs} emit_mode={emit_mode}")
     for name in ("bfs_classes1500.npz", "bfs_er300.npz", "bfs_path10.npz"):
@@ -184,6 +184,44 @@ def test_hub_cache_and_emission_variants(hubs, emit_mode):
                 src = np.arange(0, V, max(1, V // 200), dtype=np.int32)
                 np.testing.assert_array_equal(eng.run_groups(src), z["depth"][src],
                                               err_msg=f"{name} {pol} ring={ring}")
+
+
+@pytest.mark.parametrize("lane_bits", [64, 128, 256, 512])
+def test_wide_emission_widths_and_row_alignment(s16, lane_bits):
+    """Emission mode 3 (a wave per status word of 256 vertices, 16-byte
+    stores): every width, rows 16-byte aligned and not (a device output at a
+    4-byte offset takes the 4-byte fallback), vertex counts not divisible by
+    4 (golden graphs of 9, 42 and 1,500 vertices), ring wrap (continuation
+    windows skip entries), the engine, the batched launcher and the server."""
+    say(f"[gpu] wide emission lanes={lane_bits}")
+    V = s16["h"].vert_count
+    n = len(s16["src"])
+    buf = ibfs_amd.DeviceBuffer(4 * n * V + 16)
+    for off in (0, 4):
+        view = ibfs_amd.DeviceView(buf, off, 4 * n * V)
+        eng = ibfs_amd.Engine(s16["g"], lane_bits=lane_bits, emit_mode=3)
+        eng.run_groups(s16["src"], out=view)
+        np.testing.assert_array_equal(view.to_host().reshape(n, V), s16["want"], err_msg=f"off {off}")
+        b = ibfs_amd.Batch(s16["g"], contexts=2, burst=4, lane_bits=lane_bits, emit_mode=3)
+        b.run(s16["src"], out=view)
+        np.testing.assert_array_equal(view.to_host().reshape(n, V), s16["want"], err_msg=f"off {off}")
+    for name in ("bfs_cycle9.npz", "bfs_grid6x7.npz", "bfs_classes1500.npz"):
+        z = load_golden(name)
+        g = ibfs_amd.DeviceGraph(ibfs_amd.HostCsr.from_arrays(z["beg_pos"], z["adj"]))
+        Vg = len(z["beg_pos"]) - 1
+        src = np.arange(Vg, dtype=np.int32)
+        for ring in (4, 32):
+            eng = ibfs_amd.Engine(g, lane_bits=lane_bits, emit_mode=3, ring_levels=ring)
+            np.testing.assert_array_equal(eng.run_groups(src), z["depth"], err_msg=f"{name} {ring}")
+            b = ibfs_amd.Batch(g, contexts=2, burst=3, lane_bits=lane_bits, emit_mode=3,
+                               ring_levels=ring)
+            np.testing.assert_array_equal(b.run(src), z["depth"], err_msg=f"{name} {ring}")
+        srv = ibfs_amd.Server(g, lane_bits=lane_bits, emit_mode=3, ring_levels=8, admit_window=2)
+        queries = [src[i::7] for i in range(7)]
+        outs, _ = _serve(srv, queries, Vg, gap_s=0.0005)
+        for q, o in zip(queries, outs):
+            np.testing.assert_array_equal(o, z["depth"][q], err_msg=name)
+        srv.close()
 
 
 @pytest.mark.parametrize("vpt", [1, 4, 8, 16])
