@@ -326,6 +326,19 @@ class DeviceGraph:
         check(lib().ibfs_graph_hub_entries(self._h, top, C.byref(n)))
         return n.value
 
+    def traversal_noalloc(self, sources, out, cfg: Optional[Config] = None) -> int:
+        """One group of <= 128 sources through ibfs_traversal_noalloc (no
+        allocation inside the library; scratch in a file-scope device array):
+        depth arrays into `out` (a device buffer or view of k*V int32);
+        returns the traversed-edge count."""
+        s = np.ascontiguousarray(sources, dtype=np.int32)
+        if out.nbytes < 4 * len(s) * self.vert_count:
+            raise ValueError("device buffer too small")
+        e = C.c_int64()
+        check(lib().ibfs_traversal_noalloc(self._h, C.byref(cfg) if cfg is not None else None,
+                                           _i32p(s), len(s), C.c_void_p(out.ptr), C.byref(e)))
+        return e.value
+
     @classmethod
     def load_files(cls, beg_file: str, csr_file: str, device: int = 0) -> "DeviceGraph":
         return cls(HostCsr.load(beg_file, csr_file), device)

@@ -129,6 +129,7 @@ SIGNATURES = {
     "ibfs_engine_level_log": (C.c_int, [P, C.POINTER(LevelInfo), C.c_int32,
                                         C.POINTER(C.c_int32)]),
     "ibfs_run_groups": (C.c_int, [P, I32P, C.c_int64, P, C.c_int32, C.POINTER(Stats)]),
+    "ibfs_traversal_noalloc": (C.c_int, [P, C.POINTER(Config), I32P, C.c_int32, P, I64P]),
     "ibfs_traversed_edges": (C.c_int, [P, I32P, I64P]),
     "ibfs_batch_create": (C.c_int, [P, C.POINTER(Config), C.c_int32, C.c_int32, C.POINTER(P)]),
     "ibfs_batch_run": (C.c_int, [P, I32P, C.c_int64, P, C.c_int32, C.POINTER(Stats)]),

@@ -16,7 +16,7 @@ OUT = PKG / "libibfs.so"
 CLI = PKG / "gpu-ibfs"
 
 HIP_SOURCES = ["kernels_traverse.hip", "kernels_emit.hip", "engine.hip", "batch.hip",
-               "serve.hip"]
+               "serve.hip", "noalloc.hip"]
 CXX_SOURCES = ["hostcsr.cpp", "autotune.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

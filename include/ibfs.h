@@ -249,6 +249,26 @@ int ibfs_traversal(ibfs_engine_t e, const int32_t *src, int32_t k,
 int ibfs_engine_level_log(ibfs_engine_t e, ibfs_level_info *out, int32_t cap,
                           int32_t *n_levels);
 
+/* Allocation-free variant of ibfs_traversal.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Runs one
+ * group of k <= 128 sources on graph g (cfg: null for the defaults;
+ * ring_levels and split_cov are not used) and writes depth_out, a device
+ * buffer of k*V int32, as ibfs_traversal does; *traversed_edges (optional)
+ * receives the TEPS numerator.  It calls no allocating routine: every
+ * scratch buffer lives in one file-scope device array of
+ * IBFS_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until
+ * the process ends, and kernels run on the null stream.  Limits, rejected
+ * with IBFS_ERR_ARG: at most 262,144 vertices with an edge, 2^24 adjacency
+ * entries, 131,072 hubs, td_chunk >= 2048.  Calls on one device share the
+ * array and its previous contents (what must start at zero is cleared by
+ * each call); the library serialises calls within a process, and two
+ * processes have separate copies of the array. */
+#define IBFS_NOALLOC_SCRATCH_BYTES 43324928
+int ibfs_traversal_noalloc(ibfs_graph_t g, const ibfs_config *cfg,
+                           const int32_t *src, int32_t k, int32_t *depth_out,
+                           int64_t *traversed_edges);
+
 /* Runs src_count sources as consecutive groups of lane_bits (the reference's
  * main loop, main.cu:76-90); depth_out is [src_count][V]. */
 int ibfs_run_groups(ibfs_engine_t e, const int32_t *src, int64_t src_count,
