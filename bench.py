@@ -146,6 +146,44 @@ def roofline_leg(g, src, K, V, tune, kern_ms, launches, steps):
     return roof, kernels, levels
 
 
+def batch_alg_bytes(g, beg, adj, src, opts):
+    """Algorithmic bytes per step of the batched form's kernels: the sources
+    grouped as the batched launcher packs them (descending two-hop size,
+    stable), each group traversed once by a work-model engine with the same
+    settings (the device plan picks the host rule's directions); expansion
+    = top-down + bottom-up + hybrid levels (one k_expand_dd launch each)."""
+    K = int(opts.get("lane_bits", 256))
+    deg = np.diff(beg)
+    key = np.array([int(deg[adj[beg[s]:beg[s + 1]]].sum()) for s in src], dtype=np.int64)
+    order = np.argsort(-key, kind="stable") if opts.get("pack_sources", 1) else np.arange(len(src))
+    psrc = src[order]
+    cfg = {k: v for k, v in opts.items() if k not in ("contexts", "burst")}
+    model = ibfs_amd.Engine(g, work_model=1, **cfg)
+    V, Vt = g.vert_count, g.traversal_vertices
+    scratch = ibfs_amd.DeviceBuffer(4 * K * V)
+    hub_frac = g.hub_entries(model.cfg.hybrid_hubs) / max(g.edge_count, 1)
+    alg = {"expand": 0, "inspect": 0, "emit": 0}
+    launches = {"expand": 0, "inspect": 0, "emit": 0}
+    for g0 in range(0, len(psrc), K):
+        grp = psrc[g0:g0 + K]
+        model.traversal(grp, out=scratch)
+        lv = model.level_log()
+        words = 1  # status words of this group: the narrowest width that holds it
+        while words * 64 < len(grp) and words * 64 < K:
+            words *= 2
+        b = alg_bytes_per_kernel(lv, V, Vt, 8 * words, len(grp), hub_frac)
+        alg["expand"] += b["expand_td"] + b["expand_bu"] + b["expand_hy"]
+        alg["inspect"] += b["inspect"]
+        alg["emit"] += b["emit"]
+        launches["expand"] += len(lv) - 1
+        launches["inspect"] += len(lv)
+        launches["emit"] += 1
+    model.close()
+    scratch.free()
+    return {k: {"alg_GB_per_step": round(alg[k] / 1e9, 4), "launches_per_step": launches[k]}
+            for k in alg}
+
+
 KERNEL_NAMES = {"expand_bu": "k_expand_bu", "expand_td": "k_expand_td", "expand_hy": "k_expand_hy",
                 "inspect": "k_inspect", "emit": "k_emit"}
 
@@ -385,6 +423,9 @@ def main() -> int:
     ap.add_argument("--alpha", type=int, default=None)
     ap.add_argument("--beta", type=int, default=None)
     ap.add_argument("--inspect-vpt", type=int, default=None)
+    ap.add_argument("--phase-offsets", type=int, default=None,
+                    help="batched leg: lanes start at levels aligned by two-hop size (1) or all "
+                         "at level 0 (0); default: the tuned setting")
     ap.add_argument("--batch-contexts", type=int, default=None)
     ap.add_argument("--burst", type=int, default=None)
     ap.add_argument("--no-batch", action="store_true")
@@ -406,6 +447,8 @@ def main() -> int:
         tune["beta"] = args.beta
     if args.inspect_vpt is not None:
         tune["inspect_vpt"] = args.inspect_vpt
+    if args.phase_offsets is not None:
+        tune["phase_offsets"] = args.phase_offsets
     if args.gpus != 1:
         print(f"bench.py measures one GPU: libibfs has no multi-GPU path (asked for "
               f"--gpus {args.gpus}); source groups would shard across GPUs with no exchange.",
@@ -525,6 +568,9 @@ def main() -> int:
                 return 4
             verified["batched"] = len(sample_rows)
         out_b.free()
+        if beg is None:
+            beg, adj = h.arrays()
+        batched["kernels_alg"] = batch_alg_bytes(g, beg, adj, src, batched["config"])
     if verified:
         log(f"device depth arrays verified bit-exact against the oracle: {verified}")
     out.free()

@@ -49,7 +49,7 @@ class Config(C.Structure):
         ("ring_levels", C.c_int32), ("kernel_timing", C.c_int32), ("device", C.c_int32),
         ("work_model", C.c_int32), ("emit_mode", C.c_int32), ("dd_grid", C.c_int32),
         ("inspect_vpt", C.c_int32), ("hybrid_gamma", C.c_int32), ("hybrid_hubs", C.c_int32),
-        ("pack_sources", C.c_int32), ("split_cov", C.c_int32),
+        ("pack_sources", C.c_int32), ("split_cov", C.c_int32), ("phase_offsets", C.c_int32),
     ]
 
 
@@ -112,6 +112,8 @@ SIGNATURES = {
                                     C.c_int32, C.POINTER(P)]),
     "ibfs_gen_sources": (C.c_int, [P, C.c_int64, C.c_uint64, I32P]),
     "ibfs_convert_text": (C.c_int, [C.c_char_p, C.c_int32, C.c_int64, C.POINTER(P)]),
+    "ibfs_convert_text_files": (C.c_int, [C.c_char_p, C.c_int32, C.c_int64, C.c_char_p, I64P,
+                                          I64P]),
     "ibfs_graph_create": (C.c_int, [I64P, C.c_int64, I32P, C.c_int64, C.c_int, C.POINTER(P)]),
     "ibfs_graph_opts_default": (C.c_int, [C.POINTER(GraphOpts)]),
     "ibfs_graph_create_ex": (C.c_int, [I64P, C.c_int64, I32P, C.c_int64, C.c_int,

@@ -14,6 +14,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 OUT = PKG / "libibfs.so"
 CLI = PKG / "gpu-ibfs"
+CONVERTER = PKG / "text-to-bin"
 
 HIP_SOURCES = ["kernels_traverse.hip", "kernels_emit.hip", "engine.hip", "batch.hip",
                "serve.hip", "noalloc.hip"]
@@ -57,6 +58,11 @@ def build_native(verbose: bool = False) -> Path:
     if cli_src.exists():
         _run([HIPCC, *COMMON, "-x", "c++", str(cli_src), "-o", str(CLI), f"-L{PKG}", "-libfs",
               f"-Wl,-rpath,$ORIGIN", "-lpthread"])
+    # the converter's command line over the C ABI
+    conv_src = CSRC / "text_to_bin_main.cpp"
+    if conv_src.exists():
+        _run([HIPCC, *COMMON, "-x", "c++", str(conv_src), "-o", str(CONVERTER), f"-L{PKG}",
+              "-libfs", f"-Wl,-rpath,$ORIGIN", "-lpthread"])
     if verbose:
         print(f"built {OUT}")
     return OUT

@@ -1,11 +1,13 @@
 // Batched launcher: an arbitrary source set packed into lane groups, several
 // traversal contexts in flight on their own HIP streams, each level a
-// captured hipGraph of two kernels (expand, inspect) whose direction is
-// decided on the device, so the host only polls once per burst of levels.
+// captured hipGraph of three kernels (expand, inspect, direction decision):
+// the direction is decided on the device, so the host only polls once per
+// burst of levels.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstddef>
 #include <cstring>
 #include <numeric>
@@ -51,6 +53,11 @@ struct Ctx {
   int level = 0;  // level the device will expand next (host view)
   int32_t *out_host = nullptr;    // host-placed output (rows of the caller's order)
   const int64_t *rows = nullptr;  // caller's index of each lane of the group
+  // lanes that start after level 0 (phase offsets): each lane's start level,
+  // the latest one, and the admission list staged for one level
+  std::vector<int> start;
+  int a_max = 0;
+  AdmitList *d_al = nullptr, *h_al = nullptr;
 };
 
 template <class T>
@@ -117,6 +124,8 @@ void release_ctx(Ctx &c) {
   dfree(c.d_lanes);
   hfree(c.h_lanes);
   dfree(c.d_stage);
+  dfree(c.d_al);
+  hfree(c.h_al);
   if (c.ev) (void)hipEventDestroy(c.ev);
   c.ev = nullptr;
   if (c.st) (void)hipStreamDestroy(c.st);
@@ -197,6 +206,8 @@ int alloc_ctx(ibfs_batch_s *b, Ctx &c) {
   H((void **)&c.h_src, sizeof(int32_t) * (size_t)b->K);
   A((void **)&c.d_lanes, sizeof(LaneOut) * (size_t)b->K);
   H((void **)&c.h_lanes, sizeof(LaneOut) * (size_t)b->K);
+  A((void **)&c.d_al, sizeof(AdmitList));
+  H((void **)&c.h_al, sizeof(AdmitList));
   if (err == hipSuccess) err = hipEventCreateWithFlags(&c.ev, hipEventDisableTiming);
   if (err != hipSuccess)
     return fail(err == hipErrorOutOfMemory ? IBFS_ERR_NOMEM : IBFS_ERR_HIP,
@@ -221,23 +232,65 @@ int alloc_ctx(ibfs_batch_s *b, Ctx &c) {
 // level-0 inspection (whose last workgroup plans level 1).
 // src: sources in grouping order; rows[i] = caller's index of lane i (its
 // depth row in dev_out); stage: host-placed output staging (lane i at row i).
+// Start level of each lane of a group (phase offsets p = 1..3): a lane whose
+// source's two-hop size is below the group's median by f^lag, where f =
+// d^(p/2) is the frontier growth per level taken for the graph (d = its
+// average degree; lag rounded, at most 2), is `lag` levels behind the median
+// lanes, so it starts max_lag - lag levels earlier than they do.  Without
+// offsets (p = 0) every lane starts at level 0.
+void phase_starts(const ibfs_batch_s *b, const int64_t *rows, int k, std::vector<int> &start) {
+  start.assign((size_t)k, 0);
+  if (!b->cfg.phase_offsets || k < 2 || b->h_keys.empty()) return;
+  std::vector<double> lk((size_t)k);
+  for (int i = 0; i < k; ++i) lk[(size_t)i] = std::log((double)b->h_keys[(size_t)rows[i]] + 1.0);
+  std::vector<double> srt = lk;
+  std::nth_element(srt.begin(), srt.begin() + k / 2, srt.end());
+  const double med = srt[(size_t)k / 2];
+  const double avg_deg = b->g->Vt > 0 ? (double)b->g->E / (double)b->g->Vt : 2.0;
+  const double ld = std::log(std::max(2.0, avg_deg)) * 0.5 * b->cfg.phase_offsets;
+  int lag_max = 0;
+  std::vector<int> lag((size_t)k);
+  for (int i = 0; i < k; ++i) {
+    const int l = (int)std::floor((med - lk[(size_t)i]) / ld + 0.5);
+    lag[(size_t)i] = std::min(2, std::max(0, l));
+    lag_max = std::max(lag_max, lag[(size_t)i]);
+  }
+  for (int i = 0; i < k; ++i) start[(size_t)i] = lag_max - lag[(size_t)i];
+}
+
 int start_group(ibfs_batch_s *b, Ctx &c, const int32_t *src, int64_t g0, int k, int32_t *dev_out,
                 int32_t *stage, const int64_t *rows) {
   const int W = b->W;
   const int64_t V = b->g->V;
+  phase_starts(b, rows, k, c.start);
+  c.a_max = 0;
+  for (int i = 0; i < k; ++i) c.a_max = std::max(c.a_max, c.start[(size_t)i]);
   std::memset(c.h_masks, 0, sizeof(u64) * masks_words(b));
   for (int i = 0; i < k; ++i) {
-    c.h_masks[i >> 6] |= 1ull << (i & 63);
-    c.h_src[i] = b->g->h_cid[src[g0 + i]];
+    const int a = c.start[(size_t)i];
+    if (a == 0) c.h_masks[i >> 6] |= 1ull << (i & 63);  // lanes of level 0
+    c.h_masks[W + (i >> 6)] |= 1ull << (i & 63);         // every lane of the group
+    // lanes starting later are set by the admission of their level
+    c.h_src[i] = a == 0 ? b->g->h_cid[src[g0 + i]] : -1;
     c.h_lanes[i].out = dev_out ? dev_out + (size_t)rows[i] * (size_t)V
                                : (stage ? stage + (size_t)i * (size_t)V : nullptr);
-    c.h_lanes[i].admit = 0;
+    c.h_lanes[i].admit = a;
     c.h_lanes[i].src = src[g0 + i];
   }
   c.rows = rows;
-  for (int w = 0; w < W; ++w) c.h_masks[W + w] = c.h_masks[w];
-  for (size_t i = 0; i < (size_t)(b->R + 1) * W; ++i) c.h_masks[2 * W + i] = ~0ull;
+  // level masks of the first window (base 0): lanes started at or before L
+  for (int L = 0; L <= b->R; ++L)
+    for (int w = 0; w < W; ++w) {
+      u64 m = ~0ull;
+      if (L < c.a_max) {
+        m = 0;
+        for (int j = 0; j < 64 && w * 64 + j < k; ++j)
+          if (c.start[(size_t)(w * 64 + j)] <= L) m |= 1ull << j;
+      }
+      c.h_masks[2 * W + (size_t)L * W + w] = m;
+    }
   plan_init(c.h_plan, b->cfg, b->hc.rho_q10, (u64)V);
+  c.h_plan->hold = c.a_max;
   HIP_TRY(hipMemcpyAsync(c.d_masks, c.h_masks, sizeof(u64) * masks_words(b), hipMemcpyHostToDevice, c.st));
   HIP_TRY(hipMemcpyAsync(c.d_src, c.h_src, sizeof(int32_t) * k, hipMemcpyHostToDevice, c.st));
   HIP_TRY(hipMemcpyAsync(c.d_lanes, c.h_lanes, sizeof(LaneOut) * k, hipMemcpyHostToDevice, c.st));
@@ -265,6 +318,40 @@ int launch_burst(ibfs_batch_s *b, Ctx &c, ibfs_stats *stats) {
     if (stats) stats->emit_windows++;
     c.base = c.level - 1;
     B = std::min(b->burst, b->R - 1 - (c.level - c.base));
+    if (c.a_max > 0) {  // every lane has started before the new base
+      const int W = b->W;
+      for (size_t i = 0; i < (size_t)(b->R + 1) * W; ++i) c.h_masks[2 * W + i] = ~0ull;
+      HIP_TRY(hipMemcpyAsync(c.d_masks + 2 * W, c.h_masks + 2 * W,
+                             sizeof(u64) * (size_t)(b->R + 1) * W, hipMemcpyHostToDevice, c.st));
+    }
+  }
+  if (c.level <= c.a_max) {
+    // one level at a time while lanes still start: those of this level join
+    // between its expansion and its inspection
+    int n = 0;
+    for (int i = 0; i < c.k; ++i)
+      if (c.start[(size_t)i] == c.level) {
+        c.h_al->src[n] = b->g->h_cid[c.h_lanes[i].src];
+        c.h_al->lane[n] = i;
+        ++n;
+      }
+    if (n > 0) {
+      c.h_al->level = c.level;
+      c.h_al->n = n;
+      HIP_TRY(hipMemcpyAsync(c.d_al, c.h_al, sizeof(AdmitList), hipMemcpyHostToDevice, c.st));
+      const HubSplit hsp{b->hc.deg, b->hc.tie, b->hc.cut};
+      HIP_TRY(launch_expand_dd(b->W, b->g->d_beg, b->g->d_adj, b->g->d_tadj, c.ring, c.flags,
+                               b->R, b->g->Vt, c.ctl, c.d_plan, c.hs, c.q, b->chunk_shift, hsp,
+                               b->grid, c.st));
+      HIP_TRY(launch_admit_dd(b->W, c.d_al, c.d_plan, c.ring, b->R, b->g->Vt, c.ctl, c.flags,
+                              c.st));
+      HIP_TRY(launch_inspect(b->W, dd_inspect_args(b, c), c.st));
+      HIP_TRY(launch_plan_decide(c.d_plan, c.ctl, b->R, c.hs, b->W,
+                                 split_hubs(b->g, b->hc, b->cfg), c.st));
+    } else {
+      HIP_TRY(hipGraphLaunch(c.exec, c.st));
+    }
+    B = 0;
   }
   for (int i = 0; i < B; ++i) HIP_TRY(hipGraphLaunch(c.exec, c.st));
   HIP_TRY(hipMemcpyAsync(c.h_plan, c.d_plan, sizeof(Plan), hipMemcpyDeviceToHost, c.st));
@@ -287,9 +374,9 @@ int finish_group(ibfs_batch_s *b, Ctx &c, int last, ibfs_stats *stats) {
       (d == 0 ? stats->n_td_launches : d == 1 ? stats->n_bu_launches
                : d == 2 ? stats->n_hy_launches : stats->n_split_levels)++;
     }
-    if (!fused)
-      HIP_TRY(launch_count_edges(b->W, slot(b, c, last), b->g->d_deg, b->g->Vt, c.d_masks,
-                                 b->d_edges, c.st));
+    if (!fused)  // every lane of the group (lanes that started later included)
+      HIP_TRY(launch_count_edges(b->W, slot(b, c, last), b->g->d_deg, b->g->Vt,
+                                 c.d_masks + b->W, b->d_edges, c.st));
   }
   if (c.out_host) {  // rows of the caller's order: one copy when consecutive
     const size_t V = (size_t)b->g->V;
@@ -408,7 +495,9 @@ static int batch_run_impl(ibfs_batch_s *b, const int32_t *src, int64_t n, int32_
   // (no device-wide synchronisation: other engines, batches and servers on
   // the device keep running); the contexts start after it
   hipStream_t st0 = b->ctx[0].st;
-  if (b->cfg.pack_sources && n > b->K) {
+  const bool pack = b->cfg.pack_sources && n > b->K;
+  b->h_keys.clear();
+  if (pack || b->cfg.phase_offsets) {
     if (b->key_cap < n) {
       dfree(b->d_keysrc);
       dfree(b->d_keys);
@@ -425,8 +514,9 @@ static int batch_run_impl(ibfs_batch_s *b, const int32_t *src, int64_t n, int32_
     HIP_TRY(hipMemcpyAsync(b->h_keys.data(), b->d_keys, sizeof(u64) * (size_t)n,
                            hipMemcpyDeviceToHost, st0));
     HIP_TRY(hipStreamSynchronize(st0));
-    std::stable_sort(b->order.begin(), b->order.end(),
-                     [&](int64_t x, int64_t y) { return b->h_keys[x] > b->h_keys[y]; });
+    if (pack)
+      std::stable_sort(b->order.begin(), b->order.end(),
+                       [&](int64_t x, int64_t y) { return b->h_keys[x] > b->h_keys[y]; });
   }
   b->psrc.resize((size_t)n);
   for (int64_t p = 0; p < n; ++p) b->psrc[p] = src[b->order[p]];

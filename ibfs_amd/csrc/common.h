@@ -43,6 +43,8 @@ inline int check_direction_config(const ibfs_config &c) {
   if (c.hybrid_hubs < 0) return fail(IBFS_ERR_ARG, "hybrid_hubs must be >= 0");
   if (c.split_cov < 0 || c.split_cov > 100)
     return fail(IBFS_ERR_ARG, "split_cov must be in [0, 100] (got %d)", c.split_cov);
+  if (c.phase_offsets < 0 || c.phase_offsets > 3)
+    return fail(IBFS_ERR_ARG, "phase_offsets must be in [0, 3] (got %d)", c.phase_offsets);
   return IBFS_OK;
 }
 

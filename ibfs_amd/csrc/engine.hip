@@ -569,6 +569,7 @@ int ibfs_config_default(ibfs_config *cfg) {
   // 256 lanes x 2 contexts: 10.3K -> 11.7K GTEPS)
   cfg->pack_sources = 1;
   cfg->split_cov = 0;
+  cfg->phase_offsets = 0;
   cfg->wg_size = 512;
   cfg->td_chunk = 2048;
   cfg->ring_levels = 32;

@@ -5,6 +5,7 @@
 #include <cctype>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <sys/stat.h>
 #include <thread>
@@ -542,6 +543,94 @@ int ibfs_convert_text(const char *path, int32_t reverse, int64_t skip_lines,
     if (reverse) deg[d]++;
   }
   *out = h;
+  return IBFS_OK;
+}
+
+// The converter's five output files (tuple_text_to_bin.cpp:56-310):
+// <prefix>_beg_pos.bin (V+1), _csr.bin, _head.bin, _weight.bin (E each) and
+// _deg.bin (V), little-endian int64.  E counts tokens (reverse=1) or token
+// pairs (reverse=0) as the reference does, so with reverse=1 and an odd
+// trailing token the last slot stays 0.  head[p] = source of the forward edge
+// written at slot p (slots of reversed edges stay 0); weight = rand() % 63 + 1
+// per input line from the C library generator seeded with 1 (the reference
+// calls rand() unseeded), written at the forward slot and, with reverse=1,
+// at the reversed slot.  The files are created fresh (the reference keeps a
+// pre-existing file's bytes in slots it does not write).
+static int write_i64_file(const std::string &path, const int64_t *v, size_t n) {
+  FILE *f = fopen(path.c_str(), "wb");
+  if (!f) return fail(IBFS_ERR_IO, "%s open error", path.c_str());
+  const size_t w = n ? fwrite(v, sizeof(int64_t), n, f) : 0;
+  const int c = fclose(f);
+  if (w != n || c != 0) return fail(IBFS_ERR_IO, "%s write error", path.c_str());
+  return IBFS_OK;
+}
+
+int ibfs_convert_text_files(const char *path, int32_t reverse, int64_t skip_lines,
+                            const char *out_prefix, int64_t *vert_count, int64_t *edge_count) {
+  if (!path || skip_lines < 0) return fail(IBFS_ERR_ARG, "ibfs_convert_text_files: bad arguments");
+  FILE *f = fopen(path, "rb");
+  if (!f) return fail(IBFS_ERR_IO, "%s open error", path);
+  std::string text;
+  {
+    char buf[1 << 16];
+    size_t r;
+    while ((r = fread(buf, 1, sizeof buf, f)) > 0) text.append(buf, r);
+    fclose(f);
+  }
+  size_t off = 0;
+  for (int64_t k = 0; k < skip_lines; ++k) {
+    size_t nl = text.find('\n', off);
+    if (nl == std::string::npos) return fail(IBFS_ERR_FORMAT, "skip more lines than the file has");
+    off = nl + 1;
+  }
+  std::vector<long long> tok;
+  const char *p = text.c_str() + off, *end = text.c_str() + text.size();
+  while (p < end) {
+    while (p < end && std::isspace((unsigned char)*p)) ++p;
+    if (p >= end) break;
+    char *q;
+    long long x = std::strtoll(p, &q, 10);
+    if (q == p) return fail(IBFS_ERR_FORMAT, "non-integer token at byte %lld", (long long)(p - text.c_str()));
+    tok.push_back(x);
+    p = q;
+  }
+  if (tok.size() < 2) return fail(IBFS_ERR_FORMAT, "fewer than two tokens");
+  const long long vmin = *std::min_element(tok.begin(), tok.end());
+  const long long vmax = *std::max_element(tok.begin(), tok.end());
+  const int64_t V = vmax - vmin + 1, lines = (int64_t)tok.size() / 2;
+  const int64_t E = reverse ? (int64_t)tok.size() : lines;
+  std::vector<int64_t> beg((size_t)V + 1, 0), deg((size_t)V, 0), adj((size_t)E, 0),
+      head((size_t)E, 0), weight((size_t)E, 0);
+  for (int64_t l = 0; l < lines; ++l) {
+    deg[tok[2 * l] - vmin]++;
+    if (reverse) deg[tok[2 * l + 1] - vmin]++;
+  }
+  for (int64_t v = 1; v < V; ++v) beg[v] = beg[v - 1] + deg[v - 1];
+  beg[V] = E;
+  std::fill(deg.begin(), deg.end(), 0);
+  std::srand(1);
+  for (int64_t l = 0; l < lines; ++l) {
+    const int64_t s = tok[2 * l] - vmin, d = tok[2 * l + 1] - vmin;
+    // both writes of a line target their slot before either degree moves
+    // (a reversed self-loop fills one slot and leaves the next one 0)
+    adj[beg[s] + deg[s]] = d;
+    if (reverse) adj[beg[d] + deg[d]] = s;
+    const int64_t wt = std::rand() % 63 + 1;
+    weight[beg[s] + deg[s]] = wt;
+    if (reverse) weight[beg[d] + deg[d]] = wt;
+    head[beg[s] + deg[s]] = s;
+    deg[s]++;
+    if (reverse) deg[d]++;
+  }
+  const std::string pre = out_prefix ? out_prefix : path;
+  int rc = write_i64_file(pre + "_beg_pos.bin", beg.data(), beg.size());
+  if (!rc) rc = write_i64_file(pre + "_csr.bin", adj.data(), adj.size());
+  if (!rc) rc = write_i64_file(pre + "_head.bin", head.data(), head.size());
+  if (!rc) rc = write_i64_file(pre + "_weight.bin", weight.data(), weight.size());
+  if (!rc) rc = write_i64_file(pre + "_deg.bin", deg.data(), deg.size());
+  if (rc) return rc;
+  if (vert_count) *vert_count = V;
+  if (edge_count) *edge_count = E;
   return IBFS_OK;
 }
 

@@ -51,6 +51,8 @@ struct Plan {
   int split_cov;        // split levels: % of the top hubs a lane must hold to stay
                         // bottom-up (0: no split levels)
   int td_flag;          // top-down level `level` flags what it changes (flag_top_down)
+  int hold;             // levels below `hold` never end the traversal: lanes that
+                        // start at a later level are still to be admitted
   u64 nf_prev;          // frontier size of the level before `level`
   u64 V;
   int dirlog[256];      // direction per level, indexed level & 255

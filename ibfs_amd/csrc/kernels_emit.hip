@@ -36,11 +36,15 @@ __device__ __forceinline__ void emit_count(const u64 *__restrict__ ring, int R, 
 #pragma unroll
     for (int j = 0; j < NS; ++j) c[j][w] = 0;
   }
-  load_word<W>(ring + (int64_t)(e % R) * slot + i * W, fin);
-  load_word<W>(ring + (int64_t)(b % R) * slot + i * W, bs);
+  // each snapshot of the window is read once: the first is the base word,
+  // the last the final word
+  u64 s[W];
   for (int L = b; L <= e; ++L) {
-    u64 s[W];
     load_word<W>(ring + (int64_t)(L % R) * slot + i * W, s);
+    if (L == b) {
+#pragma unroll
+      for (int w = 0; w < W; ++w) bs[w] = s[w];
+    }
 #pragma unroll
     for (int w = 0; w < W; ++w) {
       u64 carry = ~s[w] & level_masks[(int64_t)(L - b) * W + w];
@@ -52,6 +56,8 @@ __device__ __forceinline__ void emit_count(const u64 *__restrict__ ring, int R, 
       }
     }
   }
+#pragma unroll
+  for (int w = 0; w < W; ++w) fin[w] = s[w];
 }
 
 // Counters of a vertex without a status word (isolated, or past the end).
@@ -82,6 +88,49 @@ __device__ __forceinline__ int32_t lane_depth(const u64 (&c)[NS][W], const u64 (
   return (set && !at_base) ? off + cnt : SKIP;
 }
 
+// Stores of one vertex's depths when every selected lane is in its first
+// window and the window has fewer than 16 levels (4 counter bits): the lane
+// loop is unrolled over 32-bit halves of the words, so each bit position is
+// a constant; eight lane pointers are read from LDS at a time (null for a
+// lane that is not emitted), eight depths computed as cnt | (set - 1) (-1
+// where the bit never arrived), then stored.  An isolated vertex has no
+// word (every depth -1 here); the lane whose source it is gets its 0 from
+// k_emit_iso_sources after the pass.
+template <int W>
+__device__ __forceinline__ void emit_first4(const u64 (&c)[4][W], const u64 (&fin)[W],
+                                            int32_t *const *s_out, int64_t v, bool nt) {
+  // The counter words' loads complete here, before the lane loop: otherwise
+  // lane blocks that branch on a null pointer wait for the vector memory
+  // counter at their start, and that count includes earlier lanes' stores,
+  // which would serialise the stores.
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt unchanged
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t c0 = (uint32_t)(c[0][w] >> (32 * h)), c1 = (uint32_t)(c[1][w] >> (32 * h));
+      const uint32_t c2 = (uint32_t)(c[2][w] >> (32 * h)), c3 = (uint32_t)(c[3][w] >> (32 * h));
+      const uint32_t f = (uint32_t)(fin[w] >> (32 * h));
+#pragma unroll
+      for (int g = 0; g < 32; g += 8) {
+        int32_t *o[8];
+        int32_t d[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int bt = g + i;
+          o[i] = s_out[w * 64 + h * 32 + bt];
+          const uint32_t cnt = ((c0 >> bt) & 1u) | (((c1 >> bt) & 1u) << 1) |
+                               (((c2 >> bt) & 1u) << 2) | (((c3 >> bt) & 1u) << 3);
+          d[i] = (int32_t)(cnt | (((f >> bt) & 1u) - 1u));
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+          if (o[i]) put_depth(o[i], v, d[i], nt);
+      }
+    }
+  }
+}
+
 template <int W, int NS>
 __global__ __launch_bounds__(BS) void k_emit(const u64 *__restrict__ ring, int R, EmitSpace sp,
                                              int b, int e, const LaneOut *__restrict__ lanes,
@@ -93,13 +142,15 @@ __global__ __launch_bounds__(BS) void k_emit(const u64 *__restrict__ ring, int R
   __shared__ int32_t s_src[MAX_LANES];
   __shared__ u64 red[BS / 64];
   u64 acc = 0;  // edges of the selected lanes' reached vertices (final window)
-  for (int i = threadIdx.x; i < k_lanes; i += BS) {
-    const bool on = (sel[i >> 6] >> (i & 63)) & 1ull;
+  int cont = 0;  // some selected lane continues an earlier window
+  for (int i = threadIdx.x; i < W * 64; i += BS) {  // null past the group
+    const bool on = i < k_lanes && ((sel[i >> 6] >> (i & 63)) & 1ull);
     s_out[i] = on ? lanes[i].out : nullptr;
-    s_off[i] = b - lanes[i].admit;
-    s_src[i] = lanes[i].src;
+    s_off[i] = on ? b - lanes[i].admit : 0;
+    s_src[i] = on ? lanes[i].src : -1;
+    cont |= on && s_off[i] > 0;
   }
-  __syncthreads();
+  const bool fast = NS == 4 && !__syncthreads_or(cont);
   for (int64_t v = (int64_t)blockIdx.x * BS + threadIdx.x; v < sp.V;
        v += (int64_t)gridDim.x * BS) {
     const int32_t ci = sp.cid[v];
@@ -112,6 +163,12 @@ __global__ __launch_bounds__(BS) void k_emit(const u64 *__restrict__ ring, int R
 #pragma unroll
       for (int w = 0; w < W; ++w) pc += __popcll(fin[w] & sel[w]);
       acc += (u64)pc * (u64)sp.deg[ci];
+    }
+    if constexpr (NS == 4) {
+      if (fast) {
+        emit_first4<W>(c, fin, s_out, v, nt != 0);
+        continue;
+      }
     }
 #pragma unroll
     for (int w = 0; w < W; ++w) {
@@ -136,6 +193,17 @@ __global__ __launch_bounds__(BS) void k_emit(const u64 *__restrict__ ring, int R
       if (s) atomicAdd(edges_out, s);
     }
   }
+}
+
+// Depth 0 of a lane whose source is isolated (no status word), in the
+// lane's first window: the fast path of k_emit writes -1 at every isolated
+// vertex.  One thread per lane, after that pass on the same stream.
+__global__ void k_emit_iso_sources(const LaneOut *__restrict__ lanes, const u64 *__restrict__ sel,
+                                   const int32_t *__restrict__ cid, int b, int k_lanes) {
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= k_lanes || !((sel[i >> 6] >> (i & 63)) & 1ull)) return;
+  const LaneOut l = lanes[i];
+  if (l.out && l.src >= 0 && b - l.admit <= 0 && cid[l.src] < 0) l.out[l.src] = 0;
 }
 
 // Tiled depth emission: same values as k_emit, but a workgroup owns 256
@@ -560,6 +628,9 @@ static void emit_ns(int mode, int vec4, const u64 *ring, int R, EmitSpace sp, in
     dim3 g(grid_for(sp.V, BS, IBFS_EMIT_GRID_CAP)), blk(BS);
     hipLaunchKernelGGL((k_emit<W, NS>), g, blk, 0, st, ring, R, sp, b, e, lanes, masks, sel, k,
                        mode == 2 ? 1 : 0, edges_out);
+    if (NS == 4)  // depth 0 at isolated sources after the fast path's -1
+      hipLaunchKernelGGL(k_emit_iso_sources, dim3((k + BS - 1) / BS), dim3(BS), 0, st, lanes, sel,
+                         sp.cid, b, k);
   }
 }
 

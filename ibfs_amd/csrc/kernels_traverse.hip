@@ -96,7 +96,7 @@ __device__ void plan_decide(Plan *p, const LevelCtl *c, int L) {
   const u64 mf = atomic_read(&c->m_front);
   const u64 mc = atomic_read(&c->m_cand);
   const u64 mfn = atomic_read(&c->m_front_nh);
-  if (nch == 0) {
+  if (nch == 0 && L >= p->hold) {
     p->done = 1;
     // level 0 always ends a window: lanes whose source is isolated change
     // nothing in the status words but still have depth 0 at the source

@@ -3,11 +3,11 @@
 //
 // Every source of a query occupies one status lane of one traversal context
 // from its admission level a until its BFS ends; its depths are L - a.  A
-// worker thread owns the GPU: at each level boundary of a context it
-// (1) emits and frees the lanes whose BFS ended (no vertex gained the lane at
-// this level), (2) issues the next expansion, (3) admits waiting sources into
-// clean lanes, (4) issues the inspection that also clears the freed lanes in
-// the next snapshot.  Lanes are independent bits, so a query's depths do not
+// worker thread owns the GPU.  With host-driven levels, at each level
+// boundary of a context it (1) emits and frees the lanes whose BFS ended (no
+// vertex gained the lane at this level), (2) issues the next expansion, (3)
+// admits waiting sources into clean lanes, (4) issues the inspection that
+// also clears the freed lanes in the next snapshot.  Lanes are independent bits, so a query's depths do not
 // depend on what shares the context with it.  Contexts are multiplexed with
 // events, so one context's level boundary overlaps another's kernels.
 //
@@ -15,10 +15,14 @@
 // have finished (a new cohort) and, within admit_window levels of that
 // start, into its remaining clean lanes.  A level's direction and cost are
 // shared by all lanes of a context, so lanes far apart in BFS phase would
-// make every level as expensive as the densest phase among them.  With
-// admit_window 0 (the default) a cohort runs on the device: captured level
-// graphs replayed in bursts, one host poll per burst, its depths emitted once
-// it ends, at the narrowest status-word width that holds its sources.
+// make every level as expensive as the densest phase among them.  By default
+// (host_levels 0) a cohort runs on the device, at the narrowest status-word
+// width that holds its first sources: captured level graphs replayed in
+// bursts with one host poll per burst; inside the admission window the host
+// takes every level boundary and the sources waiting there join the running
+// cohort through a device-side admission list; all its lanes are emitted
+// once it ends.  With host_levels 1 every level boundary is on the host and
+// lanes are emitted, freed and reused one by one.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

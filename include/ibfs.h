@@ -91,6 +91,14 @@ typedef struct ibfs_config {
                           fewer than split_cov % of the top hybrid_hubs hubs
                           push top-down instead (their sparse frontiers would
                           keep every candidate scanning); 0: never           */
+  int32_t phase_offsets; /* batched runs: p in 1..3 = a lane whose source's
+                          two-hop size is below its group's median by a
+                          factor of d^(p/2) per level (d = average degree;
+                          rounded, at most 2 levels) starts that many levels
+                          earlier than the median lanes, so that the group's
+                          lanes reach their dense levels together (depths
+                          stay relative to each lane's own start); 0 = every
+                          lane starts at level 0                              */
 } ibfs_config;
 
 /* Per-level record of the last traversal (the build's monitoring log, in place
@@ -192,6 +200,18 @@ int ibfs_gen_sources(ibfs_hostcsr_t h, int64_t count, uint64_t seed,
 int ibfs_convert_text(const char *text_path, int32_t reverse,
                       int64_t skip_lines, ibfs_hostcsr_t *out);
 
+/* The converter's files from a text edge list (tuple_text_to_bin.cpp:56-310):
+ * <out_prefix>_beg_pos.bin, _csr.bin, _head.bin, _weight.bin and _deg.bin,
+ * little-endian int64, with the converter's rules (ids shifted by the
+ * minimum, reverse=1 adds the reversed edge, no dedup; head = source of each
+ * forward edge, 0 at reversed slots; weight = rand() % 63 + 1 per line from
+ * the C library generator seeded with 1, as the reference's unseeded calls).
+ * out_prefix null: the text path, as the reference names them.  V and E
+ * (optional) are returned. */
+int ibfs_convert_text_files(const char *text_path, int32_t reverse,
+                            int64_t skip_lines, const char *out_prefix,
+                            int64_t *vert_count, int64_t *edge_count);
+
 /* ---- device graph -------------------------------------------------------- */
 /* Device-side layout options of a graph (no effect on results). */
 typedef struct ibfs_graph_opts {
@@ -277,9 +297,9 @@ int ibfs_run_groups(ibfs_engine_t e, const int32_t *src, int64_t src_count,
 /* ---- batched launcher ---------------------------------------------------- */
 /* Runs an arbitrary source set packed into lane groups of cfg->lane_bits with
  * `contexts` traversals in flight on separate HIP streams.  Each level is a
- * captured hipGraph of two kernels (expand, inspect) whose direction and
- * queue sizes are decided on the device; the host polls once per `burst`
- * levels.  depth_out is [n][V] as for ibfs_run_groups.  No reference
+ * captured hipGraph of three kernels (expand, inspect, direction decision):
+ * direction and queue sizes are decided on the device; the host polls once
+ * per `burst` levels.  depth_out is [n][V] as for ibfs_run_groups.  No reference
  * counterpart: the reference runs one 128-source group at a time with ~13
  * host synchronisations per level (main.cu:76-90, traversal.cu:101-130). */
 typedef struct ibfs_batch_s *ibfs_batch_t;

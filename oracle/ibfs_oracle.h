@@ -7,7 +7,10 @@
  * it and fails loudly when its HIP library is missing.
  *
  * Every function restates one piece of the iBFS reference (file:line cited at
- * the definition).  Pinning of each function is listed in oracle_pins.md.
+ * the definition).  Each function is pinned by tests/test_oracle.py against
+ * golden vectors written by a library routine (scipy shortest paths), brute
+ * force (Floyd-Warshall), closed forms, hand-derived known answers and the
+ * reference's own converter compiled from its source (oracle/Makefile).
  */
 #ifndef IBFS_ORACLE_H
 #define IBFS_ORACLE_H

@@ -653,6 +653,40 @@ def test_server_in_flight_admission_on_a_long_path():
     srv.close()
 
 
+@pytest.mark.parametrize("lane_bits", [64, 256, 512])
+def test_batch_phase_offsets(s16, lane_bits):
+    """Batched runs whose lanes start at different levels (phase offsets:
+    sources with small two-hop sizes start up to two levels earlier): depth
+    arrays and closeness sums equal the oracle's, with the ring wrapping
+    (5 snapshots) and without, packed and in caller order, and with a
+    mix of isolated, low-degree and hub sources."""
+    say(f"[gpu] batch phase offsets lanes={lane_bits}")
+    h, beg, adj = s16["h"], s16["beg"], s16["adj"]
+    deg = np.diff(beg)
+    rng = np.random.default_rng(lane_bits)
+    lowdeg = np.flatnonzero(deg == 1)[:40].astype(np.int32)
+    hubs = np.argsort(-deg)[:10].astype(np.int32)
+    iso = np.flatnonzero(deg == 0)[:3].astype(np.int32)
+    src = np.concatenate([s16["src"], lowdeg, hubs, iso]).astype(np.int32)
+    rng.shuffle(src)
+    want = oracle.bfs_many(beg, adj, src, threads=8)
+    wsum = np.where(want >= 0, want, 0).sum(axis=1)
+    wreach = (want >= 0).sum(axis=1)
+    for ring, pack, p in ((5, 0, 1), (5, 1, 2), (32, 0, 3), (32, 1, 2), (32, 1, 1)):
+        b = ibfs_amd.Batch(s16["g"], contexts=2, burst=3, lane_bits=lane_bits, ring_levels=ring,
+                           phase_offsets=p, pack_sources=pack)
+        np.testing.assert_array_equal(b.run(src), want, err_msg=f"ring {ring} pack {pack} p {p}")
+        dsum, reach = b.closeness(src)
+        np.testing.assert_array_equal(reach, wreach)
+        np.testing.assert_array_equal(dsum, wsum)
+    z = load_golden("bfs_classes1500.npz")
+    g = ibfs_amd.DeviceGraph(ibfs_amd.HostCsr.from_arrays(z["beg_pos"], z["adj"]))
+    V = len(z["beg_pos"]) - 1
+    gsrc = np.arange(V, dtype=np.int32)
+    b = ibfs_amd.Batch(g, contexts=2, burst=3, lane_bits=lane_bits, ring_levels=4, phase_offsets=1)
+    np.testing.assert_array_equal(b.run(gsrc), z["depth"])
+
+
 def test_batch_packing_keeps_caller_order(s16):
     """Batched runs group sources by two-hop size (default) or in the
     caller's order; depth arrays (host and device output) and closeness come

@@ -159,6 +159,41 @@ def test_converter_against_compiled_reference(tmp_path):
         np.testing.assert_array_equal(adj, want_a)
 
 
+SUFFIXES = ("_beg_pos.bin", "_csr.bin", "_head.bin", "_weight.bin", "_deg.bin")
+
+
+def test_text_to_bin_writes_the_reference_converters_files(tmp_path):
+    """The text-to-bin command (arguments of tuple_text_to_bin.cpp:61-67)
+    writes the same five files, byte for byte, as the reference converter
+    compiled from its own source: with and without reversed edges, skipped
+    header lines, a self-loop, tabs, and an odd trailing token."""
+    ref = ROOT / "oracle" / "_ref" / "text_to_bin.bin"
+    cli = ROOT / "ibfs_amd" / "text-to-bin"
+    if not ref.exists() or not cli.exists():
+        pytest.skip("reference converter or text-to-bin not built here")
+    rng = np.random.default_rng(12)
+    e = rng.integers(5, 300, size=(1500, 2))
+    body = "\n".join(f"{a}\t{b}" if i % 7 == 0 else f"{a} {b}" for i, (a, b) in enumerate(e))
+    texts = {"plain": "# header one\n# header two\n" + body + "\n9 9\n",
+             "odd": body + "\n9 9\n12\n"}
+    for name, text in texts.items():
+        for rev in (0, 1):
+            outs = {}
+            for who, exe in (("ref", ref), ("ours", cli)):
+                d = tmp_path / f"{name}{rev}{who}"  # fresh files for both
+                d.mkdir()
+                (d / "g.txt").write_text(text)
+                skip = 2 if name == "plain" else 0
+                r = subprocess.run([str(exe), str(d / "g.txt"), str(rev), str(skip)], cwd=d,
+                                   capture_output=True, text=True)
+                assert r.returncode == 0, r.stdout
+                outs[who] = {sfx: (d / ("g.txt" + sfx)).read_bytes() for sfx in SUFFIXES}
+            for sfx in SUFFIXES:
+                assert outs["ours"][sfx] == outs["ref"][sfx], (name, rev, sfx)
+    r = subprocess.run([str(cli)], capture_output=True, text=True)
+    assert r.returncode != 0 and "Wrong input" in r.stdout
+
+
 def test_traversed_edges_host():
     g = load_golden("bfs_twocomp.npz")
     h = ibfs_amd.HostCsr.from_arrays(g["beg_pos"], g["adj"])
