@@ -262,7 +262,8 @@ def edges_of(g, sources, chunk=256):
     return total
 
 
-def served_leg(graphs, rates, n_queries, seed, K, admit_window=None, check=True, threads=16):
+def served_leg(graphs, rates, n_queries, seed, K, admit_window=None, check=True, threads=16,
+               server_opts=None):
     """Poisson arrivals at each rate; every query names one graph and carries
     a log-uniform 1..256 source set.  Outputs come from a device row pool per
     graph, recycled on release (one allocation before the arrivals).  Latency
@@ -270,8 +271,9 @@ def served_leg(graphs, rates, n_queries, seed, K, admit_window=None, check=True,
     pool space counts.  Per rate, 2 rows of each of the first 3 queries on
     each graph are copied out at completion and compared with the oracle."""
     rng = np.random.default_rng(seed)
+    opts = dict(server_opts or {})
     servers = [ibfs_amd.Server(gr["g"], lane_bits=K, status_bytes=gr["status_bytes"],
-                               admit_window=admit_window) for gr in graphs]
+                               admit_window=admit_window, **opts) for gr in graphs]
     pools = [ibfs_amd.RowPool(gr["g"].vert_count, gr["pool_rows"]) for gr in graphs]
     results = []
     for lam in rates:
@@ -361,7 +363,9 @@ def served_leg(graphs, rates, n_queries, seed, K, admit_window=None, check=True,
             "cohort_lanes": (f"narrowest of 64..{K} holding the waiting sources"
                              if not admit_window else K),
             "admit_window": 0 if admit_window is None else admit_window,
+            "server_options": opts,
             "contexts": [s.contexts for s in servers],
+            "admitted_in_flight": [s.counters()["admitted_in_flight"] for s in servers],
             "output_pool_rows": [gr["pool_rows"] for gr in graphs],
             "latency": "completion - scheduled arrival", "rates": results}
     for s_ in servers:
@@ -437,6 +441,13 @@ def main() -> int:
     ap.add_argument("--serve-s22-rates", default="2000,4000",
                     help="rates of the R-MAT-only served line (empty: none)")
     ap.add_argument("--admit-window", type=int, default=None)
+    ap.add_argument("--serve-contexts", type=int, default=1,
+                    help="traversal contexts per served graph (status budget)")
+    ap.add_argument("--serve-continuous-rates", default="800,2000,4000",
+                    help="rates of the served line with in-flight admission (sources join "
+                         "running cohorts within 2 levels of their start; empty: none)")
+    ap.add_argument("--serve-phase-offsets", type=int, default=None,
+                    help="phase offsets of served cohorts (default: the batched leg's tuned value)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device 4 times (HBM counter yardstick)")
     args = ap.parse_args()
@@ -575,7 +586,7 @@ def main() -> int:
         log(f"device depth arrays verified bit-exact against the oracle: {verified}")
     out.free()
 
-    served = served_s22 = None
+    served = served_s22 = served_cont = None
     if not args.no_serve:
         log("generating Chung-Lu graph (3.07M vertices, gamma 2.2, mean 76, cap 33000)")
         t0 = time.perf_counter()
@@ -589,7 +600,7 @@ def main() -> int:
         # latency and throughput (3 + 2 contexts: p50 21 ms and 9.7K GTEPS at
         # 2,000 queries/s; 1 + 1: 11 ms and 10.3K)
         def one_context(gr):
-            return 32 * gr.traversal_vertices * 32 + 4096
+            return args.serve_contexts * 32 * gr.traversal_vertices * 32 + 4096
         check = not args.no_cpu
         if check and beg is None:
             beg, adj = h.arrays()
@@ -605,13 +616,25 @@ def main() -> int:
              "pool_rows": int(60e9 // (4 * hc.vert_count))},
         ]
         rates = [float(x) for x in args.serve_rates.split(",") if x]
+        po = args.serve_phase_offsets
+        if po is None:
+            po = int(batch_settings(args.workload)[0].get("phase_offsets", 0))
+        sopts = {"phase_offsets": po}
         served = served_leg(graphs, rates, args.serve_queries, seed=3, K=256,
-                            admit_window=args.admit_window, check=check, threads=threads)
+                            admit_window=args.admit_window, check=check, threads=threads,
+                            server_opts=sopts)
+        rates_c = [float(x) for x in args.serve_continuous_rates.split(",") if x]
+        if rates_c:
+            log("served, in-flight admission (window of 2 levels)")
+            served_cont = served_leg(graphs, rates_c, args.serve_queries, seed=5, K=256,
+                                     admit_window=2, check=check, threads=threads,
+                                     server_opts=sopts)
         rates1 = [float(x) for x in args.serve_s22_rates.split(",") if x]
         if rates1:
             log("served, R-MAT s22 alone")
             served_s22 = served_leg(graphs[:1], rates1, args.serve_queries, seed=4, K=256,
-                                    admit_window=args.admit_window, check=check, threads=threads)
+                                    admit_window=args.admit_window, check=check, threads=threads,
+                                    server_opts=sopts)
 
     res = {
         "metric": METRIC,
@@ -644,6 +667,7 @@ def main() -> int:
         "batched": batched,
         "served": served,
         "served_rmat_only": served_s22,
+        "served_continuous": served_cont,
         "levels_per_group": levels,
         "verified_sources": verified,
     }

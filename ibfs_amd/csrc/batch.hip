@@ -232,37 +232,24 @@ int alloc_ctx(ibfs_batch_s *b, Ctx &c) {
 // level-0 inspection (whose last workgroup plans level 1).
 // src: sources in grouping order; rows[i] = caller's index of lane i (its
 // depth row in dev_out); stage: host-placed output staging (lane i at row i).
-// Start level of each lane of a group (phase offsets p = 1..3): a lane whose
-// source's two-hop size is below the group's median by f^lag, where f =
-// d^(p/2) is the frontier growth per level taken for the graph (d = its
-// average degree; lag rounded, at most 2), is `lag` levels behind the median
-// lanes, so it starts max_lag - lag levels earlier than they do.  Without
-// offsets (p = 0) every lane starts at level 0.
-void phase_starts(const ibfs_batch_s *b, const int64_t *rows, int k, std::vector<int> &start) {
-  start.assign((size_t)k, 0);
-  if (!b->cfg.phase_offsets || k < 2 || b->h_keys.empty()) return;
-  std::vector<double> lk((size_t)k);
-  for (int i = 0; i < k; ++i) lk[(size_t)i] = std::log((double)b->h_keys[(size_t)rows[i]] + 1.0);
-  std::vector<double> srt = lk;
-  std::nth_element(srt.begin(), srt.begin() + k / 2, srt.end());
-  const double med = srt[(size_t)k / 2];
-  const double avg_deg = b->g->Vt > 0 ? (double)b->g->E / (double)b->g->Vt : 2.0;
-  const double ld = std::log(std::max(2.0, avg_deg)) * 0.5 * b->cfg.phase_offsets;
-  int lag_max = 0;
-  std::vector<int> lag((size_t)k);
-  for (int i = 0; i < k; ++i) {
-    const int l = (int)std::floor((med - lk[(size_t)i]) / ld + 0.5);
-    lag[(size_t)i] = std::min(2, std::max(0, l));
-    lag_max = std::max(lag_max, lag[(size_t)i]);
+// Start level of each lane of a group (phase offsets, the rule of
+// ibfs::phase_starts over the sources' two-hop sizes).
+void group_starts(const ibfs_batch_s *b, const int64_t *rows, int k, std::vector<int> &start) {
+  if (!b->cfg.phase_offsets || b->h_keys.empty()) {
+    start.assign((size_t)k, 0);
+    return;
   }
-  for (int i = 0; i < k; ++i) start[(size_t)i] = lag_max - lag[(size_t)i];
+  std::vector<double> key((size_t)k);
+  for (int i = 0; i < k; ++i) key[(size_t)i] = (double)b->h_keys[(size_t)rows[i]];
+  const double avg_deg = b->g->Vt > 0 ? (double)b->g->E / (double)b->g->Vt : 2.0;
+  phase_starts(key, avg_deg, b->cfg.phase_offsets, start);
 }
 
 int start_group(ibfs_batch_s *b, Ctx &c, const int32_t *src, int64_t g0, int k, int32_t *dev_out,
                 int32_t *stage, const int64_t *rows) {
   const int W = b->W;
   const int64_t V = b->g->V;
-  phase_starts(b, rows, k, c.start);
+  group_starts(b, rows, k, c.start);
   c.a_max = 0;
   for (int i = 0; i < k; ++i) c.a_max = std::max(c.a_max, c.start[(size_t)i]);
   std::memset(c.h_masks, 0, sizeof(u64) * masks_words(b));

@@ -1,6 +1,8 @@
 // Internal declarations shared by the host sources of libibfs.
 #pragma once
 
+#include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
@@ -46,6 +48,33 @@ inline int check_direction_config(const ibfs_config &c) {
   if (c.phase_offsets < 0 || c.phase_offsets > 3)
     return fail(IBFS_ERR_ARG, "phase_offsets must be in [0, 3] (got %d)", c.phase_offsets);
   return IBFS_OK;
+}
+
+// Start level of each lane of a group from its source's two-hop size
+// (phase offsets p = 1..3, ibfs_config::phase_offsets): a lane whose key is
+// below the group's median by f^lag, with f = d^(p/2) the frontier growth
+// per level taken for a graph of average degree d (lag rounded, at most 2),
+// is `lag` levels behind the median lanes and starts max_lag - lag levels
+// earlier than they do; p = 0 or fewer than two lanes: every lane at 0.
+inline void phase_starts(const std::vector<double> &two_hop, double avg_deg, int p,
+                         std::vector<int> &start) {
+  const size_t k = two_hop.size();
+  start.assign(k, 0);
+  if (p <= 0 || k < 2) return;
+  std::vector<double> lk(k);
+  for (size_t i = 0; i < k; ++i) lk[i] = std::log(two_hop[i] + 1.0);
+  std::vector<double> srt = lk;
+  std::nth_element(srt.begin(), srt.begin() + (long)(k / 2), srt.end());
+  const double med = srt[k / 2];
+  const double ld = std::log(avg_deg > 2.0 ? avg_deg : 2.0) * 0.5 * p;
+  int lag_max = 0;
+  std::vector<int> lag(k);
+  for (size_t i = 0; i < k; ++i) {
+    const int l = (int)std::floor((med - lk[i]) / ld + 0.5);
+    lag[i] = l < 0 ? 0 : (l > 2 ? 2 : l);
+    lag_max = lag[i] > lag_max ? lag[i] : lag_max;
+  }
+  for (size_t i = 0; i < k; ++i) start[i] = lag_max - lag[i];
 }
 
 // Parallel helper: runs fn(t) on nthreads std::threads.

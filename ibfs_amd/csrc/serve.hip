@@ -116,6 +116,10 @@ struct SCtx {
   int coh = 0;     // 0: host-driven levels, 1: cohort levels on the device, 2: cohort emitting
   int dlevel = 0;  // cohort level the device expands next (host view)
   int last_admit = 0;  // latest level at which sources joined the cohort
+  // phase offsets: lanes of the cohort's first sources that start after
+  // level 0 (lane, start level), and the latest start
+  std::vector<std::pair<int, int>> deferred;
+  int a_max = 0;
   WaitClock clk;   // since the context's last completed event (wait limit)
 };
 
@@ -548,10 +552,10 @@ int process_level(ibfs_server_s *s, SCtx &c) {
 }
 
 // Waiting sources take the clean lanes of a running cohort at level L
-// (caller holds the lock): the admission list for the device, the lane
-// table, host bookkeeping.  *n_adm = sources admitted.
-int admit_into_cohort(ibfs_server_s *s, SCtx &c, int L, int *n_adm) {
-  int n = 0;
+// (caller holds the lock): entries n0, n0+1, ... of the admission list, the
+// lane table, host bookkeeping.  *n_new = sources admitted.
+void take_pending(ibfs_server_s *s, SCtx &c, int L, int n0, int *n_new) {
+  int n = n0;
   for (int k = 0; k < c.K && !s->pending.empty(); ++k) {
     if (c.lanes[k].state != CLEAN) continue;
     const PendingSrc p = s->pending.front();
@@ -568,21 +572,9 @@ int admit_into_cohort(ibfs_server_s *s, SCtx &c, int L, int *n_adm) {
     if (q.t_admit < 0) q.t_admit = now_ms();
     ++n;
   }
-  *n_adm = n;
-  if (n == 0) return IBFS_OK;
-  c.h_al->level = L;
-  c.h_al->n = n;
-  c.last_admit = L;
-  s->lanes_admitted += n;
-  s->lanes_admitted_in_flight += n;
-  HIP_CHK(hipMemcpyAsync(c.d_al, c.h_al, offsetof(AdmitList, src) + sizeof(int32_t) * (size_t)n,
-                         hipMemcpyHostToDevice, c.st));
-  HIP_CHK(hipMemcpyAsync(reinterpret_cast<char *>(c.d_al) + offsetof(AdmitList, lane),
-                         reinterpret_cast<char *>(c.h_al) + offsetof(AdmitList, lane),
-                         sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, c.st));
-  HIP_CHK(hipMemcpyAsync(c.d_lanes, c.h_lanes, sizeof(LaneOut) * (size_t)c.K,
-                         hipMemcpyHostToDevice, c.st));
-  return IBFS_OK;
+  *n_new = n - n0;
+  s->lanes_admitted += n - n0;
+  s->lanes_admitted_in_flight += n - n0;
 }
 
 // Launches the cohort's next levels, then a copy of the plan and the
@@ -606,15 +598,34 @@ int cohort_burst(ibfs_server_s *s, SCtx &c, bool locked) {
   }
   hipGraphExec_t x = c.exec[__builtin_ctz((unsigned)c.W)];
   const int win = s->admit_window;
-  if (win > 0 && c.dlevel <= win) {
-    int n = 0;
-    if (locked) {
-      if (int rc = admit_into_cohort(s, c, c.dlevel, &n)) return rc;
-    } else {
-      std::lock_guard<std::mutex> lk(s->mu);
-      if (int rc = admit_into_cohort(s, c, c.dlevel, &n)) return rc;
+  const bool in_window = win > 0 && c.dlevel <= win;
+  if (in_window || c.dlevel <= c.a_max) {
+    // lanes of the cohort's first sources that start at this level, then
+    // (inside the admission window) sources waiting now
+    int n = 0, fresh = 0;
+    for (const auto &d : c.deferred)
+      if (d.second == c.dlevel) {
+        c.h_al->src[n] = s->g->h_cid[c.h_lanes[d.first].src];
+        c.h_al->lane[n] = d.first;
+        ++n;
+      }
+    if (in_window) {
+      if (locked) {
+        take_pending(s, c, c.dlevel, n, &fresh);
+      } else {
+        std::lock_guard<std::mutex> lk(s->mu);
+        take_pending(s, c, c.dlevel, n, &fresh);
+      }
     }
+    n += fresh;
     if (n > 0) {
+      c.h_al->level = c.dlevel;
+      c.h_al->n = n;
+      c.last_admit = c.dlevel;
+      HIP_CHK(hipMemcpyAsync(c.d_al, c.h_al, sizeof(AdmitList), hipMemcpyHostToDevice, c.st));
+      if (fresh > 0)
+        HIP_CHK(hipMemcpyAsync(c.d_lanes, c.h_lanes, sizeof(LaneOut) * (size_t)c.K,
+                               hipMemcpyHostToDevice, c.st));
       const HubSplit hsp{s->hc.deg, s->hc.tie, s->hc.cut};
       HIP_CHK(launch_expand_dd(c.W, s->g->d_beg, s->g->d_adj, s->g->d_tadj, c.ring, c.flags, s->R,
                                s->g->Vt, c.ctl, c.d_plan, c.hs, c.q, s->chunk_shift, hsp, s->grid,
@@ -679,12 +690,57 @@ int start_cohort(ibfs_server_s *s, SCtx &c) {
   HIP_CHK(launch_fill_u64(slot(s, c, 0), s->g->Vt * W, 0ull, c.st));
   HIP_CHK(launch_fill_u64(reinterpret_cast<u64 *>(c.flags), (s->g->Vt + 7) / 8, 0ull, c.st));
   HIP_CHK(launch_fill_u64(reinterpret_cast<u64 *>(c.ctl), (int64_t)s->R * CTL_WORDS, 0ull, c.st));
+  // the cohort's first sources take lanes 0, 1, ...; with phase offsets a
+  // lane whose source's two-hop size says it runs ahead of the others starts
+  // at a later level (admitted then through the admission list)
   std::memset(c.h_all, 0, sizeof(u64) * W);
   int n = 0;
-  int rc = admit(s, c, 0, c.h_all, &n);
-  if (rc) return rc;
+  for (int k = 0; k < c.K && !s->pending.empty(); ++k) {
+    const PendingSrc p = s->pending.front();
+    s->pending.pop_front();
+    Query &q = s->queries[p.qid];
+    c.lanes[k].state = ACTIVE;
+    c.lanes[k].qid = p.qid;
+    c.lanes[k].admit = 0;
+    c.h_adm[n] = s->g->h_cid[q.src[p.idx]];
+    c.h_adm[s->K + n] = k;
+    c.h_lanes[k].out = q.out + (size_t)p.idx * (size_t)s->g->V;
+    c.h_lanes[k].admit = 0;
+    c.h_lanes[k].src = q.src[p.idx];
+    if (q.t_admit < 0) q.t_admit = now_ms();
+    ++n;
+  }
+  s->lanes_admitted += n;
+  c.deferred.clear();
+  c.a_max = 0;
+  if (s->cfg.phase_offsets > 0 && n > 1 && !s->h_key.empty()) {
+    std::vector<double> key((size_t)n);
+    for (int i = 0; i < n; ++i) key[(size_t)i] = (double)s->h_key[(size_t)c.h_lanes[i].src];
+    std::vector<int> start;
+    const double avg_deg = s->g->Vt > 0 ? (double)s->g->E / (double)s->g->Vt : 2.0;
+    phase_starts(key, avg_deg, s->cfg.phase_offsets, start);
+    for (int i = 0; i < n; ++i) {
+      const int a = start[(size_t)i];
+      if (a == 0) continue;
+      c.lanes[i].admit = a;
+      c.h_lanes[i].admit = a;
+      c.h_adm[i] = -1;  // not set at level 0
+      c.deferred.emplace_back(i, a);
+      c.a_max = std::max(c.a_max, a);
+    }
+  }
+  for (int i = 0; i < n; ++i)
+    if (c.lanes[i].admit == 0) set_bit(c.h_all, i);
+  if (n > 0) {
+    HIP_CHK(hipMemcpyAsync(c.d_adm, c.h_adm, sizeof(int32_t) * 2 * (size_t)s->K,
+                           hipMemcpyHostToDevice, c.st));
+    HIP_CHK(hipMemcpyAsync(c.d_lanes, c.h_lanes, sizeof(LaneOut) * (size_t)s->K,
+                           hipMemcpyHostToDevice, c.st));
+    HIP_CHK(launch_admit(c.W, c.d_adm, c.d_adm + s->K, n, slot(s, c, 0), c.flags, c.st));
+  }
   HIP_CHK(hipMemcpyAsync(c.d_all, c.h_all, sizeof(u64) * W, hipMemcpyHostToDevice, c.st));
   plan_init(c.h_plan, s->cfg, s->hc.rho_q10, (u64)s->g->V);
+  c.h_plan->hold = c.a_max;
   HIP_CHK(hipMemcpyAsync(c.d_plan, c.h_plan, sizeof(Plan), hipMemcpyHostToDevice, c.st));
   HIP_CHK(launch_inspect(W, plan_args(s, c), c.st));
   HIP_CHK(launch_plan_decide(c.d_plan, c.ctl, s->R, c.hs, W, split_hubs(s->g, s->hc, s->cfg), c.st));

@@ -621,6 +621,29 @@ def test_server_sources_join_running_device_cohorts(s16, lane_bits):
     alone.close()
 
 
+@pytest.mark.parametrize("p,window", [(2, 0), (3, 0), (2, 2), (1, 3)])
+def test_server_phase_offsets(s16, p, window):
+    """Served cohorts whose first lanes start at levels set by their sources'
+    two-hop sizes (phase offsets), alone and with sources joining running
+    cohorts (admission window): isolated, degree-1 and hub sources mixed in
+    queries of 1-200 sources; every depth array equals the oracle's."""
+    say(f"[gpu] server phase offsets p={p} window={window}")
+    beg, adj = s16["beg"], s16["adj"]
+    deg = np.diff(beg)
+    V = s16["h"].vert_count
+    rng = np.random.default_rng(p * 10 + window)
+    pool = np.concatenate([s16["h"].gen_sources(400, seed=41), np.flatnonzero(deg == 1)[:50],
+                           np.argsort(-deg)[:20], np.flatnonzero(deg == 0)[:5]]).astype(np.int32)
+    queries = [rng.choice(pool, size=int(n), replace=False).astype(np.int32)
+               for n in (1, 200, 7, 60, 3, 120, 1, 33)]
+    srv = ibfs_amd.Server(s16["g"], lane_bits=256, ring_levels=8, phase_offsets=p,
+                          admit_window=window)
+    outs, _ = _serve(srv, queries, V, gap_s=0.0003)
+    for q, o in zip(queries, outs):
+        np.testing.assert_array_equal(o, oracle.bfs_many(beg, adj, q, threads=8))
+    srv.close()
+
+
 def test_server_in_flight_admission_on_a_long_path():
     """A path of 4,000 vertices: a cohort started by a source at one end runs
     for thousands of levels, so the queries submitted after it has started
