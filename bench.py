@@ -214,14 +214,7 @@ TUNED_DIR = ROOT / "ibfs_amd" / "tuned"
 
 
 def batch_settings(workload):
-    path = TUNED_DIR / f"batch_rmat_{workload}.json"
-    if not path.exists():
-        return {"lane_bits": 256}, 2, 4, None
-    rep = json.loads(path.read_text())
-    best = dict(rep["best"])
-    contexts = int(best.pop("contexts", 2))
-    burst = int(best.pop("burst", 4))
-    return best, contexts, burst, path.name
+    return ibfs_amd.tuned_batch_settings(f"rmat_{workload}")
 
 
 def batch_leg(g, src, steps, warmup, tune, emit_mode, out, workload, contexts=None, burst=None):

@@ -487,6 +487,24 @@ def autotune_batch(g: DeviceGraph, sample_sources, contexts: int = 2, burst: int
     return best, bc.value, bb.value, report
 
 
+TUNED_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned")
+
+
+def tuned_batch_settings(name: str) -> tuple[dict, int, int, Optional[str]]:
+    """Batched-launcher settings from the autotuner report saved as
+    tuned/batch_<name>.json: (config fields, contexts, burst, report file
+    name); defaults (256 lanes, 2 contexts, bursts of 4) when there is none."""
+    import json
+    path = os.path.join(TUNED_DIR, f"batch_{name}.json")
+    if not os.path.exists(path):
+        return {"lane_bits": 256}, 2, 4, None
+    with open(path) as f:
+        best = dict(json.load(f)["best"])
+    contexts = int(best.pop("contexts", 2))
+    burst = int(best.pop("burst", 4))
+    return best, contexts, burst, os.path.basename(path)
+
+
 def config_from_dict(d: dict, device: int = 0) -> Config:
     """Config from the "best" record of an autotune report (the batched
     launcher's contexts and burst, which are not config fields, are skipped)."""

@@ -26,9 +26,9 @@ def cpu_threads():
     return max(1, min(16, len(os.sched_getaffinity(0))))
 
 
-# settings of the batched leg of the measurement (256 lanes, 2 contexts,
-# bursts of 4; top-down hub chunks of 8,192 edges, lrg_shed 4,096, beta 96)
-BATCH_OPTS = dict(contexts=2, burst=4, lane_bits=256, lrg_shed=4096, beta=96, td_chunk=8192)
+# settings of the batched leg of the measurement: the autotuner's saved
+# report for this workload (ibfs_amd/tuned/batch_rmat_s22.json)
+BATCH_OPTS, BATCH_CONTEXTS, BATCH_BURST, _ = ibfs_amd.tuned_batch_settings("rmat_s22")
 
 
 def test_rmat_s22_1024_sources_plain_and_batched():
@@ -52,9 +52,8 @@ def test_rmat_s22_1024_sources_plain_and_batched():
     eng = ibfs_amd.Engine(g, lane_bits=128)
     eng.run_groups(src, out=out_p)
     edges_p = eng.last_stats.traversed_edges
-    say("[gpu] headline: batched 256 lanes x 2 contexts")
-    opts = dict(BATCH_OPTS)
-    b = ibfs_amd.Batch(g, opts.pop("contexts"), opts.pop("burst"), **opts)
+    say(f"[gpu] headline: batched {BATCH_OPTS} x {BATCH_CONTEXTS} contexts")
+    b = ibfs_amd.Batch(g, BATCH_CONTEXTS, BATCH_BURST, **BATCH_OPTS)
     b.run(src, out=out_b)
     edges_b = b.last_stats.traversed_edges
     assert edges_b == edges_p
@@ -83,7 +82,8 @@ def _one_context(g, lanes=256, ring=32):
 
 def test_serving_two_graphs_poisson_queries():
     """The serving workload: R-MAT s22 and Chung-Lu 3.07M resident at once,
-    one server each (256-lane contexts), 200 queries arriving as a Poisson
+    one server each (256-lane contexts, the measured phase offsets of cohort
+    lanes), 200 queries arriving as a Poisson
     stream (1,500 per second), each naming one graph and a log-uniform set of
     1-256 distinct sources drawn from a seeded pool of 256 sources per graph;
     outputs come from recycled device row pools.  Every row of every query
@@ -103,7 +103,8 @@ def test_serving_two_graphs_poisson_queries():
         del beg, adj
         graphs.append(dict(name=name, g=g, V=h.vert_count, pool=pool, want=want,
                            rows=ibfs_amd.RowPool(h.vert_count, 1536),
-                           srv=ibfs_amd.Server(g, lane_bits=256, status_bytes=_one_context(g))))
+                           srv=ibfs_amd.Server(g, lane_bits=256, status_bytes=_one_context(g),
+                                               phase_offsets=BATCH_OPTS.get("phase_offsets", 0))))
     rng = np.random.default_rng(11)
     nq = 200
     gi = rng.integers(0, 2, size=nq)
