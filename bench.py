@@ -196,11 +196,14 @@ def measured_traffic(workload, K, kernel):
     if not path.exists():
         return None, None
     want = f"{KERNEL_NAMES[kernel]}<{K // 64}"
+    col = None
     for line in path.read_text().splitlines():
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 7 and cells[0].startswith(want):
+        if "HBM bytes/launch" in cells:
+            col = cells.index("HBM bytes/launch")
+        elif col is not None and len(cells) > col and cells[0].startswith(want):
             try:
-                return float(cells[4]), path.name
+                return float(cells[col]), path.name
             except ValueError:
                 return None, None
     return None, None
