@@ -242,7 +242,8 @@ def batch_leg(g, src, steps, warmup, tune, emit_mode, out, workload, contexts=No
                 lane_bits=opts.get("lane_bits", 256), contexts=contexts, burst=burst,
                 config=opts, tuned_from=source,
                 levels_per_direction={"td": st.n_td_launches, "bu": st.n_bu_launches,
-                                      "hybrid": st.n_hy_launches})
+                                      "hybrid": st.n_hy_launches, "tail": st.n_tail_levels},
+                tail_reruns=st.n_tail_reruns)
 
 
 def edges_of(g, sources, chunk=256):
@@ -426,6 +427,10 @@ def main() -> int:
     ap.add_argument("--phase-offsets", type=int, default=None,
                     help="batched leg: lanes start at levels aligned by two-hop size (1) or all "
                          "at level 0 (0); default: the tuned setting")
+    ap.add_argument("--tail-div", type=int, default=None,
+                    help="batched leg: sparse tail levels once the frontier has at most "
+                         "V/tail_div vertices and E/tail_div edges (0: never); default: the "
+                         "tuned setting, else the library default")
     ap.add_argument("--batch-contexts", type=int, default=None)
     ap.add_argument("--burst", type=int, default=None)
     ap.add_argument("--no-batch", action="store_true")
@@ -456,6 +461,8 @@ def main() -> int:
         tune["inspect_vpt"] = args.inspect_vpt
     if args.phase_offsets is not None:
         tune["phase_offsets"] = args.phase_offsets
+    if args.tail_div is not None:
+        tune["tail_div"] = args.tail_div
     if args.gpus != 1:
         print(f"bench.py measures one GPU: libibfs has no multi-GPU path (asked for "
               f"--gpus {args.gpus}); source groups would shard across GPUs with no exchange.",

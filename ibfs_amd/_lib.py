@@ -50,6 +50,7 @@ class Config(C.Structure):
         ("work_model", C.c_int32), ("emit_mode", C.c_int32), ("dd_grid", C.c_int32),
         ("inspect_vpt", C.c_int32), ("hybrid_gamma", C.c_int32), ("hybrid_hubs", C.c_int32),
         ("pack_sources", C.c_int32), ("split_cov", C.c_int32), ("phase_offsets", C.c_int32),
+        ("tail_div", C.c_int32),
     ]
 
 
@@ -80,6 +81,7 @@ class Stats(C.Structure):
         ("n_td_launches", C.c_int32), ("n_bu_launches", C.c_int32),
         ("n_inspect_launches", C.c_int32), ("n_emit_launches", C.c_int32),
         ("expand_hy_ms", C.c_double), ("n_hy_launches", C.c_int32), ("n_split_levels", C.c_int32),
+        ("n_tail_levels", C.c_int32), ("n_tail_reruns", C.c_int32),
     ]
 
 

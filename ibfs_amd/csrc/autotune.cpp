@@ -82,8 +82,8 @@ void append_setting(std::string &s, const Setting &x, bool batch) {
   if (batch) {
     std::snprintf(buf, sizeof buf,
                   ", \"dd_grid\": %d, \"pack_sources\": %d, \"phase_offsets\": %d, "
-                  "\"contexts\": %d, \"burst\": %d",
-                  c.dd_grid, c.pack_sources, c.phase_offsets, x.contexts, x.burst);
+                  "\"tail_div\": %d, \"contexts\": %d, \"burst\": %d",
+                  c.dd_grid, c.pack_sources, c.phase_offsets, c.tail_div, x.contexts, x.burst);
     s += buf;
   }
   s += "}";
@@ -124,6 +124,7 @@ int tune(ibfs_graph_t g, const int32_t *src, int64_t n, const ibfs_config *start
     params.push_back({"dd_grid", &ibfs_config::dd_grid, nullptr, {0, 2048, 4096, 16384}});
     params.push_back({"pack_sources", &ibfs_config::pack_sources, nullptr, {0, 1}});
     params.push_back({"phase_offsets", &ibfs_config::phase_offsets, nullptr, {0, 1, 2, 3}});
+    params.push_back({"tail_div", &ibfs_config::tail_div, nullptr, {0, 8, 32, 256}});
     params.push_back({"contexts", nullptr, &Setting::contexts, {1, 2, 3, 4}});
     params.push_back({"burst", nullptr, &Setting::burst, {4, 8, 16}});
   }

@@ -58,6 +58,13 @@ struct Ctx {
   std::vector<int> start;
   int a_max = 0;
   AdmitList *d_al = nullptr, *h_al = nullptr;
+  // sparse tail log, and the group's arguments for a rerun without the tail
+  // after the log overflowed
+  TailLog tl{};
+  u64 tail_cap = 0;
+  bool no_tail = false;
+  const int32_t *src = nullptr;
+  int32_t *dev_out = nullptr, *stage = nullptr;
 };
 
 template <class T>
@@ -126,6 +133,9 @@ void release_ctx(Ctx &c) {
   dfree(c.d_stage);
   dfree(c.d_al);
   hfree(c.h_al);
+  dfree(c.tl.v);
+  dfree(c.tl.lev);
+  dfree(c.tl.bits);
   if (c.ev) (void)hipEventDestroy(c.ev);
   c.ev = nullptr;
   if (c.st) (void)hipStreamDestroy(c.st);
@@ -208,16 +218,23 @@ int alloc_ctx(ibfs_batch_s *b, Ctx &c) {
   H((void **)&c.h_lanes, sizeof(LaneOut) * (size_t)b->K);
   A((void **)&c.d_al, sizeof(AdmitList));
   H((void **)&c.h_al, sizeof(AdmitList));
+  if (b->cfg.tail_div > 0) {
+    c.tail_cap = (u64)Vq + 65536;
+    A((void **)&c.tl.v, sizeof(int32_t) * (size_t)c.tail_cap);
+    A((void **)&c.tl.lev, sizeof(int32_t) * (size_t)c.tail_cap);
+    A((void **)&c.tl.bits, sizeof(u64) * (size_t)b->W * (size_t)c.tail_cap);
+  }
   if (err == hipSuccess) err = hipEventCreateWithFlags(&c.ev, hipEventDisableTiming);
   if (err != hipSuccess)
     return fail(err == hipErrorOutOfMemory ? IBFS_ERR_NOMEM : IBFS_ERR_HIP,
                 "batch context allocation failed: %s", hipGetErrorString(err));
   // capture one level body: device-driven expansion + inspection
+  clear_hip_error();
   HIP_TRY(hipStreamBeginCapture(c.st, hipStreamCaptureModeThreadLocal));
   hipError_t e1 = launch_expand_dd(b->W, b->g->d_beg, b->g->d_adj, b->g->d_tadj, c.ring, c.flags,
                                    b->R, Vt,
                                    c.ctl, c.d_plan, c.hs, c.q, b->chunk_shift,
-                                   HubSplit{b->hc.deg, b->hc.tie, b->hc.cut}, b->grid, c.st);
+                                   HubSplit{b->hc.deg, b->hc.tie, b->hc.cut}, c.tl, b->grid, c.st);
   hipError_t e2 = launch_inspect(b->W, dd_inspect_args(b, c), c.st);
   if (e2 == hipSuccess) e2 = launch_plan_decide(c.d_plan, c.ctl, b->R, c.hs, b->W, split_hubs(b->g, b->hc, b->cfg), c.st);
   hipError_t e3 = hipStreamEndCapture(c.st, &c.graph);
@@ -278,6 +295,11 @@ int start_group(ibfs_batch_s *b, Ctx &c, const int32_t *src, int64_t g0, int k, 
     }
   plan_init(c.h_plan, b->cfg, b->hc.rho_q10, (u64)V);
   c.h_plan->hold = c.a_max;
+  if (c.tail_cap > 0 && !c.no_tail) {
+    c.h_plan->tail_div = b->cfg.tail_div;
+    c.h_plan->tail_cap = c.tail_cap;
+    c.h_plan->E = (u64)b->g->E;
+  }
   HIP_TRY(hipMemcpyAsync(c.d_masks, c.h_masks, sizeof(u64) * masks_words(b), hipMemcpyHostToDevice, c.st));
   HIP_TRY(hipMemcpyAsync(c.d_src, c.h_src, sizeof(int32_t) * k, hipMemcpyHostToDevice, c.st));
   HIP_TRY(hipMemcpyAsync(c.d_lanes, c.h_lanes, sizeof(LaneOut) * k, hipMemcpyHostToDevice, c.st));
@@ -292,6 +314,9 @@ int start_group(ibfs_batch_s *b, Ctx &c, const int32_t *src, int64_t g0, int k, 
   c.k = k;
   c.base = 0;
   c.level = 1;
+  c.src = src;
+  c.dev_out = dev_out;
+  c.stage = stage;
   return IBFS_OK;
 }
 
@@ -299,7 +324,8 @@ int start_group(ibfs_batch_s *b, Ctx &c, const int32_t *src, int64_t g0, int k, 
 // current emission window is overwritten), then a copy of the plan and the
 // context's event.
 int launch_burst(ibfs_batch_s *b, Ctx &c, ibfs_stats *stats) {
-  int B = std::min(b->burst, b->R - 1 - (c.level - c.base));
+  // sparse tail levels write no new snapshot: no window to flush
+  int B = c.h_plan->tail_start ? b->burst : std::min(b->burst, b->R - 1 - (c.level - c.base));
   if (B < 1) {  // flush the window of complete snapshots [base, level-1]
     HIP_TRY(window_out(b, c, c.base, c.level - 1));
     if (stats) stats->emit_windows++;
@@ -329,7 +355,7 @@ int launch_burst(ibfs_batch_s *b, Ctx &c, ibfs_stats *stats) {
       const HubSplit hsp{b->hc.deg, b->hc.tie, b->hc.cut};
       HIP_TRY(launch_expand_dd(b->W, b->g->d_beg, b->g->d_adj, b->g->d_tadj, c.ring, c.flags,
                                b->R, b->g->Vt, c.ctl, c.d_plan, c.hs, c.q, b->chunk_shift, hsp,
-                               b->grid, c.st));
+                               c.tl, b->grid, c.st));
       HIP_TRY(launch_admit_dd(b->W, c.d_al, c.d_plan, c.ring, b->R, b->g->Vt, c.ctl, c.flags,
                               c.st));
       HIP_TRY(launch_inspect(b->W, dd_inspect_args(b, c), c.st));
@@ -347,22 +373,34 @@ int launch_burst(ibfs_batch_s *b, Ctx &c, ibfs_stats *stats) {
 }
 
 int finish_group(ibfs_batch_s *b, Ctx &c, int last, ibfs_stats *stats) {
+  // levels of a sparse tail after its first level T share T's slot, which
+  // holds the final words: the window ends at T and the tail's log then
+  // corrects the depths of the bits that arrived after T
+  const int T = c.h_plan->tail_start;
+  const int e = (T > 0 && last > T) ? T : last;
   // a final depth window counts the TEPS numerator during its pass
-  const bool fused = stats && !b->closeness && last >= c.base;
-  if (last >= c.base) {
-    HIP_TRY(window_out(b, c, c.base, last, fused ? b->d_edges : nullptr));
+  const bool fused = stats && !b->closeness && e >= c.base;
+  if (e >= c.base) {
+    HIP_TRY(window_out(b, c, c.base, e, fused ? b->d_edges : nullptr));
     if (stats) stats->emit_windows++;
   }
+  if (e < last)
+    HIP_TRY(launch_tail_patch(b->W, c.tl, (int64_t)std::min(c.h_plan->tail_count, c.tail_cap), T,
+                              b->g->d_vid, c.d_lanes, c.d_masks + b->W, c.k,
+                              b->closeness ? b->d_sum + c.g0 : nullptr, c.st));
   if (stats) {
     stats->levels = std::max(stats->levels, last + 1);
     // one device-driven expansion per level, by the direction the plan logged
-    for (int l = 1; l <= last && l < 256; ++l) {
+    const int dense_last = T > 0 ? std::min(last, T - 1) : last;
+    for (int l = 1; l <= dense_last && l < 256; ++l) {
       const int d = c.h_plan->dirlog[l];
       (d == 0 ? stats->n_td_launches : d == 1 ? stats->n_bu_launches
                : d == 2 ? stats->n_hy_launches : stats->n_split_levels)++;
     }
+    // tail expansions: T..last and the one that found nothing
+    if (T > 0) stats->n_tail_levels += last - T + 2;
     if (!fused)  // every lane of the group (lanes that started later included)
-      HIP_TRY(launch_count_edges(b->W, slot(b, c, last), b->g->d_deg, b->g->Vt,
+      HIP_TRY(launch_count_edges(b->W, slot(b, c, e), b->g->d_deg, b->g->Vt,
                                  c.d_masks + b->W, b->d_edges, c.st));
   }
   if (c.out_host) {  // rows of the caller's order: one copy when consecutive
@@ -470,6 +508,7 @@ static int batch_run_impl(ibfs_batch_s *b, const int32_t *src, int64_t n, int32_
   if (stats) std::memset(stats, 0, sizeof *stats);
   if (n == 0) return IBFS_OK;
   HIP_TRY(hipSetDevice(b->g->device));
+  clear_hip_error();
   if (out_place == IBFS_OUT_HOST) {
     for (auto &c : b->ctx)
       if (!c.d_stage) HIP_TRY(hipMalloc((void **)&c.d_stage, sizeof(int32_t) * (size_t)b->K * (size_t)V));
@@ -532,6 +571,7 @@ static int batch_run_impl(ibfs_batch_s *b, const int32_t *src, int64_t n, int32_
           }
         }
         ++next;
+        c.no_tail = false;
         rc = start_group(b, c, b->psrc.data(), g0, k, dev_out, stage, b->order.data() + g0);
         if (!rc) rc = launch_burst(b, c, stats);
         c.state = RUNNING;
@@ -545,7 +585,21 @@ static int batch_run_impl(ibfs_batch_s *b, const int32_t *src, int64_t n, int32_
         }
         progressed = true;
         if (c.state == RUNNING) {
-          if (c.h_plan->done) {
+          if (c.h_plan->done && c.h_plan->tail_start && c.h_plan->tail_count > c.tail_cap) {
+            // the sparse tail's log overflowed: the group again, dense (a
+            // closeness window flushed before the tail is counted again, so
+            // the group's sums restart from zero)
+            c.no_tail = true;
+            if (b->closeness) {
+              rc = launch_fill_u64(b->d_sum + c.g0, c.k, 0ull, c.st) == hipSuccess &&
+                           launch_fill_u64(b->d_reach + c.g0, c.k, 0ull, c.st) == hipSuccess
+                       ? IBFS_OK
+                       : fail(IBFS_ERR_HIP, "closeness sums reset failed");
+            }
+            if (!rc) rc = start_group(b, c, c.src, c.g0, c.k, c.dev_out, c.stage, c.rows);
+            if (!rc) rc = launch_burst(b, c, stats);
+            if (stats) stats->n_tail_reruns++;
+          } else if (c.h_plan->done) {
             rc = finish_group(b, c, c.h_plan->last, stats);
             c.state = EMITTING;
           } else if (c.h_plan->level > V + 2) {

@@ -47,6 +47,7 @@ inline int check_direction_config(const ibfs_config &c) {
     return fail(IBFS_ERR_ARG, "split_cov must be in [0, 100] (got %d)", c.split_cov);
   if (c.phase_offsets < 0 || c.phase_offsets > 3)
     return fail(IBFS_ERR_ARG, "phase_offsets must be in [0, 3] (got %d)", c.phase_offsets);
+  if (c.tail_div < 0) return fail(IBFS_ERR_ARG, "tail_div must be >= 0 (got %d)", c.tail_div);
   return IBFS_OK;
 }
 
@@ -123,6 +124,7 @@ struct ibfs_graph_s {
   int32_t *d_deg = nullptr;   // Vt
   int32_t *d_hubs = nullptr;  // the H highest-degree vertices
   int32_t *d_cid = nullptr;   // V: original id -> traversal id or -1
+  int32_t *d_vid = nullptr;   // Vt: traversal id -> original id
   std::vector<int32_t> h_cid;
   int64_t H = 0;
   int64_t max_deg = 0;

@@ -41,7 +41,8 @@ enum { Q_TD_S = 0, Q_TD_M = 1, Q_TD_L = 2, Q_BU_S = 3, Q_BU_M = 4, Q_BU_L = 5 };
 // next level's direction here after each inspection, the expander reads it.
 struct Plan {
   int level;            // level to expand next (== level being inspected)
-  int dir;              // direction of `level`: 0 top-down, 1 bottom-up, 2 hybrid, 3 split
+  int dir;              // direction of `level`: 0 top-down, 1 bottom-up, 2 hybrid, 3 split,
+                        // 4 sparse tail
   int done;             // 1 once a level produced no change
   int last;             // deepest level with a change (valid when done)
   unsigned int blocks_done;  // unused (kept for the layout the host copies)
@@ -53,6 +54,20 @@ struct Plan {
   int td_flag;          // top-down level `level` flags what it changes (flag_top_down)
   int hold;             // levels below `hold` never end the traversal: lanes that
                         // start at a later level are still to be admitted
+  // Sparse tail: once the shrinking top-down frontier of a level L >= 2 (at
+  // or after `hold`) has at most V/tail_div vertices and E/tail_div edges,
+  // every later level pushes only the bits its frontier gained one level
+  // before into one cumulative slot (that of level tail_start = L+1) and
+  // appends a record (vertex, level, gained bits) per gain to a TailLog; the
+  // inspection and its full-slot copy are skipped.  tail_div 0: never.
+  int tail_div;
+  int tail_start;       // first sparse level (0: every level so far was dense)
+  int tail_pad[2];
+  u64 tail_cap;         // records the log holds
+  u64 tail_lo, tail_hi; // records of level `level`-1: the frontier of `level`
+  u64 tail_count;       // records appended (device atomics); > tail_cap: the
+                        // log overflowed and the run must be repeated dense
+  u64 E;                // adjacency entries (the tail's edge threshold)
   u64 nf_prev;          // frontier size of the level before `level`
   u64 V;
   int dirlog[256];      // direction per level, indexed level & 255
@@ -70,8 +85,18 @@ struct AdmitList {
   int32_t lane[MAX_LANES];
 };
 
+// Records of a sparse tail: record i says that traversal vertex v[i] gained
+// the lanes bits[i*W .. i*W+W) at level lev[i].
+struct TailLog {
+  int32_t *v;
+  int32_t *lev;
+  u64 *bits;
+};
+
+
 // Plan of a device-driven traversal before its level 0 (V: vertex count
-// of the beta rule; rho: the graph's hub-entry fraction, x1024).
+// of the beta rule; rho: the graph's hub-entry fraction, x1024).  The sparse
+// tail stays off until the caller sets tail_div, tail_cap and E.
 inline void plan_init(Plan *p, const ibfs_config &c, int rho, u64 V) {
   std::memset(p, 0, sizeof(Plan));
   p->policy = c.policy;
@@ -190,6 +215,11 @@ struct InspectArgs {
   HubSplit hsplit;      // frontier degrees outside the pulled hubs are counted separately
 };
 
+// The launchers report hipGetLastError() after their launches; a HIP call
+// that failed earlier on the same thread (its result already handled) leaves
+// that error set, so each operation clears it before its first launch.
+inline void clear_hip_error() { (void)hipGetLastError(); }
+
 // Launchers (all asynchronous on `st`).  W = status words (u64) per vertex.
 hipError_t launch_fill_u64(u64 *p, int64_t n, u64 value, hipStream_t st);
 // A control block posted to coherent pinned host memory: the block's words,
@@ -228,11 +258,20 @@ hipError_t launch_expand_bu(int W, const int64_t *beg, const int32_t *tadj, cons
 hipError_t launch_expand_hy(int W, const int64_t *beg, const int32_t *adj, const int32_t *tadj,
                             const u64 *prev, const u64 *hs, u64 *cur, const LevelCtl *ctl_prev,
                             Queues q, int chunk_shift, HubSplit hsplit, int grid, hipStream_t st);
-// Device-driven expansion of plan->level (any direction), fixed grid.
+// Device-driven expansion of plan->level (any direction, or a sparse tail
+// level appending to tl), fixed grid.
 hipError_t launch_expand_dd(int W, const int64_t *beg, const int32_t *adj, const int32_t *tadj,
                             u64 *ring, uint8_t *flags, int R, int64_t V, const LevelCtl *ctl_ring,
-                            const Plan *plan, const u64 *hs, Queues q, int chunk_shift,
-                            HubSplit hsplit, int grid, hipStream_t st);
+                            Plan *plan, const u64 *hs, Queues q, int chunk_shift,
+                            HubSplit hsplit, TailLog tl, int grid, hipStream_t st);
+// After the emission of a window that ends at a sparse tail's first level T
+// (whose slot holds the final words, so every tail bit was written as
+// T - admit): records [0, n) of level > T write lev - admit into the rows of
+// the selected lanes (vid: traversal id -> original id), or, with sum_out,
+// add lev - T to the lanes' closeness sums.
+hipError_t launch_tail_patch(int W, TailLog tl, int64_t n, int T, const int32_t *vid,
+                             const LaneOut *lanes, const u64 *sel, int k_lanes, u64 *sum_out,
+                             hipStream_t st);
 // Device-driven direction decision for plan->level + 1; a bottom-up choice
 // becomes a split level when some active lanes hold fewer than
 // plan->split_cov % of the top `hubs` hubs in the hub cache hs (W words per

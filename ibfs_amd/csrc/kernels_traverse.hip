@@ -68,7 +68,7 @@ template <int W>
 __global__ void k_admit_dd(const AdmitList *al, const Plan *plan, u64 *ring, int R, int64_t V,
                            LevelCtl *ctl_ring, uint8_t *flags) {
   const int L = plan->level;
-  if (plan->done || L < 1 || al->level != L) return;
+  if (plan->done || plan->tail_start || L < 1 || al->level != L) return;
   const int n = al->n;
   u64 *S = ring + (int64_t)(L % R) * V * W;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -89,8 +89,24 @@ __global__ void k_admit_dd(const AdmitList *al, const Plan *plan, u64 *ring, int
 }
 
 // Direction of level L+1 from the counters of level L (same rule as the
-// host-driven driver): fixed switch level, or alpha/beta.
+// host-driven driver): fixed switch level, or alpha/beta; or the entry into
+// and the progress of the sparse tail (Plan::tail_div).
 __device__ void plan_decide(Plan *p, const LevelCtl *c, int L) {
+  if (p->tail_start) {
+    // sparse tail level L: the records it appended are the next frontier;
+    // none ends the traversal, an overflowing log ends it for a dense rerun
+    const u64 cnt = atomic_read(&p->tail_count);
+    if (cnt > p->tail_cap || cnt == p->tail_hi) {
+      p->done = 1;
+      p->last = L - 1;
+      return;
+    }
+    p->tail_lo = p->tail_hi;
+    p->tail_hi = cnt;
+    p->dirlog[(L + 1) & 255] = 4;
+    p->level = L + 1;
+    return;
+  }
   const u64 nch = atomic_read(&c->n_changed);
   const u64 nf = atomic_read(&c->n_front);
   const u64 mf = atomic_read(&c->m_front);
@@ -104,6 +120,17 @@ __device__ void plan_decide(Plan *p, const LevelCtl *c, int L) {
     return;
   }
   const int Ln = L + 1;
+  if (p->tail_div > 0 && L >= 2 && L >= p->hold && nf <= p->nf_prev &&
+      nf * (u64)p->tail_div <= p->V && mf * (u64)p->tail_div <= p->E) {
+    p->tail_start = Ln;
+    p->tail_lo = p->tail_hi = 0;
+    p->nf_prev = nf;
+    p->dir = 4;
+    p->td_flag = 0;
+    p->dirlog[Ln & 255] = 4;
+    p->level = Ln;
+    return;
+  }
   const int dir_prev = (L == 0) ? -1 : p->dir;
   const int dir = choose_direction(p->policy, p->sw_level, p->alpha, p->beta, p->gamma, p->rho,
                                    p->V, Ln, dir_prev, p->nf_prev, nf, mf, mc, mfn);
@@ -155,7 +182,7 @@ __global__ __launch_bounds__(NW * 64) void k_inspect(InspectArgs a) {
   LevelCtl *ctl = a.ctl, *ctl_zero = a.ctl_zero;
   int full = a.full_diff;
   if (a.plan) {
-    if (a.plan->done) return;
+    if (a.plan->done || a.plan->tail_start) return;  // sparse tail levels skip the pass
     const int L = a.plan->level;
     full = (L == 0 || !(a.plan->dir == 1 || (a.plan->dir == 0 && a.plan->td_flag)));
     const int64_t slot = a.V * W;
@@ -798,8 +825,159 @@ __global__ __launch_bounds__(BS) void k_expand_hy(const int64_t *__restrict__ be
   expand_hybrid<W>(beg, adj, tadj, gs, cur, lam, ctl, q, cs, hsp, part, nh);
 }
 
+// Sparse tail level L >= T = plan->tail_start: the frontier items push their
+// bits along their adjacency ranges into the cumulative slot of level T.  At
+// L = T the items are the top-down queues of level T-1 (small vertices one
+// per thread, mid vertices and large-vertex chunks one per wave) with the
+// bits S_{T-1} & ~S_{T-2}; after that they are the records of level L-1
+// (vertices of degree < 64 one per thread, the others one per wave).  A
+// neighbour word the push changes gains the bits that were absent from the
+// atomic OR's previous value; each gain is appended as a record of level L
+// (one 64-bit reservation per wave and step; records past the capacity are
+// dropped and the plan ends the run for a dense rerun).  A bit is gained
+// once, so each lane's arrival at a vertex is in exactly one record.
+template <int W>
+__device__ __forceinline__ bool tail_visit(const int32_t *__restrict__ adj, u64 *cum,
+                                           const u64 (&p)[W], int64_t j, int32_t &v,
+                                           u64 (&g)[W]) {
+  v = adj[j];
+  u64 c[W];
+  load_word<W>(cum + (int64_t)v * W, c);
+  bool any = false;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    g[w] = 0;
+    if (p[w] & ~c[w]) {
+      g[w] = p[w] & ~atomicOr(&cum[(int64_t)v * W + w], p[w]);
+      any |= g[w] != 0;
+    }
+  }
+  return any;
+}
+
+// Appends the gains of the active lanes (any: this lane gained g at v); the
+// lowest active lane reserves the wave's records.  Called in divergent code:
+// the ballots see the active lanes only.
+template <int W>
+__device__ __forceinline__ void tail_append(const Plan *plan, const TailLog &tl, u64 cap, int L,
+                                            bool any, int32_t v, const u64 (&g)[W]) {
+  const u64 m = __ballot(any);
+  if (!m) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __builtin_ctzll(__ballot(1));
+  u64 base = 0;
+  // the plan's only field written during the level
+  if (lane == leader) base = atomicAdd(const_cast<u64 *>(&plan->tail_count), (u64)__popcll(m));
+  base = __shfl(base, leader, 64);
+  if (any) {
+    const u64 pos = base + lanes_below(m);
+    if (pos < cap) {
+      tl.v[pos] = v;
+      tl.lev[pos] = L;
+      store_word<W>(tl.bits + pos * W, g);
+    }
+  }
+}
+
+// One item's range [lo, hi) pushed by one wave (64 neighbours per step).
+template <int W>
+__device__ __forceinline__ void tail_wave(const int32_t *__restrict__ adj, u64 *cum,
+                                          const Plan *plan, const TailLog &tl, u64 cap, int L,
+                                          const u64 (&p)[W], int64_t lo, int64_t hi) {
+  const int lane = threadIdx.x & 63;
+  for (int64_t j0 = lo; j0 < hi; j0 += 64) {
+    u64 g[W];
+    int32_t v = -1;
+    bool any = false;
+    if (j0 + lane < hi) any = tail_visit<W>(adj, cum, p, j0 + lane, v, g);
+    tail_append<W>(plan, tl, cap, L, any, v, g);
+  }
+}
+
+// One item's range [lo, hi) pushed by one thread.
+template <int W>
+__device__ __forceinline__ void tail_thread(const int32_t *__restrict__ adj, u64 *cum,
+                                            const Plan *plan, const TailLog &tl, u64 cap, int L,
+                                            const u64 (&p)[W], int64_t lo, int64_t hi) {
+  for (int64_t j = lo; j < hi; ++j) {
+    u64 g[W];
+    int32_t v;
+    const bool any = tail_visit<W>(adj, cum, p, j, v, g);
+    tail_append<W>(plan, tl, cap, L, any, v, g);
+  }
+}
+
+constexpr int TAIL_THREAD_DEG = 64;  // records of smaller degree: one thread each
+
+template <int W>
+__device__ void tail_level(const int64_t *__restrict__ beg, const int32_t *__restrict__ adj,
+                           u64 *ring, int R, int64_t V, const LevelCtl *ctl_ring,
+                           const Plan *__restrict__ plan, const Queues &q, int cs,
+                           const TailLog &tl) {
+  const int L = plan->level, T = plan->tail_start;
+  const int64_t slot = V * W;
+  u64 *cum = ring + (int64_t)(T % R) * slot;
+  const int64_t nthreads = (int64_t)gridDim.x * BS, nwaves = nthreads / 64;
+  const int64_t tid = (int64_t)blockIdx.x * BS + threadIdx.x, wid = tid >> 6;
+  const u64 cap = plan->tail_cap;
+  if (L == T) {
+    const LevelCtl *c = ctl_ring + (T - 1) % R;
+    const int64_t ns = c->q[Q_TD_S], nm = c->q[Q_TD_M], nl = c->q[Q_TD_L];
+    const u64 *s1 = ring + (int64_t)((T - 1) % R) * slot;
+    const u64 *s2 = ring + (int64_t)((T - 2) % R) * slot;
+    auto bits = [&](int32_t u, u64(&p)[W]) {
+      u64 o[W];
+      load_word<W>(s1 + (int64_t)u * W, p);
+      load_word<W>(s2 + (int64_t)u * W, o);
+#pragma unroll
+      for (int w = 0; w < W; ++w) p[w] &= ~o[w];
+    };
+    for (int64_t i = tid; i < ns; i += nthreads) {
+      const int32_t u = q.q[Q_TD_S][i];
+      u64 p[W];
+      bits(u, p);
+      tail_thread<W>(adj, cum, plan, tl, cap, L, p, beg[u], beg[u + 1]);
+    }
+    for (int64_t i = wid; i < nm + nl; i += nwaves) {
+      int32_t u;
+      int64_t lo, hi;
+      if (i < nm) {
+        u = q.q[Q_TD_M][i];
+        lo = beg[u];
+        hi = beg[u + 1];
+      } else {
+        const int2 e = q.chunks[i - nm];
+        u = e.x;
+        td_chunk_range(beg, e, cs, lo, hi);
+      }
+      u64 p[W];
+      bits(u, p);
+      tail_wave<W>(adj, cum, plan, tl, cap, L, p, lo, hi);
+    }
+    return;
+  }
+  const int64_t r0 = (int64_t)plan->tail_lo, n = (int64_t)plan->tail_hi - r0;
+  for (int64_t i = tid; i < n; i += nthreads) {
+    const int32_t u = tl.v[r0 + i];
+    const int64_t lo = beg[u], hi = beg[u + 1];
+    if (hi - lo >= TAIL_THREAD_DEG) continue;
+    u64 p[W];
+    load_word<W>(tl.bits + (r0 + i) * W, p);
+    tail_thread<W>(adj, cum, plan, tl, cap, L, p, lo, hi);
+  }
+  for (int64_t i = wid; i < n; i += nwaves) {
+    const int32_t u = tl.v[r0 + i];
+    const int64_t lo = beg[u], hi = beg[u + 1];
+    if (hi - lo < TAIL_THREAD_DEG) continue;
+    u64 p[W];
+    load_word<W>(tl.bits + (r0 + i) * W, p);
+    tail_wave<W>(adj, cum, plan, tl, cap, L, p, lo, hi);
+  }
+}
+
 // Device-driven expansion of level plan->level: a fixed grid strides over
-// the class queues of the direction the inspector chose.
+// the class queues of the direction the inspector chose, or over the
+// frontier of a sparse tail level.
 template <int W>
 __global__ __launch_bounds__(BS) void k_expand_dd(const int64_t *__restrict__ beg,
                                                   const int32_t *__restrict__ adj,
@@ -808,10 +986,14 @@ __global__ __launch_bounds__(BS) void k_expand_dd(const int64_t *__restrict__ be
                                                   const LevelCtl *__restrict__ ctl_ring,
                                                   const Plan *__restrict__ plan,
                                                   const u64 *__restrict__ hs, Queues q, int cs,
-                                                  HubSplit hsp) {
+                                                  HubSplit hsp, TailLog tl) {
   __shared__ u64 part[4][W];
   __shared__ int nh[4];
   if (plan->done || plan->level < 1) return;
+  if (plan->tail_start) {
+    tail_level<W>(beg, adj, ring, R, V, ctl_ring, plan, q, cs, tl);
+    return;
+  }
   const int L = plan->level, dir = plan->dir;
   const LevelCtl *c = ctl_ring + (L - 1) % R;
   const int64_t slot = V * W;
@@ -880,6 +1062,40 @@ __global__ __launch_bounds__(BS) void k_expand_dd(const int64_t *__restrict__ be
       bu_wave<W>(beg, tadj, gs, cur, flags, lam, q.q[Q_BU_M][i], lane);
     for (int64_t i = blockIdx.x; i < nl; i += G)
       bu_block<W>(beg, tadj, gs, cur, flags, lam, q.q[Q_BU_L][i], part, nh);
+  }
+}
+
+// Depth patch after a sparse tail (launch_tail_patch): one thread per
+// record; record i of level lev > T writes lev - admit into the row of every
+// selected lane it holds, or adds lev - T to that lane's closeness sum.  Each
+// (vertex, lane) arrival is in one record, so no two threads write the same
+// entry.
+template <int W>
+__global__ __launch_bounds__(BS) void k_tail_patch(TailLog tl, int64_t n, int T,
+                                                   const int32_t *__restrict__ vid,
+                                                   const LaneOut *__restrict__ lanes,
+                                                   const u64 *__restrict__ sel, int k_lanes,
+                                                   u64 *sum_out) {
+  for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * BS) {
+    const int lev = tl.lev[i];
+    if (lev <= T) continue;
+    const int32_t v = vid[tl.v[i]];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      u64 b = tl.bits[i * W + w] & sel[w];
+      while (b) {
+        const int k = w * 64 + __builtin_ctzll(b);
+        b &= b - 1;
+        if (k >= k_lanes) break;
+        if (sum_out) {
+          atomicAdd(&sum_out[k], (u64)(lev - T));
+        } else {
+          int32_t *out = lanes[k].out;
+          if (out) out[v] = lev - lanes[k].admit;
+        }
+      }
+    }
   }
 }
 
@@ -1064,11 +1280,22 @@ hipError_t launch_expand_bu(int W, const int64_t *beg, const int32_t *tadj, cons
 
 hipError_t launch_expand_dd(int W, const int64_t *beg, const int32_t *adj, const int32_t *tadj,
                             u64 *ring, uint8_t *flags, int R, int64_t V, const LevelCtl *ctl_ring,
-                            const Plan *plan, const u64 *hs, Queues q, int chunk_shift,
-                            HubSplit hsplit, int grid, hipStream_t st) {
+                            Plan *plan, const u64 *hs, Queues q, int chunk_shift,
+                            HubSplit hsplit, TailLog tl, int grid, hipStream_t st) {
   dim3 g(grid > 0 ? grid : 1), blk(BS);
   IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_expand_dd<WW>, g, blk, 0, st, beg, adj, tadj, ring,
-                                        flags, R, V, ctl_ring, plan, hs, q, chunk_shift, hsplit));
+                                        flags, R, V, ctl_ring, plan, hs, q, chunk_shift, hsplit,
+                                        tl));
+  return hipGetLastError();
+}
+
+hipError_t launch_tail_patch(int W, TailLog tl, int64_t n, int T, const int32_t *vid,
+                             const LaneOut *lanes, const u64 *sel, int k_lanes, u64 *sum_out,
+                             hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  dim3 g(grid_for(n, BS, 4096)), blk(BS);
+  IBFS_DISPATCH_W(W, hipLaunchKernelGGL(k_tail_patch<WW>, g, blk, 0, st, tl, n, T, vid, lanes,
+                                        sel, k_lanes, sum_out));
   return hipGetLastError();
 }
 

@@ -242,7 +242,7 @@ int run(ibfs_graph_s *g, const ibfs_config &cfg, const int32_t *src, int k, int3
     }
     for (int i = 0; i < B; ++i) {
       NA_TRY(launch_expand_dd(W, g->d_beg, g->d_adj, g->d_tadj, ring, flags, R, Vt, ctl, plan, hs,
-                              q, cs, hsp, grid, st));
+                              q, cs, hsp, TailLog{}, grid, st));
       NA_TRY(launch_inspect(W, a, st));
       NA_TRY(launch_plan_decide(plan, ctl, R, hs, W, 0, st));
     }

@@ -281,10 +281,12 @@ int alloc_ctx(ibfs_server_s *s, SCtx &c) {
   c.K = s->K;
   for (int i = 0; i < 4 && (1 << i) <= s->W; ++i) {
     const int Wc = 1 << i;
+    clear_hip_error();
     HIP_CHK(hipStreamBeginCapture(c.st, hipStreamCaptureModeThreadLocal));
     hipError_t e1 = launch_expand_dd(Wc, s->g->d_beg, s->g->d_adj, s->g->d_tadj, c.ring, c.flags,
                                      s->R, s->g->Vt, c.ctl, c.d_plan, c.hs, c.q, s->chunk_shift,
-                                     HubSplit{s->hc.deg, s->hc.tie, s->hc.cut}, s->grid, c.st);
+                                     HubSplit{s->hc.deg, s->hc.tie, s->hc.cut}, TailLog{},
+                                     s->grid, c.st);
     hipError_t e2 = launch_inspect(Wc, plan_args(s, c), c.st);
     if (e2 == hipSuccess) e2 = launch_plan_decide(c.d_plan, c.ctl, s->R, c.hs, Wc, split_hubs(s->g, s->hc, s->cfg), c.st);
     hipError_t e3 = hipStreamEndCapture(c.st, &c.graph[i]);
@@ -628,8 +630,8 @@ int cohort_burst(ibfs_server_s *s, SCtx &c, bool locked) {
                                hipMemcpyHostToDevice, c.st));
       const HubSplit hsp{s->hc.deg, s->hc.tie, s->hc.cut};
       HIP_CHK(launch_expand_dd(c.W, s->g->d_beg, s->g->d_adj, s->g->d_tadj, c.ring, c.flags, s->R,
-                               s->g->Vt, c.ctl, c.d_plan, c.hs, c.q, s->chunk_shift, hsp, s->grid,
-                               c.st));
+                               s->g->Vt, c.ctl, c.d_plan, c.hs, c.q, s->chunk_shift, hsp,
+                               TailLog{}, s->grid, c.st));
       HIP_CHK(launch_admit_dd(c.W, c.d_al, c.d_plan, c.ring, s->R, s->g->Vt, c.ctl, c.flags, c.st));
       HIP_CHK(launch_inspect(c.W, plan_args(s, c), c.st));
       HIP_CHK(launch_plan_decide(c.d_plan, c.ctl, s->R, c.hs, c.W,

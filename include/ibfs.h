@@ -99,6 +99,16 @@ typedef struct ibfs_config {
                           lanes reach their dense levels together (depths
                           stay relative to each lane's own start); 0 = every
                           lane starts at level 0                              */
+  int32_t tail_div;     /* batched runs (device-driven levels): once the
+                          shrinking top-down frontier of a level >= 2 has at
+                          most V/tail_div vertices and E/tail_div edges, the
+                          remaining levels push only newly gained bits into
+                          one cumulative status slot and log each gain, with
+                          no inspection pass; the depths of those levels are
+                          written from the log after the emission.  A log
+                          that overflows (more records than traversal
+                          vertices + 65,536) reruns the group without it.
+                          0: never; default 32                                */
 } ibfs_config;
 
 /* Per-level record of the last traversal (the build's monitoring log, in place
@@ -131,6 +141,10 @@ typedef struct ibfs_stats {
   int32_t n_hy_launches;
   int32_t n_split_levels;  /* batched runs: split levels (dense lanes pull,
                               sparse lanes push; cfg.split_cov)               */
+  int32_t n_tail_levels;   /* batched runs: sparse tail expansions, the last
+                              one (which found nothing) included (tail_div) */
+  int32_t n_tail_reruns;   /* batched runs: groups rerun without the sparse
+                              tail after its log overflowed                   */
 } ibfs_stats;
 
 /* ---- errors / version ---------------------------------------------------- */
