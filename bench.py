@@ -575,8 +575,8 @@ def main() -> int:
         batched["gteps"] = round(batched["gteps"], 3)
         if plain:
             batched["vs_plain"] = round(batched["gteps"] / plain["gteps"], 3)
-        log(f"batched: {batched['ms_per_step']:.2f} ms/step, {batched['gteps']} GTEPS "
-            f"({batched.get('vs_plain')}x plain)")
+        ratio = f" ({batched['vs_plain']}x plain)" if "vs_plain" in batched else ""
+        log(f"batched: {batched['ms_per_step']:.2f} ms/step, {batched['gteps']} GTEPS{ratio}")
         if want is not None:
             if not verify_sample(out_b, sample_rows, want, V, "batched"):
                 return 4

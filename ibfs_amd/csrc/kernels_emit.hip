@@ -37,27 +37,47 @@ __device__ __forceinline__ void emit_count(const u64 *__restrict__ ring, int R, 
     for (int j = 0; j < NS; ++j) c[j][w] = 0;
   }
   // each snapshot of the window is read once: the first is the base word,
-  // the last the final word
-  u64 s[W];
-  for (int L = b; L <= e; ++L) {
-    load_word<W>(ring + (int64_t)(L % R) * slot + i * W, s);
-    if (L == b) {
+  // the last the final word.  NB levels' loads are issued before any of them
+  // is counted, so a vertex waits for ceil(levels / NB) round trips rather
+  // than one per level (one at a time, the loop waited for each level's load
+  // before issuing the next).  R-MAT s22: emission 4.20 -> 3.87 ms per plain
+  // step (128 lanes, NB 8), batched step 9.87 -> 9.5 ms (256 lanes, NB 4);
+  // the batches cost registers (256 lanes: 6 -> 4 waves per SIMD), which the
+  // shorter load chains repay.
+  constexpr int NB = W <= 2 ? 8 : (W == 4 ? 4 : 2);
 #pragma unroll
-      for (int w = 0; w < W; ++w) bs[w] = s[w];
+  for (int w = 0; w < W; ++w) bs[w] = fin[w] = 0;
+  for (int L0 = b; L0 <= e; L0 += NB) {
+    u64 s[NB][W];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      if (L0 + k <= e) {
+        load_word<W>(ring + (int64_t)((L0 + k) % R) * slot + i * W, s[k]);
+      } else {
+#pragma unroll
+        for (int w = 0; w < W; ++w) s[k][w] = ~0ull;  // nothing absent
+      }
+    }
+    if (L0 == b) {
+#pragma unroll
+      for (int w = 0; w < W; ++w) bs[w] = s[0][w];
     }
 #pragma unroll
-    for (int w = 0; w < W; ++w) {
-      u64 carry = ~s[w] & level_masks[(int64_t)(L - b) * W + w];
+    for (int k = 0; k < NB; ++k) {
+      if (L0 + k > e) break;
 #pragma unroll
-      for (int j = 0; j < NS; ++j) {
-        const u64 nc = c[j][w] & carry;
-        c[j][w] ^= carry;
-        carry = nc;
+      for (int w = 0; w < W; ++w) {
+        u64 carry = ~s[k][w] & level_masks[(int64_t)(L0 + k - b) * W + w];
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+          const u64 nc = c[j][w] & carry;
+          c[j][w] ^= carry;
+          carry = nc;
+        }
+        fin[w] = s[k][w];
       }
     }
   }
-#pragma unroll
-  for (int w = 0; w < W; ++w) fin[w] = s[w];
 }
 
 // Counters of a vertex without a status word (isolated, or past the end).
