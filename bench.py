@@ -372,6 +372,104 @@ def served_leg(graphs, rates, n_queries, seed, K, admit_window=None, check=True,
     return info
 
 
+def config_leg(name, h, src0, threads, K=128, batch_opts=None, contexts=2, burst=4, steps=5,
+               cpu_n=64):
+    """One configuration of the workload list: plain groups of K lanes and the
+    batched launcher on the same GroupBy-ordered sources (each output checked
+    against the oracle on up to cpu_n spread rows), the bottom-up kernel's
+    algorithmic HBM fraction, and the oracle's GTEPS on those rows."""
+    import oracle  # checker and CPU baseline only
+    beg, adj = h.arrays()
+    V = h.vert_count
+    g = ibfs_amd.DeviceGraph(h)
+    src = g.groupby(src0)
+    n = len(src)
+    out = ibfs_amd.DeviceBuffer(4 * n * V)
+    p = plain_leg(g, src, K, steps, 1, {}, 0, out)
+    rows = np.arange(0, n, max(1, n // cpu_n))[:cpu_n]
+    t0 = time.perf_counter()
+    want = oracle.bfs_many(beg, adj, src[rows], threads=threads)
+    cpu_s = time.perf_counter() - t0
+    cpu_edges = sum(oracle.traversed_edges(beg, d) for d in want)
+    ok = verify_sample(out, rows, want, V, name + " plain")
+    roof, kernels, _ = roofline_leg(g, src, K, V, {}, p["kern_ms"], p["launches"], steps)
+    b = ibfs_amd.Batch(g, contexts=contexts, burst=burst, **(batch_opts or {}))
+    b.run(src, out=out)
+    edges = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.run(src, out=out)
+        edges += b.last_stats.traversed_edges
+    bt = (time.perf_counter() - t0) / steps
+    ok = verify_sample(out, rows, want, V, name + " batched") and ok
+    b.close()
+    out.free()
+    bu = kernels["expand_bu"]
+    rec = {"sources": n, "plain_gteps": round(p["gteps"], 1),
+           "plain_ms_per_step": round(p["ms_per_step"], 3),
+           "batched_gteps": round(edges / steps / bt / 1e9, 1),
+           "batched_ms_per_step": round(bt * 1e3, 3),
+           "bu_alg_fraction": round((bu["alg_GBps"] or 0.0) / HBM_PEAK_GBS, 3),
+           "dominant_kernel": roof["kernel"], "dominant_fraction": roof["frac"],
+           "cpu_gteps": round(cpu_edges / cpu_s / 1e9, 3), "cpu_threads": threads,
+           "verified_rows": len(rows), "verified": bool(ok)}
+    log(f"{name}: {json.dumps(rec)}")
+    return g, rec
+
+
+def all_configs_leg(threads):
+    """The workload list beyond the headline: R-MAT s16 with 128 sources,
+    Chung-Lu 3.07M with 512 sources, and R-MAT s18 with every vertex as a
+    source for closeness (plain GTEPS on 4,096 of them; the batched
+    closeness sums and reach over all, checked against the oracle on 64)."""
+    import oracle  # checker and CPU baseline only
+    recs = {}
+    h = ibfs_amd.HostCsr.rmat(16, 16, 0.57, 0.19, 0.19, seed=1, permute=True, threads=threads)
+    g, recs["rmat_s16_128src"] = config_leg("rmat_s16_128src", h, h.gen_sources(128, seed=2),
+                                            threads, batch_opts={"lane_bits": 128}, contexts=1,
+                                            steps=20, cpu_n=128)
+    g.close()
+    h = ibfs_amd.HostCsr.chung_lu(3_070_000, 2.2, 76.0, 33000.0, seed=1, threads=threads)
+    g, recs["chung_lu_3.07m_512src"] = config_leg("chung_lu_3.07m_512src", h,
+                                                  h.gen_sources(512, seed=2), threads,
+                                                  batch_opts={"lane_bits": 256})
+    g.close()
+    h = ibfs_amd.HostCsr.rmat(18, 16, 0.57, 0.19, 0.19, seed=1, permute=True, threads=threads)
+    beg, adj = h.arrays()
+    V = h.vert_count
+    g = ibfs_amd.DeviceGraph(h)
+    sample = g.groupby(h.gen_sources(4096, seed=2))
+    out = ibfs_amd.DeviceBuffer(4 * len(sample) * V)
+    p = plain_leg(g, sample, 128, 3, 1, {}, 0, out)
+    _, kernels, _ = roofline_leg(g, sample, 128, V, {}, p["kern_ms"], p["launches"], 3)
+    out.free()
+    allv = np.arange(V, dtype=np.int32)
+    b = ibfs_amd.Batch(g, contexts=2, burst=4, lane_bits=256)
+    b.closeness(allv[:4096])
+    t0 = time.perf_counter()
+    dsum, reach = b.closeness(allv)
+    ct = time.perf_counter() - t0
+    edges = b.last_stats.traversed_edges
+    b.close()
+    chk = np.arange(0, V, V // 64)[:64].astype(np.int32)
+    t0 = time.perf_counter()
+    want = oracle.bfs_many(beg, adj, chk, threads=threads)
+    cpu_s = time.perf_counter() - t0
+    cpu_edges = sum(oracle.traversed_edges(beg, d) for d in want)
+    ok = (np.array_equal(dsum[chk], np.where(want >= 0, want, 0).sum(axis=1))
+          and np.array_equal(reach[chk], (want >= 0).sum(axis=1)))
+    recs["rmat_s18_closeness_all_sources"] = {
+        "sources": V, "plain_gteps": round(p["gteps"], 1), "plain_sample_sources": len(sample),
+        "batched_closeness_gteps": round(edges / ct / 1e9, 1),
+        "batched_closeness_s": round(ct, 3),
+        "bu_alg_fraction": round((kernels["expand_bu"]["alg_GBps"] or 0.0) / HBM_PEAK_GBS, 3),
+        "cpu_gteps": round(cpu_edges / cpu_s / 1e9, 3), "cpu_threads": threads,
+        "verified_rows": len(chk), "verified": bool(ok)}
+    log(f"rmat_s18_closeness_all_sources: {json.dumps(recs['rmat_s18_closeness_all_sources'])}")
+    g.close()
+    return recs
+
+
 def host_cpu_info():
     """Host cores: those this process may run on, the machine's count, the
     cgroup CPU quota if one is set, and the lscpu model name."""
@@ -449,6 +547,9 @@ def main() -> int:
                          "running cohorts within 2 levels of their start; empty: none)")
     ap.add_argument("--serve-phase-offsets", type=int, default=None,
                     help="phase offsets of served cohorts (default: the batched leg's tuned value)")
+    ap.add_argument("--all-configs", action="store_true",
+                    help="measure the other workloads instead (R-MAT s16 x 128, Chung-Lu "
+                         "3.07M x 512, R-MAT s18 all-source closeness) and print their JSON")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device 4 times (HBM counter yardstick)")
     args = ap.parse_args()
@@ -471,6 +572,10 @@ def main() -> int:
     if ibfs_amd.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 3
+    if args.all_configs:
+        threads = max(1, min(16, host_cpu_info()["affinity_cpus"]))
+        print(json.dumps({"configs": all_configs_leg(threads), "host": host_cpu_info()}))
+        return 0
     copy_cal = None
     if args.copy_calibration > 0:
         # plain device copy of known size: the yardstick for HBM byte counters
