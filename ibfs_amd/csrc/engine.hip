@@ -849,12 +849,17 @@ int ibfs_groupby(ibfs_graph_t g, const int32_t *src, int64_t n, int32_t *out) {
   int rc = check_device(g->device);
   if (rc) return rc;
   int32_t *d = nullptr;
+  void *ws = nullptr;
+  const size_t wsb = groupby_workspace_bytes(n);
   HIP_TRY(hipMalloc((void **)&d, sizeof(int32_t) * (size_t)n * 3));
-  hipError_t err = hipMemcpy(d, src, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice);
+  hipError_t err = hipMalloc(&ws, wsb);
+  if (err == hipSuccess) err = hipMemcpy(d, src, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice);
   if (err == hipSuccess)
-    err = launch_groupby(g->d_beg, g->d_adj, g->d_deg, g->d_cid, d, n, d + n, d + 2 * n, nullptr);
+    err = launch_groupby(g->d_beg, g->d_adj, g->d_deg, g->d_cid, d, n, d + n, d + 2 * n, ws, wsb,
+                         nullptr);
   if (err == hipSuccess)
     err = hipMemcpy(out, d + 2 * n, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost);
+  (void)hipFree(ws);
   (void)hipFree(d);
   if (err != hipSuccess) return fail(IBFS_ERR_HIP, "groupby failed: %s", hipGetErrorString(err));
   return IBFS_OK;

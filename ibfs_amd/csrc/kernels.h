@@ -308,11 +308,13 @@ hipError_t launch_two_hop(const int64_t *beg, const int32_t *adj, const int32_t 
                           const int32_t *cid, const int32_t *src, int64_t n, u64 *key,
                           hipStream_t st);
 // GroupBy (groupby.cu:36-149): key[i] = first neighbour of degree > 128 of a
-// source of degree <= 128, else -1; then the stable placement.  src holds
-// original ids (cid maps them to the traversal CSR); keys are traversal ids,
-// whose order is the original order.
+// source of degree <= 128, else -1; then the stable placement, as a stable
+// radix sort (O(n)).  src holds original ids (cid maps them to the traversal
+// CSR); keys are traversal ids, whose order is the original order.  ws: device
+// workspace of groupby_workspace_bytes(n) bytes.
+size_t groupby_workspace_bytes(int64_t n);
 hipError_t launch_groupby(const int64_t *beg, const int32_t *adj, const int32_t *deg,
                           const int32_t *cid, const int32_t *src, int64_t n, int32_t *key,
-                          int32_t *out, hipStream_t st);
+                          int32_t *out, void *ws, size_t ws_bytes, hipStream_t st);
 
 }  // namespace ibfs

@@ -1,6 +1,8 @@
 // Depth emission, TEPS numerator and GroupBy kernels (gfx950, wave64).
 #include <hip/hip_runtime.h>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "device_util.h"
 
 namespace ibfs {
@@ -587,35 +589,15 @@ __global__ __launch_bounds__(BS) void k_groupby_key(const int64_t *__restrict__ 
   if (lane == 0) key[i] = h;
 }
 
-// Stable placement (groupby.cu:69-115): an eligible source goes after every
-// eligible source with a smaller hub id or the same hub and a smaller input
-// index; ineligible sources follow all eligible ones in input order.  Each
-// thread counts its rank over all keys staged through LDS in tiles.
-__global__ __launch_bounds__(BS) void k_groupby_place(const int32_t *__restrict__ key,
-                                                      const int32_t *__restrict__ src, int64_t n,
-                                                      int32_t *__restrict__ out) {
-  __shared__ int32_t tile[1024];
-  const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  const int32_t ki = (i < n) ? key[i] : 0;
-  int64_t less = 0, elig = 0, inel_before = 0;
-  for (int64_t base = 0; base < n; base += 1024) {
-    for (int r = threadIdx.x; r < 1024; r += BS)
-      tile[r] = (base + r < n) ? key[base + r] : 0;
-    __syncthreads();
-    const int m = (n - base) < 1024 ? (int)(n - base) : 1024;
-    for (int jj = 0; jj < m; ++jj) {
-      const int32_t kj = tile[jj];
-      const int64_t j = base + jj;
-      if (kj >= 0) {
-        ++elig;
-        if (ki >= 0 && (kj < ki || (kj == ki && j < i))) ++less;
-      } else if (j < i) {
-        ++inel_before;
-      }
-    }
-    __syncthreads();
-  }
-  if (i < n) out[(ki >= 0) ? less : (elig + inel_before)] = src[i];
+// Sort key of the placement (groupby.cu:69-115): an eligible source sorts by
+// its hub id, an ineligible one after every eligible one; a stable sort then
+// keeps input order among equal keys, which is the reference's placement
+// (eligible sources grouped by hub in input order, then the others in input
+// order).
+__global__ void k_groupby_sortkey(const int32_t *__restrict__ key, int64_t n,
+                                  uint32_t *__restrict__ ukey) {
+  for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS)
+    ukey[i] = key[i] >= 0 ? (uint32_t)key[i] : 0xFFFFFFFFu;
 }
 
 }  // namespace
@@ -733,17 +715,39 @@ hipError_t launch_two_hop(const int64_t *beg, const int32_t *adj, const int32_t 
   return hipGetLastError();
 }
 
+// Workspace of launch_groupby: the sort keys in and out, then the radix
+// sort's temporary storage (256-byte aligned).
+static size_t groupby_sort_bytes(int64_t n) {
+  size_t tb = 0;
+  (void)rocprim::radix_sort_pairs(nullptr, tb, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                  (const int32_t *)nullptr, (int32_t *)nullptr, (size_t)n, 0, 32);
+  return tb;
+}
+
+size_t groupby_workspace_bytes(int64_t n) {
+  if (n <= 0) return 0;
+  const size_t keys = ((size_t)n * 2 * sizeof(uint32_t) + 255) & ~(size_t)255;
+  return keys + groupby_sort_bytes(n);
+}
+
 hipError_t launch_groupby(const int64_t *beg, const int32_t *adj, const int32_t *deg,
                           const int32_t *cid, const int32_t *src, int64_t n, int32_t *key,
-                          int32_t *out, hipStream_t st) {
+                          int32_t *out, void *ws, size_t ws_bytes, hipStream_t st) {
   if (n <= 0) return hipSuccess;
+  if (!ws || ws_bytes < groupby_workspace_bytes(n)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_groupby_key, dim3(grid_for(n, 4, 1 << 30)), dim3(BS), 0, st, beg, adj, deg,
                      cid, src, n, key);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
-  hipLaunchKernelGGL(k_groupby_place, dim3(grid_for(n, BS, 1 << 30)), dim3(BS), 0, st, key, src, n,
-                     out);
-  return hipGetLastError();
+  uint32_t *ukey = static_cast<uint32_t *>(ws);
+  const size_t keys = ((size_t)n * 2 * sizeof(uint32_t) + 255) & ~(size_t)255;
+  hipLaunchKernelGGL(k_groupby_sortkey, dim3(grid_for(n, BS * 4, 4096)), dim3(BS), 0, st, key, n,
+                     ukey);
+  err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  size_t tb = groupby_sort_bytes(n);
+  return rocprim::radix_sort_pairs(static_cast<char *>(ws) + keys, tb, ukey, ukey + n, src, out,
+                                   (size_t)n, 0, 32, st);
 }
 
 }  // namespace ibfs

@@ -443,6 +443,17 @@ def test_groupby_matches_oracle_s16(s16):
     np.testing.assert_array_equal(got, oracle.groupby(s16["beg"], s16["adj"], src))
 
 
+@pytest.mark.parametrize("n", [1, 2, 255, 1_000_000])
+def test_groupby_large_and_tiny_s16(s16, n):
+    """The placement is a stable radix sort: a million sources drawn with
+    repeats (every vertex ~15 times, isolated ones included) keep input order
+    among equal hub keys, as do one and two sources."""
+    say(f"[gpu] groupby s16 n={n}")
+    src = np.random.default_rng(4).integers(0, s16["h"].vert_count, size=n).astype(np.int32)
+    got = s16["g"].groupby(src)
+    np.testing.assert_array_equal(got, oracle.groupby(s16["beg"], s16["adj"], src))
+
+
 def test_reference_mirror_classes(tmp_path):
     say("[gpu] mirror classes")
     h = ibfs_amd.HostCsr.rmat(12, 16, seed=4)
