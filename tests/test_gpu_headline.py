@@ -80,37 +80,17 @@ def _one_context(g, lanes=256, ring=32):
     return ring * g.traversal_vertices * (lanes // 8) + 4096
 
 
-def test_serving_two_graphs_poisson_queries():
-    """The serving workload: R-MAT s22 and Chung-Lu 3.07M resident at once,
-    one server each (256-lane contexts, the measured phase offsets of cohort
-    lanes), 200 queries arriving as a Poisson
-    stream (1,500 per second), each naming one graph and a log-uniform set of
-    1-256 distinct sources drawn from a seeded pool of 256 sources per graph;
-    outputs come from recycled device row pools.  Every row of every query
-    equals the oracle's BFS of its source.  Then three of the queries run
-    alone and give the same arrays as in the mix."""
-    th = cpu_threads()
-    say("[gpu] serving: generating R-MAT s22 and Chung-Lu 3.07M")
-    hs = ibfs_amd.HostCsr.rmat(22, 16, 0.57, 0.19, 0.19, seed=1, permute=True, threads=th)
-    hc = ibfs_amd.HostCsr.chung_lu(3_070_000, 2.2, 76.0, 33000.0, seed=1, threads=th)
-    graphs = []
-    for name, h in (("rmat_s22", hs), ("chung_lu", hc)):
-        beg, adj = h.arrays()
-        pool = h.gen_sources(256, seed=3)
-        say(f"[gpu] serving: oracle for the {name} pool")
-        want = oracle.bfs_many(beg, adj, pool, threads=th)
-        g = ibfs_amd.DeviceGraph(h)
-        del beg, adj
-        graphs.append(dict(name=name, g=g, V=h.vert_count, pool=pool, want=want,
-                           rows=ibfs_amd.RowPool(h.vert_count, 1536),
-                           srv=ibfs_amd.Server(g, lane_bits=256, status_bytes=_one_context(g),
-                                               phase_offsets=BATCH_OPTS.get("phase_offsets", 0))))
-    rng = np.random.default_rng(11)
-    nq = 200
+def _serve_stream(graphs, nq, rate, seed, label):
+    """Submits nq queries as a Poisson stream at `rate` per second from a
+    producer thread, each naming one graph and a log-uniform set of 1-256
+    distinct sources of its pool, into recycled device rows; checks every row
+    of every query against the oracle as it completes.  Returns three of the
+    queries (index -> depth arrays) for the alone-vs-mix check."""
+    rng = np.random.default_rng(seed)
     gi = rng.integers(0, 2, size=nq)
     sizes = np.floor(np.exp(rng.uniform(0, np.log(256), size=nq))).astype(int)
     idx = [rng.choice(256, size=int(k), replace=False) for k in sizes]
-    gaps = rng.exponential(1.0 / 1500.0, size=nq)
+    gaps = rng.exponential(1.0 / rate, size=nq)
     ids, views = [None] * nq, [None] * nq
     err = []
     stop = threading.Event()  # set when the checking side gives up
@@ -152,27 +132,66 @@ def test_serving_two_graphs_poisson_queries():
                 if not np.array_equal(got[r], gr["want"][j]):
                     v = int(np.argmax(got[r] != gr["want"][j]))
                     raise AssertionError(
-                        f"query {i} ({gr['name']}, {k} sources) row {r} source "
+                        f"{label}: query {i} ({gr['name']}, {k} sources) row {r} source "
                         f"{gr['pool'][j]} vertex {v}: {got[r, v]} != {gr['want'][j][v]}")
             checked_rows += k
             if len(keep) < 3 and k >= 2:
-                keep[i] = got.copy()
+                keep[i] = (gi[i], idx[i], got.copy())
             gr["srv"].release(ids[i])
             gr["rows"].free(views[i])
             views[i] = None
             if i % 25 == 0:
-                say(f"[gpu] serving: {i + 1}/{nq} queries checked ({checked_rows} rows)")
+                say(f"[gpu] serving ({label}): {i + 1}/{nq} queries checked ({checked_rows} rows)")
     finally:
         stop.set()
         tp.join()
     assert not err, err
-    say(f"[gpu] serving: {nq} queries, {checked_rows} rows equal the oracle; alone runs")
-    for i, mixed in keep.items():
-        gr = graphs[gi[i]]
-        view = gr["rows"].alloc(len(idx[i]))
-        q = gr["srv"].submit(gr["pool"][idx[i]], view)
+    say(f"[gpu] serving ({label}): {nq} queries, {checked_rows} rows equal the oracle")
+    return keep
+
+
+def test_serving_two_graphs_poisson_queries():
+    """The serving workload: R-MAT s22 and Chung-Lu 3.07M resident at once,
+    one server each (256-lane contexts, the measured phase offsets of cohort
+    lanes), queries arriving as a Poisson stream (1,500 per second), each
+    naming one graph and a log-uniform set of 1-256 distinct sources drawn
+    from a seeded pool of 256 sources per graph; outputs come from recycled
+    device row pools.  First 200 queries with admission at cohort starts (the
+    measured mode), then 120 with in-flight admission (sources join a running
+    context within 2 levels of its cohort's start).  Every row of every query
+    equals the oracle's BFS of its source.  Then three of the first queries
+    run alone and give the same arrays as in the mix."""
+    th = cpu_threads()
+    say("[gpu] serving: generating R-MAT s22 and Chung-Lu 3.07M")
+    hs = ibfs_amd.HostCsr.rmat(22, 16, 0.57, 0.19, 0.19, seed=1, permute=True, threads=th)
+    hc = ibfs_amd.HostCsr.chung_lu(3_070_000, 2.2, 76.0, 33000.0, seed=1, threads=th)
+    graphs = []
+    for name, h in (("rmat_s22", hs), ("chung_lu", hc)):
+        beg, adj = h.arrays()
+        pool = h.gen_sources(256, seed=3)
+        say(f"[gpu] serving: oracle for the {name} pool")
+        want = oracle.bfs_many(beg, adj, pool, threads=th)
+        g = ibfs_amd.DeviceGraph(h)
+        del beg, adj
+        graphs.append(dict(name=name, g=g, V=h.vert_count, pool=pool, want=want,
+                           rows=ibfs_amd.RowPool(h.vert_count, 1536),
+                           srv=ibfs_amd.Server(g, lane_bits=256, status_bytes=_one_context(g),
+                                               phase_offsets=BATCH_OPTS.get("phase_offsets", 0))))
+    keep = _serve_stream(graphs, 200, 1500.0, 11, "cohort admission")
+    for gr in graphs:
+        gr["srv"].set_admission(2)
+    _serve_stream(graphs, 120, 1500.0, 12, "in-flight admission")
+    inflight = sum(gr["srv"].counters()["admitted_in_flight"] for gr in graphs)
+    say(f"[gpu] serving: sources admitted in flight {inflight}")
+    for gr in graphs:
+        gr["srv"].set_admission(0)
+    say("[gpu] serving: alone runs")
+    for i, (g_i, q_idx, mixed) in keep.items():
+        gr = graphs[g_i]
+        view = gr["rows"].alloc(len(q_idx))
+        q = gr["srv"].submit(gr["pool"][q_idx], view)
         gr["srv"].wait(q, timeout_ms=300000)
-        alone = view.to_host(count=len(idx[i]) * gr["V"]).reshape(mixed.shape)
+        alone = view.to_host(count=len(q_idx) * gr["V"]).reshape(mixed.shape)
         gr["srv"].release(q)
         gr["rows"].free(view)
         np.testing.assert_array_equal(alone, mixed, err_msg=f"query {i} alone vs in the mix")
