@@ -745,8 +745,10 @@ hipError_t launch_groupby(const int64_t *beg, const int32_t *adj, const int32_t 
                      ukey);
   err = hipGetLastError();
   if (err != hipSuccess) return err;
+  // same iterator types as the size query in groupby_sort_bytes
   size_t tb = groupby_sort_bytes(n);
-  return rocprim::radix_sort_pairs(static_cast<char *>(ws) + keys, tb, ukey, ukey + n, src, out,
+  return rocprim::radix_sort_pairs(static_cast<char *>(ws) + keys, tb,
+                                   static_cast<const uint32_t *>(ukey), ukey + n, src, out,
                                    (size_t)n, 0, 32, st);
 }
 
