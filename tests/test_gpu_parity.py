@@ -144,6 +144,60 @@ def test_server_isolated_sources():
     srv.close()
 
 
+def _degenerate(case):
+    """(beg, adj) of the degenerate graphs: one vertex, no edges at all (no
+    vertex has a status word), a single undirected edge among isolated
+    vertices, and a star whose centre is the only hub."""
+    if case == "one_vertex":
+        return np.zeros(2, dtype=np.int64), np.zeros(0, dtype=np.int32)
+    if case == "no_edges":
+        return np.zeros(6, dtype=np.int64), np.zeros(0, dtype=np.int32)
+    if case == "one_edge":
+        return symmetric_csr_from_edges(5, [(1, 3)])
+    return symmetric_csr_from_edges(300, [(0, i) for i in range(1, 300)])
+
+
+@pytest.mark.parametrize("case", ["one_vertex", "no_edges", "one_edge", "star"])
+def test_degenerate_graphs_every_runner(case):
+    """Every runner on graphs with no or almost no edges: engine (host and
+    device output, the group loop), GroupBy, batched runs and closeness, a
+    server, and the allocation-free entry point.  Depths equal the oracle's;
+    a source's reach counts the source itself."""
+    say(f"[gpu] degenerate graph {case}")
+    beg, adj = _degenerate(case)
+    V = len(beg) - 1
+    g = ibfs_amd.DeviceGraph(ibfs_amd.HostCsr.from_arrays(beg, adj))
+    deg = np.diff(beg)
+    indeg = np.bincount(adj, minlength=V) if len(adj) else np.zeros(V, dtype=np.int64)
+    assert g.traversal_vertices == int(((deg > 0) | (indeg > 0)).sum())
+    src = np.array([0, V - 1, 0, V // 2, min(1, V - 1), min(3, V - 1)] * 2, dtype=np.int32)
+    want = oracle.bfs_many(beg, adj, src, threads=4)
+    eng = ibfs_amd.Engine(g, lane_bits=64)
+    np.testing.assert_array_equal(eng.traversal(src), want)
+    buf = ibfs_amd.DeviceBuffer(4 * len(src) * V)
+    eng.traversal(src, out=buf)
+    np.testing.assert_array_equal(buf.to_host().reshape(len(src), V), want)
+    np.testing.assert_array_equal(eng.run_groups(src), want)
+    assert eng.last_stats.traversed_edges == sum(oracle.traversed_edges(beg, d) for d in want)
+    np.testing.assert_array_equal(g.groupby(src), oracle.groupby(beg, adj, src))
+    b = ibfs_amd.Batch(g, contexts=2, burst=4, lane_bits=64)
+    np.testing.assert_array_equal(b.run(src), want)
+    dsum, reach = b.closeness(src)
+    d64 = want.astype(np.int64)
+    np.testing.assert_array_equal(reach, (d64 >= 0).sum(axis=1))
+    np.testing.assert_array_equal(dsum, np.where(d64 >= 0, d64, 0).sum(axis=1))
+    b.close()
+    srv = ibfs_amd.Server(g, lane_bits=64, ring_levels=8)
+    queries = [src[:1], src, src[3:7]]
+    outs, _ = _serve(srv, queries, V)
+    for q, o in zip(queries, outs):
+        np.testing.assert_array_equal(o, oracle.bfs_many(beg, adj, q, threads=4))
+    srv.close()
+    g.traversal_noalloc(src, buf)
+    np.testing.assert_array_equal(buf.to_host().reshape(len(src), V), want)
+    eng.close()
+
+
 def test_lane_position_invariance():
     say("[gpu] lane position")
     z = load_golden("bfs_grid6x7.npz")
