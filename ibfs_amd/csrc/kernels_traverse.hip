@@ -111,6 +111,7 @@ __device__ void plan_decide(Plan *p, const LevelCtl *c, int L) {
   const u64 nf = atomic_read(&c->n_front);
   const u64 mf = atomic_read(&c->m_front);
   const u64 mc = atomic_read(&c->m_cand);
+  const u64 nc = atomic_read(&c->n_cand);
   const u64 mfn = atomic_read(&c->m_front_nh);
   if (nch == 0 && L >= p->hold) {
     p->done = 1;
@@ -120,8 +121,13 @@ __device__ void plan_decide(Plan *p, const LevelCtl *c, int L) {
     return;
   }
   const int Ln = L + 1;
+  // the candidates bound what the tail can still append (every record holds
+  // a bit one of them gains): on a long-diameter graph (a grid, a path) most
+  // vertices still miss some lane when the frontier is small, and their
+  // records would overflow the log and cost a dense rerun
   if (p->tail_div > 0 && L >= 2 && L >= p->hold && nf <= p->nf_prev &&
-      nf * (u64)p->tail_div <= p->V && mf * (u64)p->tail_div <= p->E) {
+      nf * (u64)p->tail_div <= p->V && mf * (u64)p->tail_div <= p->E &&
+      nc * (u64)p->tail_div <= p->V) {
     p->tail_start = Ln;
     p->tail_lo = p->tail_hi = 0;
     p->nf_prev = nf;

@@ -118,3 +118,61 @@ def test_split_levels_chung_lu_400k():
         say(f"[gpu] chung-lu 400K split run {rep}")
         np.testing.assert_array_equal(b.run(src), want)
     assert b.last_stats.n_split_levels > 0
+
+
+def _grid_csr(n):
+    """Symmetric CSR of the n x n grid graph (4-neighbourhood), sorted lists."""
+    V = n * n
+    idx = np.arange(V).reshape(n, n)
+    a = np.concatenate([idx[:, :-1].ravel(), idx[:, 1:].ravel(), idx[:-1, :].ravel(),
+                        idx[1:, :].ravel()])
+    b = np.concatenate([idx[:, 1:].ravel(), idx[:, :-1].ravel(), idx[1:, :].ravel(),
+                        idx[:-1, :].ravel()])
+    o = np.lexsort((b, a))
+    a, b = a[o], b[o]
+    beg = np.zeros(V + 1, dtype=np.int64)
+    np.add.at(beg, a + 1, 1)
+    return np.cumsum(beg), b.astype(np.int32)
+
+
+def test_grid_1024_long_traversals():
+    """A 1,024 x 1,024 grid (1M vertices, diameter 2,046): 256 random sources
+    take up to about 2,000 levels, so the 32-snapshot ring flushes dozens of
+    times per group.  While the frontier is small most vertices still miss
+    some lane, and a tail log would overflow (every vertex gains each lane
+    at its own level): the default tail setting must not start a tail that
+    ends in a dense rerun, while tail_div 1 does and reruns.  Depth arrays
+    of the plain engine (two 128-lane groups) and of the batched launcher
+    (256 lanes, tail off, default and 1), TEPS numerators and closeness
+    equal the oracle's."""
+    say("[gpu] grid 1024 x 1024, 256 sources")
+    beg, adj = _grid_csr(1024)
+    V = len(beg) - 1
+    g = ibfs_amd.DeviceGraph(ibfs_amd.HostCsr.from_arrays(beg, adj))
+    src = np.random.default_rng(7).integers(0, V, size=256).astype(np.int32)
+    want = oracle.bfs_many(beg, adj, src, threads=8)
+    edges = sum(oracle.traversed_edges(beg, d) for d in want)
+    d64 = want.astype(np.int64)
+    want_sum, want_reach = np.where(d64 >= 0, d64, 0).sum(axis=1), (d64 >= 0).sum(axis=1)
+    out = ibfs_amd.DeviceBuffer(4 * len(src) * V)
+    eng = ibfs_amd.Engine(g, lane_bits=128)
+    eng.run_groups(src, out=out)
+    np.testing.assert_array_equal(out.to_host().reshape(len(src), V), want)
+    assert eng.last_stats.traversed_edges == edges
+    say(f"      plain: {eng.last_stats.levels} levels, {eng.last_stats.emit_windows} windows")
+    eng.close()
+    for tail_div in (0, 32, 1):
+        b = ibfs_amd.Batch(g, contexts=2, burst=8, lane_bits=256, tail_div=tail_div)
+        b.run(src, out=out)
+        np.testing.assert_array_equal(out.to_host().reshape(len(src), V), want,
+                                      err_msg=f"tail_div {tail_div}")
+        st = b.last_stats
+        assert st.traversed_edges == edges
+        say(f"      batched tail_div={tail_div}: tail levels {st.n_tail_levels}, "
+            f"reruns {st.n_tail_reruns}")
+        if tail_div != 1:
+            assert st.n_tail_reruns == 0
+        dsum, reach = b.closeness(src)
+        np.testing.assert_array_equal(reach, want_reach)
+        np.testing.assert_array_equal(dsum, want_sum)
+        b.close()
