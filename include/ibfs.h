@@ -101,8 +101,9 @@ typedef struct ibfs_config {
                           lane starts at level 0                              */
   int32_t tail_div;     /* batched runs (device-driven levels): once the
                           shrinking top-down frontier of a level >= 2 has at
-                          most V/tail_div vertices and E/tail_div edges, the
-                          remaining levels push only newly gained bits into
+                          most V/tail_div vertices and E/tail_div edges and at
+                          most V/tail_div vertices still miss an active lane,
+                          the remaining levels push only newly gained bits into
                           one cumulative status slot and log each gain, with
                           no inspection pass; the depths of those levels are
                           written from the log after the emission.  A log
