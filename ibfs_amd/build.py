@@ -17,7 +17,7 @@ CLI = PKG / "gpu-ibfs"
 CONVERTER = PKG / "text-to-bin"
 
 HIP_SOURCES = ["kernels_traverse.hip", "kernels_emit.hip", "engine.hip", "batch.hip",
-               "serve.hip", "noalloc.hip"]
+               "serve.hip", "noalloc.hip", "host_copy.hip"]
 CXX_SOURCES = ["hostcsr.cpp", "autotune.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

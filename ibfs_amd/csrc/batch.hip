@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "common.h"
+#include "host_copy.h"
 #include "kernels.h"
 
 using namespace ibfs;
@@ -403,17 +404,20 @@ int finish_group(ibfs_batch_s *b, Ctx &c, int last, ibfs_stats *stats) {
       HIP_TRY(launch_count_edges(b->W, slot(b, c, e), b->g->d_deg, b->g->Vt,
                                  c.d_masks + b->W, b->d_edges, c.st));
   }
-  if (c.out_host) {  // rows of the caller's order: one copy when consecutive
+  if (c.out_host) {  // rows of the caller's order: one piece when consecutive
     const size_t V = (size_t)b->g->V;
     bool consecutive = true;
     for (int i = 1; i < c.k && consecutive; ++i) consecutive = c.rows[i] == c.rows[0] + i;
+    std::vector<D2HPiece> pcs;
     if (consecutive)
-      HIP_TRY(hipMemcpyAsync(c.out_host + (size_t)c.rows[0] * V, c.d_stage,
-                             sizeof(int32_t) * (size_t)c.k * V, hipMemcpyDeviceToHost, c.st));
+      pcs.push_back({c.out_host + (size_t)c.rows[0] * V, c.d_stage,
+                     sizeof(int32_t) * (size_t)c.k * V});
     else
       for (int i = 0; i < c.k; ++i)
-        HIP_TRY(hipMemcpyAsync(c.out_host + (size_t)c.rows[i] * V, c.d_stage + (size_t)i * V,
-                               sizeof(int32_t) * V, hipMemcpyDeviceToHost, c.st));
+        pcs.push_back({c.out_host + (size_t)c.rows[i] * V, c.d_stage + (size_t)i * V,
+                       sizeof(int32_t) * V});
+    const int rc = copy_to_host(pcs.data(), pcs.size(), c.st);
+    if (rc) return rc;
   }
   HIP_TRY(hipEventRecord(c.ev, c.st));
   return IBFS_OK;

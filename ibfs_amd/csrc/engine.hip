@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "common.h"
+#include "host_copy.h"
 #include "kernels.h"
 
 using namespace ibfs;
@@ -607,8 +608,21 @@ int ibfs_device_free(void *dptr) {
 }
 
 int ibfs_memcpy_d2h(void *dst, const void *src, size_t bytes) {
-  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
-  return IBFS_OK;
+  if (bytes && (!dst || !src)) return fail(IBFS_ERR_ARG, "ibfs_memcpy_d2h: null pointer");
+  if (!bytes) return IBFS_OK;
+  // on a library stream of the source's device, as ibfs_memcpy_d2d
+  hipPointerAttribute_t pa{};
+  HIP_TRY(hipPointerGetAttributes(&pa, src));
+  int cur = 0;
+  HIP_TRY(hipGetDevice(&cur));
+  const int dev = pa.device >= 0 ? pa.device : cur;
+  hipStream_t st = nullptr;
+  if (int rc = copy_stream(dev, &st)) return rc;
+  HIP_TRY(hipSetDevice(dev));
+  const D2HPiece pc{dst, src, bytes};
+  const int rc = copy_to_host(&pc, 1, st);
+  (void)hipSetDevice(cur);
+  return rc;
 }
 
 int ibfs_memcpy_h2d(void *dst, const void *src, size_t bytes) {
@@ -977,9 +991,10 @@ int ibfs_traversal(ibfs_engine_t e, const int32_t *src, int32_t k, int32_t *dept
   }
   rc = traverse(e, src, k, dout, stats);
   if (rc) return rc;
-  if (out_place == IBFS_OUT_HOST)
-    HIP_TRY(hipMemcpy(depth_out, dout, sizeof(int32_t) * (size_t)k * (size_t)V,
-                      hipMemcpyDeviceToHost));
+  if (out_place == IBFS_OUT_HOST) {
+    const D2HPiece pc{depth_out, dout, sizeof(int32_t) * (size_t)k * (size_t)V};
+    return copy_to_host(&pc, 1, e->st);
+  }
   return IBFS_OK;
 }
 
