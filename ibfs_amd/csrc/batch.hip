@@ -46,6 +46,10 @@ struct Ctx {
   int32_t *d_stage = nullptr;  // host-placed output staging, K*V
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // a full burst's levels in one graph: consecutive graph launches leave
+  // ≈8 µs between one level's last kernel and the next level's first
+  hipGraph_t graph_burst = nullptr;
+  hipGraphExec_t exec_burst = nullptr;
   hipEvent_t ev = nullptr;
   int state = IDLE;
   int64_t g0 = 0;
@@ -115,6 +119,8 @@ void release_ctx(Ctx &c) {
   if (c.st) (void)hipStreamSynchronize(c.st);
   if (c.exec) (void)hipGraphExecDestroy(c.exec);
   if (c.graph) (void)hipGraphDestroy(c.graph);
+  if (c.exec_burst) (void)hipGraphExecDestroy(c.exec_burst);
+  if (c.graph_burst) (void)hipGraphDestroy(c.graph_burst);
   c.exec = nullptr;
   c.graph = nullptr;
   dfree(c.ring);
@@ -229,20 +235,31 @@ int alloc_ctx(ibfs_batch_s *b, Ctx &c) {
   if (err != hipSuccess)
     return fail(err == hipErrorOutOfMemory ? IBFS_ERR_NOMEM : IBFS_ERR_HIP,
                 "batch context allocation failed: %s", hipGetErrorString(err));
-  // capture one level body: device-driven expansion + inspection
-  clear_hip_error();
-  HIP_TRY(hipStreamBeginCapture(c.st, hipStreamCaptureModeThreadLocal));
-  hipError_t e1 = launch_expand_dd(b->W, b->g->d_beg, b->g->d_adj, b->g->d_tadj, c.ring, c.flags,
-                                   b->R, Vt,
-                                   c.ctl, c.d_plan, c.hs, c.q, b->chunk_shift,
-                                   HubSplit{b->hc.deg, b->hc.tie, b->hc.cut}, c.tl, b->grid, c.st);
-  hipError_t e2 = launch_inspect(b->W, dd_inspect_args(b, c), c.st);
-  if (e2 == hipSuccess) e2 = launch_plan_decide(c.d_plan, c.ctl, b->R, c.hs, b->W, split_hubs(b->g, b->hc, b->cfg), c.st);
-  hipError_t e3 = hipStreamEndCapture(c.st, &c.graph);
-  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
-    return fail(IBFS_ERR_HIP, "level graph capture failed: %s / %s / %s", hipGetErrorString(e1),
-                hipGetErrorString(e2), hipGetErrorString(e3));
-  HIP_TRY(hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0));
+  // capture the level body (device-driven expansion + inspection + plan),
+  // once and, for full bursts, `burst` times in a row: every kernel reads
+  // its level from the plan, so repeated bodies run consecutive levels
+  for (int pass = 0; pass < 2; ++pass) {
+    const int reps = pass == 0 ? 1 : b->burst;
+    if (pass == 1 && reps < 2) break;  // bursts of one level need no second graph
+    clear_hip_error();
+    HIP_TRY(hipStreamBeginCapture(c.st, hipStreamCaptureModeThreadLocal));
+    hipError_t e1 = hipSuccess, e2 = hipSuccess;
+    for (int r = 0; r < reps && e1 == hipSuccess && e2 == hipSuccess; ++r) {
+      e1 = launch_expand_dd(b->W, b->g->d_beg, b->g->d_adj, b->g->d_tadj, c.ring, c.flags, b->R,
+                            Vt, c.ctl, c.d_plan, c.hs, c.q, b->chunk_shift,
+                            HubSplit{b->hc.deg, b->hc.tie, b->hc.cut}, c.tl, b->grid, c.st);
+      e2 = launch_inspect(b->W, dd_inspect_args(b, c), c.st);
+      if (e2 == hipSuccess)
+        e2 = launch_plan_decide(c.d_plan, c.ctl, b->R, c.hs, b->W,
+                                split_hubs(b->g, b->hc, b->cfg), c.st);
+    }
+    hipGraph_t &gr = pass == 0 ? c.graph : c.graph_burst;
+    hipError_t e3 = hipStreamEndCapture(c.st, &gr);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+      return fail(IBFS_ERR_HIP, "level graph capture failed: %s / %s / %s",
+                  hipGetErrorString(e1), hipGetErrorString(e2), hipGetErrorString(e3));
+    HIP_TRY(hipGraphInstantiate(pass == 0 ? &c.exec : &c.exec_burst, gr, nullptr, nullptr, 0));
+  }
   return IBFS_OK;
 }
 
@@ -367,7 +384,9 @@ int launch_burst(ibfs_batch_s *b, Ctx &c, ibfs_stats *stats) {
     }
     B = 0;
   }
-  for (int i = 0; i < B; ++i) HIP_TRY(hipGraphLaunch(c.exec, c.st));
+  if (B == b->burst && c.exec_burst) HIP_TRY(hipGraphLaunch(c.exec_burst, c.st));
+  else
+    for (int i = 0; i < B; ++i) HIP_TRY(hipGraphLaunch(c.exec, c.st));
   HIP_TRY(hipMemcpyAsync(c.h_plan, c.d_plan, sizeof(Plan), hipMemcpyDeviceToHost, c.st));
   HIP_TRY(hipEventRecord(c.ev, c.st));
   return IBFS_OK;

@@ -112,6 +112,10 @@ struct SCtx {
   // cohort runs at the narrowest width that holds its sources
   hipGraph_t graph[4] = {};
   hipGraphExec_t exec[4] = {};
+  // per width, kCohortBurst levels in one graph for full bursts (consecutive
+  // graph launches leave ≈8 µs between levels)
+  hipGraph_t graph_burst[4] = {};
+  hipGraphExec_t exec_burst[4] = {};
   int W = 0, K = 0;  // current width: words per vertex and lanes (<= the server's)
   int coh = 0;     // 0: host-driven levels, 1: cohort levels on the device, 2: cohort emitting
   int dlevel = 0;  // cohort level the device expands next (host view)
@@ -187,8 +191,10 @@ void release_ctx(SCtx &c) {
   for (int i = 0; i < 4; ++i) {
     if (c.exec[i]) (void)hipGraphExecDestroy(c.exec[i]);
     if (c.graph[i]) (void)hipGraphDestroy(c.graph[i]);
-    c.exec[i] = nullptr;
-    c.graph[i] = nullptr;
+    if (c.exec_burst[i]) (void)hipGraphExecDestroy(c.exec_burst[i]);
+    if (c.graph_burst[i]) (void)hipGraphDestroy(c.graph_burst[i]);
+    c.exec[i] = c.exec_burst[i] = nullptr;
+    c.graph[i] = c.graph_burst[i] = nullptr;
   }
   dfree(c.d_plan);
   hfree(c.h_plan);
@@ -279,21 +285,33 @@ int alloc_ctx(ibfs_server_s *s, SCtx &c) {
   // direction decision (every kernel derives its slot stride from its width)
   c.W = s->W;
   c.K = s->K;
+  // (and kCohortBurst of them in a row: every kernel reads its level from
+  // the plan, so repeated bodies run consecutive levels)
   for (int i = 0; i < 4 && (1 << i) <= s->W; ++i) {
     const int Wc = 1 << i;
-    clear_hip_error();
-    HIP_CHK(hipStreamBeginCapture(c.st, hipStreamCaptureModeThreadLocal));
-    hipError_t e1 = launch_expand_dd(Wc, s->g->d_beg, s->g->d_adj, s->g->d_tadj, c.ring, c.flags,
-                                     s->R, s->g->Vt, c.ctl, c.d_plan, c.hs, c.q, s->chunk_shift,
-                                     HubSplit{s->hc.deg, s->hc.tie, s->hc.cut}, TailLog{},
-                                     s->grid, c.st);
-    hipError_t e2 = launch_inspect(Wc, plan_args(s, c), c.st);
-    if (e2 == hipSuccess) e2 = launch_plan_decide(c.d_plan, c.ctl, s->R, c.hs, Wc, split_hubs(s->g, s->hc, s->cfg), c.st);
-    hipError_t e3 = hipStreamEndCapture(c.st, &c.graph[i]);
-    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
-      return fail(IBFS_ERR_HIP, "cohort level capture failed: %s / %s / %s", hipGetErrorString(e1),
-                  hipGetErrorString(e2), hipGetErrorString(e3));
-    HIP_CHK(hipGraphInstantiate(&c.exec[i], c.graph[i], nullptr, nullptr, 0));
+    for (int pass = 0; pass < 2; ++pass) {
+      const int reps = pass == 0 ? 1 : kCohortBurst;
+      clear_hip_error();
+      HIP_CHK(hipStreamBeginCapture(c.st, hipStreamCaptureModeThreadLocal));
+      hipError_t e1 = hipSuccess, e2 = hipSuccess;
+      for (int r = 0; r < reps && e1 == hipSuccess && e2 == hipSuccess; ++r) {
+        e1 = launch_expand_dd(Wc, s->g->d_beg, s->g->d_adj, s->g->d_tadj, c.ring, c.flags, s->R,
+                              s->g->Vt, c.ctl, c.d_plan, c.hs, c.q, s->chunk_shift,
+                              HubSplit{s->hc.deg, s->hc.tie, s->hc.cut}, TailLog{}, s->grid,
+                              c.st);
+        e2 = launch_inspect(Wc, plan_args(s, c), c.st);
+        if (e2 == hipSuccess)
+          e2 = launch_plan_decide(c.d_plan, c.ctl, s->R, c.hs, Wc,
+                                  split_hubs(s->g, s->hc, s->cfg), c.st);
+      }
+      hipGraph_t &gr = pass == 0 ? c.graph[i] : c.graph_burst[i];
+      hipError_t e3 = hipStreamEndCapture(c.st, &gr);
+      if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+        return fail(IBFS_ERR_HIP, "cohort level capture failed: %s / %s / %s",
+                    hipGetErrorString(e1), hipGetErrorString(e2), hipGetErrorString(e3));
+      HIP_CHK(hipGraphInstantiate(pass == 0 ? &c.exec[i] : &c.exec_burst[i], gr, nullptr,
+                                  nullptr, 0));
+    }
   }
   return IBFS_OK;
 }
@@ -639,6 +657,8 @@ int cohort_burst(ibfs_server_s *s, SCtx &c, bool locked) {
     } else {
       HIP_CHK(hipGraphLaunch(x, c.st));
     }
+  } else if (B == kCohortBurst) {
+    HIP_CHK(hipGraphLaunch(c.exec_burst[__builtin_ctz((unsigned)c.W)], c.st));
   } else {
     for (int i = 0; i < B; ++i) HIP_CHK(hipGraphLaunch(x, c.st));
   }
