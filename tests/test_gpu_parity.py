@@ -910,3 +910,42 @@ def test_wait_limit_ends_a_wait_and_the_engine_recovers():
     eng = ibfs_amd.Engine(g, lane_bits=512)
     got = eng.traversal(src[:16])
     np.testing.assert_array_equal(got, oracle.bfs_many(beg, adj, src[:16], threads=8))
+
+
+def test_host_copies_every_size_class():
+    """Device-to-host copies (ibfs_memcpy_d2h, which DeviceBuffer.to_host and
+    host-placed depth arrays use): tiny copies, just under and at the size
+    where copies start streaming through the pinned chunks (4 MiB), a 64-MiB
+    chunk plus a few bytes, three chunks plus an odd tail; device sources
+    at odd offsets, host destinations at odd addresses, nothing written past
+    the destination; then two threads copying at once."""
+    import threading
+    say("[gpu] host copies")
+    L = ibfs_amd._lib.lib()
+    rng = np.random.default_rng(5)
+    total = 3 * (64 << 20) + 4096 + 7
+    pattern = rng.integers(0, 256, size=total, dtype=np.uint8)
+    buf = ibfs_amd.DeviceBuffer(total)
+    assert L.ibfs_memcpy_h2d(buf.ptr, pattern.ctypes.data, total) == 0
+    cases = [(0, 1), (3, 4095), (1, (4 << 20) - 1), (0, 4 << 20), (5, (64 << 20) + 3),
+             (0, 3 * (64 << 20) + 7), (11, total - 11)]
+    for off, n in cases:
+        host = np.full(n + 16, 0xAB, dtype=np.uint8)
+        dst = host[1:1 + n]
+        assert L.ibfs_memcpy_d2h(dst.ctypes.data, buf.ptr + off, n) == 0, (off, n)
+        np.testing.assert_array_equal(dst, pattern[off:off + n], err_msg=f"{off}+{n}")
+        assert host[0] == 0xAB and (host[1 + n:] == 0xAB).all()
+    np.testing.assert_array_equal(buf.to_host(dtype=np.uint8), pattern)
+    got = [None, None]
+
+    def worker(i):
+        got[i] = buf.to_host(dtype=np.uint8, count=total - 64 * i, offset=64 * i)
+
+    th = [threading.Thread(target=worker, args=(i,)) for i in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for i in range(2):
+        np.testing.assert_array_equal(got[i], pattern[64 * i:])
+    buf.free()
