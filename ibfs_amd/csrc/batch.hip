@@ -121,8 +121,8 @@ void release_ctx(Ctx &c) {
   if (c.graph) (void)hipGraphDestroy(c.graph);
   if (c.exec_burst) (void)hipGraphExecDestroy(c.exec_burst);
   if (c.graph_burst) (void)hipGraphDestroy(c.graph_burst);
-  c.exec = nullptr;
-  c.graph = nullptr;
+  c.exec = c.exec_burst = nullptr;
+  c.graph = c.graph_burst = nullptr;
   dfree(c.ring);
   dfree(c.ctl);
   for (int i = 0; i < 6; ++i) dfree(c.q.q[i]);
