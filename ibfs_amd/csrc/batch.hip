@@ -39,6 +39,11 @@ struct Ctx {
   Queues q{};
   u64 *hs = nullptr;
   uint8_t *flags = nullptr;  // changed-at-this-level byte per traversal vertex
+  // a group's set-up data (level masks, sources, lane table, plan) lives in
+  // one pinned block and one device block of the same layout, so a group
+  // starts with one copy; the pointers below point into them
+  char *d_setup = nullptr, *h_setup = nullptr;
+  size_t setup_bytes = 0;
   Plan *d_plan = nullptr, *h_plan = nullptr;
   u64 *d_masks = nullptr, *h_masks = nullptr;  // [0,W) lanes_all, [W,2W) sel, then (R+1)W level masks
   int32_t *d_src = nullptr, *h_src = nullptr;
@@ -129,14 +134,12 @@ void release_ctx(Ctx &c) {
   dfree(c.q.chunks);
   dfree(c.hs);
   dfree(c.flags);
-  dfree(c.d_plan);
-  hfree(c.h_plan);
-  dfree(c.d_masks);
-  hfree(c.h_masks);
-  dfree(c.d_src);
-  hfree(c.h_src);
-  dfree(c.d_lanes);
-  hfree(c.h_lanes);
+  dfree(c.d_setup);
+  hfree(c.h_setup);
+  c.d_plan = c.h_plan = nullptr;
+  c.d_masks = c.h_masks = nullptr;
+  c.d_src = c.h_src = nullptr;
+  c.d_lanes = c.h_lanes = nullptr;
   dfree(c.d_stage);
   dfree(c.d_al);
   hfree(c.h_al);
@@ -215,14 +218,26 @@ int alloc_ctx(ibfs_batch_s *b, Ctx &c) {
   A((void **)&c.q.chunks, sizeof(int2) * ((size_t)Vq + (size_t)(b->g->E >> b->chunk_shift) + 2));
   A((void **)&c.hs, sizeof(u64) * (size_t)b->W * (size_t)std::max<int64_t>(b->g->H, 1));
   A((void **)&c.flags, sizeof(u64) * (size_t)((Vq + 7) / 8));
-  A((void **)&c.d_plan, sizeof(Plan));
-  H((void **)&c.h_plan, sizeof(Plan));
-  A((void **)&c.d_masks, sizeof(u64) * masks_words(b));
-  H((void **)&c.h_masks, sizeof(u64) * masks_words(b));
-  A((void **)&c.d_src, sizeof(int32_t) * (size_t)b->K);
-  H((void **)&c.h_src, sizeof(int32_t) * (size_t)b->K);
-  A((void **)&c.d_lanes, sizeof(LaneOut) * (size_t)b->K);
-  H((void **)&c.h_lanes, sizeof(LaneOut) * (size_t)b->K);
+  {  // set-up block: masks, sources, lane table, plan (256-byte aligned parts)
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_src = al(sizeof(u64) * masks_words(b));
+    const size_t o_lanes = o_src + al(sizeof(int32_t) * (size_t)b->K);
+    const size_t o_plan = o_lanes + al(sizeof(LaneOut) * (size_t)b->K);
+    c.setup_bytes = o_plan + al(sizeof(Plan));
+    A((void **)&c.d_setup, c.setup_bytes);
+    H((void **)&c.h_setup, c.setup_bytes);
+    if (err == hipSuccess) {
+      c.d_masks = reinterpret_cast<u64 *>(c.d_setup);
+      c.h_masks = reinterpret_cast<u64 *>(c.h_setup);
+      c.d_src = reinterpret_cast<int32_t *>(c.d_setup + o_src);
+      c.h_src = reinterpret_cast<int32_t *>(c.h_setup + o_src);
+      c.d_lanes = reinterpret_cast<LaneOut *>(c.d_setup + o_lanes);
+      c.h_lanes = reinterpret_cast<LaneOut *>(c.h_setup + o_lanes);
+      c.d_plan = reinterpret_cast<Plan *>(c.d_setup + o_plan);
+      c.h_plan = reinterpret_cast<Plan *>(c.h_setup + o_plan);
+      std::memset(c.h_setup, 0, c.setup_bytes);
+    }
+  }
   A((void **)&c.d_al, sizeof(AdmitList));
   H((void **)&c.h_al, sizeof(AdmitList));
   if (b->cfg.tail_div > 0) {
@@ -318,13 +333,13 @@ int start_group(ibfs_batch_s *b, Ctx &c, const int32_t *src, int64_t g0, int k, 
     c.h_plan->tail_cap = c.tail_cap;
     c.h_plan->E = (u64)b->g->E;
   }
-  HIP_TRY(hipMemcpyAsync(c.d_masks, c.h_masks, sizeof(u64) * masks_words(b), hipMemcpyHostToDevice, c.st));
-  HIP_TRY(hipMemcpyAsync(c.d_src, c.h_src, sizeof(int32_t) * k, hipMemcpyHostToDevice, c.st));
-  HIP_TRY(hipMemcpyAsync(c.d_lanes, c.h_lanes, sizeof(LaneOut) * k, hipMemcpyHostToDevice, c.st));
-  HIP_TRY(hipMemcpyAsync(c.d_plan, c.h_plan, sizeof(Plan), hipMemcpyHostToDevice, c.st));
-  HIP_TRY(launch_fill_u64(slot(b, c, 0), b->g->Vt * W, 0ull, c.st));
-  HIP_TRY(launch_fill_u64(reinterpret_cast<u64 *>(c.flags), (b->g->Vt + 7) / 8, 0ull, c.st));
-  HIP_TRY(launch_fill_u64(reinterpret_cast<u64 *>(c.ctl), (int64_t)b->R * CTL_WORDS, 0ull, c.st));
+  // masks, sources, lane table and plan in one copy (lane entries past k are
+  // stale and never read: every kernel takes the group's k); then S_0, the
+  // flags and the control blocks zeroed in one launch
+  HIP_TRY(hipMemcpyAsync(c.d_setup, c.h_setup, c.setup_bytes, hipMemcpyHostToDevice, c.st));
+  HIP_TRY(launch_zero3_u64(slot(b, c, 0), b->g->Vt * W, reinterpret_cast<u64 *>(c.flags),
+                           (b->g->Vt + 7) / 8, reinterpret_cast<u64 *>(c.ctl),
+                           (int64_t)b->R * CTL_WORDS, c.st));
   HIP_TRY(launch_init_sources(W, c.d_src, k, 0, slot(b, c, 0), c.flags, c.st));
   HIP_TRY(launch_inspect(W, dd_inspect_args(b, c), c.st));
   HIP_TRY(launch_plan_decide(c.d_plan, c.ctl, b->R, c.hs, W, split_hubs(b->g, b->hc, b->cfg), c.st));

@@ -222,6 +222,9 @@ inline void clear_hip_error() { (void)hipGetLastError(); }
 
 // Launchers (all asynchronous on `st`).  W = status words (u64) per vertex.
 hipError_t launch_fill_u64(u64 *p, int64_t n, u64 value, hipStream_t st);
+// zeros in [p0, p0+n0), [p1, p1+n1) and [p2, p2+n2) (u64 words), one launch
+hipError_t launch_zero3_u64(u64 *p0, int64_t n0, u64 *p1, int64_t n1, u64 *p2, int64_t n2,
+                            hipStream_t st);
 // A control block posted to coherent pinned host memory: the block's words,
 // then `tag` with a system-scope release, so a host that reads the tag with
 // an acquire load sees the block without a copy command or an event.

@@ -15,6 +15,17 @@ __global__ void k_fill_u64(u64 *p, int64_t n, u64 value) {
     p[i] = value;
 }
 
+// Three ranges filled with zeros in one launch (a group's first snapshot,
+// changed flags and control blocks), over their concatenated index space.
+__global__ void k_zero3_u64(u64 *p0, int64_t n0, u64 *p1, int64_t n1, u64 *p2, int64_t n2) {
+  const int64_t n01 = n0 + n1, n = n01 + n2;
+  for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
+    if (i < n0) p0[i] = 0;
+    else if (i < n01) p1[i - n0] = 0;
+    else p2[i - n01] = 0;
+  }
+}
+
 // One wave: thread t < CTL_WORDS copies word t of the block (vector stores),
 // makes it visible system-wide, and after the barrier thread 0 releases the tag.
 __global__ void k_post_ctl(const LevelCtl *src, CtlPost *dst, u64 tag) {
@@ -1167,6 +1178,16 @@ __global__ __launch_bounds__(BS) void k_bu_model(const int64_t *__restrict__ beg
 hipError_t launch_fill_u64(u64 *p, int64_t n, u64 value, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, BS * 8, 8192)), dim3(BS), 0, st, p, n, value);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero3_u64(u64 *p0, int64_t n0, u64 *p1, int64_t n1, u64 *p2, int64_t n2,
+                            hipStream_t st) {
+  if (n0 < 0 || n1 < 0 || n2 < 0) return hipErrorInvalidValue;
+  const int64_t n = n0 + n1 + n2;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero3_u64, dim3(grid_for(n, BS * 8, 8192)), dim3(BS), 0, st, p0, n0, p1, n1,
+                     p2, n2);
   return hipGetLastError();
 }
 
