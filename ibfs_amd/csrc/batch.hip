@@ -30,7 +30,7 @@ using namespace ibfs;
 
 namespace {
 
-enum CtxState { IDLE = 0, RUNNING = 1, EMITTING = 2 };
+enum CtxState { IDLE = 0, RUNNING = 1 };
 
 struct Ctx {
   hipStream_t st = nullptr;
@@ -638,17 +638,20 @@ static int batch_run_impl(ibfs_batch_s *b, const int32_t *src, int64_t n, int32_
             if (!rc) rc = launch_burst(b, c, stats);
             if (stats) stats->n_tail_reruns++;
           } else if (c.h_plan->done) {
+            // the group's emission is enqueued; the context's next group can
+            // be enqueued behind it at once (stream order: its set-up copy
+            // and fills start after the emission has read the ring, the lane
+            // table and the tail log), and the final stream synchronisation
+            // waits for the last one
             rc = finish_group(b, c, c.h_plan->last, stats);
-            c.state = EMITTING;
+            c.state = IDLE;
+            ++finished;
           } else if (c.h_plan->level > V + 2) {
             rc = fail(IBFS_ERR_STATE, "traversal exceeded V+2 levels");
           } else {
             c.level = c.h_plan->level;
             rc = launch_burst(b, c, stats);
           }
-        } else {  // EMITTING finished
-          c.state = IDLE;
-          ++finished;
         }
       }
       if (rc) break;
