@@ -421,9 +421,10 @@ int start_ctx(ibfs_server_s *s, SCtx &c) {
   c.cohort = 0;
   c.dir_prev = -1;
   c.nf_prev = ~0ull;
-  HIP_CHK(launch_fill_u64(slot(s, c, 0), s->g->Vt * W, 0ull, c.st));
-  HIP_CHK(launch_fill_u64(reinterpret_cast<u64 *>(c.flags), (s->g->Vt + 7) / 8, 0ull, c.st));
-  HIP_CHK(launch_fill_u64(reinterpret_cast<u64 *>(c.ctl), (int64_t)s->R * CTL_WORDS, 0ull, c.st));
+  // S_0, the changed flags and the control blocks zeroed in one launch
+  HIP_CHK(launch_zero3_u64(slot(s, c, 0), s->g->Vt * W, reinterpret_cast<u64 *>(c.flags),
+                           (s->g->Vt + 7) / 8, reinterpret_cast<u64 *>(c.ctl),
+                           (int64_t)s->R * CTL_WORDS, c.st));
   std::vector<u64> A(W, 0), keep(W, ~0ull), nand(W, ~0ull);
   int n = 0;
   int rc = admit(s, c, 0, A.data(), &n);
@@ -709,9 +710,10 @@ int start_cohort(ibfs_server_s *s, SCtx &c) {
   pick_cohort(s, c.K);
   const int W = c.W;
   for (auto &l : c.lanes) l = Lane{};
-  HIP_CHK(launch_fill_u64(slot(s, c, 0), s->g->Vt * W, 0ull, c.st));
-  HIP_CHK(launch_fill_u64(reinterpret_cast<u64 *>(c.flags), (s->g->Vt + 7) / 8, 0ull, c.st));
-  HIP_CHK(launch_fill_u64(reinterpret_cast<u64 *>(c.ctl), (int64_t)s->R * CTL_WORDS, 0ull, c.st));
+  // S_0, the changed flags and the control blocks zeroed in one launch
+  HIP_CHK(launch_zero3_u64(slot(s, c, 0), s->g->Vt * W, reinterpret_cast<u64 *>(c.flags),
+                           (s->g->Vt + 7) / 8, reinterpret_cast<u64 *>(c.ctl),
+                           (int64_t)s->R * CTL_WORDS, c.st));
   // the cohort's first sources take lanes 0, 1, ...; with phase offsets a
   // lane whose source's two-hop size says it runs ahead of the others starts
   // at a later level (admitted then through the admission list)
