@@ -319,9 +319,8 @@ int traverse(ibfs_engine_s *e, const int32_t *src, int k, int32_t *dout, ibfs_st
   HIP_TRY(hipEventRecord(t_begin, e->st));
 
   // level 0: S_0 = source bits; inspection builds the level-1 queues
-  HIP_TRY(launch_fill_u64(slot(e, 0), Vt * W, 0ull, e->st));
-  HIP_TRY(launch_fill_u64(reinterpret_cast<u64 *>(e->flags), (Vt + 7) / 8, 0ull, e->st));
-  HIP_TRY(launch_fill_u64(reinterpret_cast<u64 *>(e->d_ctl), (int64_t)e->R * CTL_WORDS, 0ull, e->st));
+  HIP_TRY(launch_zero3_u64(slot(e, 0), Vt * W, reinterpret_cast<u64 *>(e->flags), (Vt + 7) / 8,
+                           reinterpret_cast<u64 *>(e->d_ctl), (int64_t)e->R * CTL_WORDS, e->st));
   HIP_TRY(launch_init_sources(W, e->d_src, k, 0, slot(e, 0), e->flags, e->st));
   hipEvent_t ti0 = nullptr, ti1 = nullptr;
   if (timing) { ti0 = next_event(e); HIP_TRY(hipEventRecord(ti0, e->st)); }
