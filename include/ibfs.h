@@ -167,6 +167,11 @@ int ibfs_get_wait_timeout(double *seconds);
 /* ---- device memory helpers (callers without a GPU framework) ------------- */
 int ibfs_device_alloc(int device, size_t bytes, void **dptr);
 int ibfs_device_free(void *dptr);
+/* Device-to-host copy into pageable memory (synchronous, on a library stream
+ * of the source's device).  Copies of 4 MiB or more stream through two
+ * 64-MiB pinned chunks while eight host threads copy into dst (the library
+ * keeps them for the process; such copies are serialised across threads).
+ * Host-placed depth arrays (IBFS_OUT_HOST) take the same path. */
 int ibfs_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int ibfs_memcpy_h2d(void *dst, const void *src, size_t bytes);
 /* Device-to-device copy (synchronous): moving depth arrays between device
