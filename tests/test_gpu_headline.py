@@ -35,9 +35,10 @@ def test_rmat_s22_1024_sources_plain_and_batched():
     """The headline workload (the reference's group loop, main.cu:76-90, at
     concurrent-count 128, readme.md:27): R-MAT s22 EF16 (seed 1, permuted
     ids), 1,024 sources (seed 2) in GroupBy order, as 8 groups of 128 lanes
-    on the engine and through the batched launcher.  Every one of the 1,024
-    depth arrays of both forms equals the oracle's, and the traversed-edge
-    counts of both forms equal the oracle's sum."""
+    on the engine and through the batched launcher (the tuned settings, then
+    512 lanes with the tuned scheduling, then 64 lanes unpacked with no tail).
+    Every one of the 1,024 depth arrays of every form equals the oracle's,
+    and the traversed-edge counts equal the oracle's sum."""
     th = cpu_threads()
     say(f"[gpu] headline: generating R-MAT s22 ({th} threads)")
     h = ibfs_amd.HostCsr.rmat(22, 16, 0.57, 0.19, 0.19, seed=1, permute=True, threads=th)
@@ -57,6 +58,18 @@ def test_rmat_s22_1024_sources_plain_and_batched():
     b.run(src, out=out_b)
     edges_b = b.last_stats.traversed_edges
     assert edges_b == edges_p
+    b.close()
+    out_w = ibfs_amd.DeviceBuffer(4 * n * V)
+    out_n = ibfs_amd.DeviceBuffer(4 * n * V)
+    say("[gpu] headline: batched 512 lanes; 64 lanes unpacked, no tail")
+    b = ibfs_amd.Batch(g, 2, BATCH_BURST, **dict(BATCH_OPTS, lane_bits=512))
+    b.run(src, out=out_w)
+    assert b.last_stats.traversed_edges == edges_p
+    b.close()
+    b = ibfs_amd.Batch(g, 3, 8, lane_bits=64, pack_sources=0, tail_div=0)
+    b.run(src, out=out_n)
+    assert b.last_stats.traversed_edges == edges_p
+    b.close()
     chunk = 64
     host = np.zeros(chunk * V, dtype=np.int32)
     want_edges = 0
@@ -64,7 +77,8 @@ def test_rmat_s22_1024_sources_plain_and_batched():
         say(f"[gpu] headline: oracle rows {i}..{i + chunk - 1}")
         w = oracle.bfs_many(beg, adj, src[i:i + chunk], threads=th)
         want_edges += sum(oracle.traversed_edges(beg, d) for d in w)
-        for form, buf in (("plain", out_p), ("batched", out_b)):
+        for form, buf in (("plain", out_p), ("batched", out_b), ("batched 512", out_w),
+                          ("batched 64", out_n)):
             got = buf.to_host(count=len(w) * V, offset=i * V, out=host).reshape(w.shape)
             if not np.array_equal(got, w):
                 bad = np.argwhere(got != w)
