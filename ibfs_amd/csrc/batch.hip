@@ -673,7 +673,21 @@ static int batch_run_impl(ibfs_batch_s *b, const int32_t *src, int64_t n, int32_
       for (auto &c : b->ctx) (void)hipStreamSynchronize(c.st);
     return rc;
   }
-  for (auto &c : b->ctx) HIP_TRY(hipStreamSynchronize(c.st));
+  // the last groups' emissions (enqueued behind their levels) end within the
+  // wait limit like every other wait of the scheduler
+  for (auto &c : b->ctx) HIP_TRY(hipEventRecord(c.ev, c.st));
+  for (auto &c : b->ctx) {
+    WaitClock wclk;
+    for (;;) {
+      const hipError_t q = hipEventQuery(c.ev);
+      if (q == hipSuccess) break;
+      if (q != hipErrorNotReady) return fail(IBFS_ERR_HIP, "context event: %s", hipGetErrorString(q));
+      if (wclk.expired())
+        return fail(IBFS_ERR_STATE, "emission did not finish within the wait limit (%.1f s)",
+                    wait_timeout_s());
+      std::this_thread::yield();
+    }
+  }
   if (stats) {
     stats->total_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
