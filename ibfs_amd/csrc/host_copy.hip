@@ -178,9 +178,11 @@ int copy_to_host(const D2HPiece *pieces, size_t n, hipStream_t st) {
     rc = wait_event(ev[last & 1]);
     if (rc == IBFS_OK) s.pool->copy(q[last].dst, s.pin + (last & 1) * kChunk, q[last].bytes);
   } else if (issued > 0) {
-    // an error part-way: the DMA already issued must end before another
-    // copy reuses the chunks
-    (void)hipStreamSynchronize(st);
+    // An error part-way (a failed copy or a wait past the limit): DMA already
+    // issued may still write the chunks, and waiting for it could outlast the
+    // limit.  The chunks are abandoned, never freed, and the next pipelined
+    // copy allocates new ones.
+    s.pin = nullptr;
   }
   for (hipEvent_t x : ev)
     if (x) (void)hipEventDestroy(x);

@@ -918,7 +918,8 @@ def test_host_copies_every_size_class():
     where copies start streaming through the pinned chunks (4 MiB), a 64-MiB
     chunk plus a few bytes, three chunks plus an odd tail; device sources
     at odd offsets, host destinations at odd addresses, nothing written past
-    the destination; then two threads copying at once."""
+    the destination; then two threads copying at once, and a copy whose wait
+    passes the limit part-way."""
     import threading
     say("[gpu] host copies")
     L = ibfs_amd._lib.lib()
@@ -948,4 +949,19 @@ def test_host_copies_every_size_class():
         t.join()
     for i in range(2):
         np.testing.assert_array_equal(got[i], pattern[64 * i:])
+    # a wait past the limit part-way through a pipelined copy (the limit set
+    # from the host, the DMA itself ends normally): the call fails with
+    # IBFS_ERR_STATE unless the copy won the race, and the next copy (on new
+    # pinned chunks) is exact
+    old = ibfs_amd.get_wait_timeout()
+    host = np.zeros(total, dtype=np.uint8)
+    ibfs_amd.set_wait_timeout(1e-9)
+    try:
+        rc = L.ibfs_memcpy_d2h(host.ctypes.data, buf.ptr, total)
+    finally:
+        ibfs_amd.set_wait_timeout(old)
+    assert rc in (0, ibfs_amd._lib.IBFS_ERR_STATE), rc
+    if rc == 0:
+        np.testing.assert_array_equal(host, pattern)
+    np.testing.assert_array_equal(buf.to_host(dtype=np.uint8), pattern)
     buf.free()
