@@ -108,6 +108,10 @@ struct ibfs_batch_s {
   u64 *d_edges = nullptr, *h_edges = nullptr;
   // closeness runs: per-source depth sums and reach counts instead of depths
   bool closeness = false;
+  // a run failed with work possibly still enqueued on the contexts' streams
+  // (a wait past the limit): the next run drains them first, since its
+  // set-up rewrites the pinned blocks that work may still copy into
+  bool unsettled = false;
   u64 *d_sum = nullptr, *d_reach = nullptr;
   int64_t sum_cap = 0;
 };
@@ -536,6 +540,25 @@ int ibfs_batch_destroy(ibfs_batch_t b) {
 
 }  // extern "C"
 
+// Waits, within the wait limit, for the work enqueued so far on every
+// context's stream.
+static int wait_contexts(ibfs_batch_s *b, const char *what) {
+  for (auto &c : b->ctx) HIP_TRY(hipEventRecord(c.ev, c.st));
+  for (auto &c : b->ctx) {
+    WaitClock clk;
+    for (;;) {
+      const hipError_t q = hipEventQuery(c.ev);
+      if (q == hipSuccess) break;
+      if (q != hipErrorNotReady) return fail(IBFS_ERR_HIP, "context event: %s", hipGetErrorString(q));
+      if (clk.expired())
+        return fail(IBFS_ERR_STATE, "%s did not finish within the wait limit (%.1f s)", what,
+                    wait_timeout_s());
+      std::this_thread::yield();
+    }
+  }
+  return IBFS_OK;
+}
+
 // Scheduler shared by depth runs and closeness runs (b->closeness).
 static int batch_run_impl(ibfs_batch_s *b, const int32_t *src, int64_t n, int32_t *depth_out,
                           int32_t out_place, ibfs_stats *stats) {
@@ -547,6 +570,10 @@ static int batch_run_impl(ibfs_batch_s *b, const int32_t *src, int64_t n, int32_
   if (n == 0) return IBFS_OK;
   HIP_TRY(hipSetDevice(b->g->device));
   clear_hip_error();
+  if (b->unsettled) {
+    if (int rc = wait_contexts(b, "the work of an earlier failed run")) return rc;
+    b->unsettled = false;
+  }
   if (out_place == IBFS_OUT_HOST) {
     for (auto &c : b->ctx)
       if (!c.d_stage) HIP_TRY(hipMalloc((void **)&c.d_stage, sizeof(int32_t) * (size_t)b->K * (size_t)V));
@@ -668,25 +695,20 @@ static int batch_run_impl(ibfs_batch_s *b, const int32_t *src, int64_t n, int32_
     }
   }
   if (rc) {
-    // a context that exceeded the wait limit is not waited for again
+    // a context that exceeded the wait limit is not waited for here; the
+    // next run drains it
     if (rc != IBFS_ERR_STATE)
       for (auto &c : b->ctx) (void)hipStreamSynchronize(c.st);
+    else
+      b->unsettled = true;
     return rc;
   }
   // the last groups' emissions (enqueued behind their levels) end within the
   // wait limit like every other wait of the scheduler
-  for (auto &c : b->ctx) HIP_TRY(hipEventRecord(c.ev, c.st));
-  for (auto &c : b->ctx) {
-    WaitClock wclk;
-    for (;;) {
-      const hipError_t q = hipEventQuery(c.ev);
-      if (q == hipSuccess) break;
-      if (q != hipErrorNotReady) return fail(IBFS_ERR_HIP, "context event: %s", hipGetErrorString(q));
-      if (wclk.expired())
-        return fail(IBFS_ERR_STATE, "emission did not finish within the wait limit (%.1f s)",
-                    wait_timeout_s());
-      std::this_thread::yield();
-    }
+  rc = wait_contexts(b, "emission");
+  if (rc) {
+    b->unsettled = true;
+    return rc;
   }
   if (stats) {
     stats->total_ms =

@@ -56,6 +56,10 @@ struct ibfs_engine_s {
   std::vector<hipEvent_t> ev;  // event pool
   size_t ev_used = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> emit_ev;  // emission timings of a traversal
+  // a traversal failed with work possibly still enqueued on st (a wait past
+  // the limit): the next one drains it first, since its set-up rewrites the
+  // pinned blocks that work may still copy into
+  bool unsettled = false;
 };
 
 namespace {
@@ -982,6 +986,11 @@ int ibfs_traversal(ibfs_engine_t e, const int32_t *src, int32_t k, int32_t *dept
   if (k == 0) return IBFS_OK;
   int rc = check_device(e->g->device);
   if (rc) return rc;
+  if (e->unsettled) {
+    rc = poll_stream(e->st, e->ev_wait);
+    if (rc) return rc;
+    e->unsettled = false;
+  }
   int32_t *dout = depth_out;
   if (out_place == IBFS_OUT_HOST) {
     rc = ensure_out(e, (size_t)k * (size_t)V);
@@ -989,12 +998,12 @@ int ibfs_traversal(ibfs_engine_t e, const int32_t *src, int32_t k, int32_t *dept
     dout = e->d_out;
   }
   rc = traverse(e, src, k, dout, stats);
-  if (rc) return rc;
-  if (out_place == IBFS_OUT_HOST) {
+  if (rc == IBFS_OK && out_place == IBFS_OUT_HOST) {
     const D2HPiece pc{depth_out, dout, sizeof(int32_t) * (size_t)k * (size_t)V};
-    return copy_to_host(&pc, 1, e->st);
+    rc = copy_to_host(&pc, 1, e->st);
   }
-  return IBFS_OK;
+  if (rc) e->unsettled = true;
+  return rc;
 }
 
 int ibfs_run_groups(ibfs_engine_t e, const int32_t *src, int64_t n, int32_t *depth_out,

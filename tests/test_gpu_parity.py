@@ -887,29 +887,43 @@ def test_engines_on_threads_share_a_graph(s16):
 
 def test_wait_limit_ends_a_wait_and_the_engine_recovers():
     """A host wait that exceeds the library's wait limit returns
-    IBFS_ERR_STATE instead of spinning; every kernel still runs to its end
-    (destroying the engine drains its stream), and with the limit restored a
-    new engine gives the oracle's depths.  The condition is forced from the
-    host with a 50-microsecond limit on a traversal of several milliseconds."""
+    IBFS_ERR_STATE instead of spinning; every kernel still runs to its end.
+    With the limit restored, the same engine and the same batch (whose next
+    run first drains the failed run's work) and a new engine give the
+    oracle's depths.  The condition is forced from the host with a tiny limit
+    on runs of several milliseconds."""
     say("[gpu] wait limit")
     h = ibfs_amd.HostCsr.rmat(20, 16, seed=3)
     beg, adj = h.arrays()
     g = ibfs_amd.DeviceGraph(h)
     src = h.gen_sources(512, seed=4)
+    want = oracle.bfs_many(beg, adj, src[:16], threads=8)
     old = ibfs_amd.get_wait_timeout()
     eng = ibfs_amd.Engine(g, lane_bits=512)
+    bat = ibfs_amd.Batch(g, lane_bits=256, contexts=2, burst=4)
+    buf = ibfs_amd.DeviceBuffer(4 * len(src) * h.vert_count)
     try:
-        ibfs_amd.set_wait_timeout(50e-6)
-        with pytest.raises(ibfs_amd.IbfsError) as ei:
-            eng.traversal(src)
-        assert ei.value.code == ibfs_amd._lib.IBFS_ERR_STATE
-        assert "wait limit" in str(ei.value)
+        try:
+            ibfs_amd.set_wait_timeout(50e-6)
+            with pytest.raises(ibfs_amd.IbfsError) as ei:
+                eng.traversal(src)
+            assert ei.value.code == ibfs_amd._lib.IBFS_ERR_STATE
+            assert "wait limit" in str(ei.value)
+            ibfs_amd.set_wait_timeout(1e-9)
+            with pytest.raises(ibfs_amd.IbfsError) as ei:
+                bat.run(src, out=buf)
+            assert ei.value.code == ibfs_amd._lib.IBFS_ERR_STATE
+            assert "wait limit" in str(ei.value)
+        finally:
+            ibfs_amd.set_wait_timeout(old)
+        np.testing.assert_array_equal(eng.traversal(src[:16]), want)
+        np.testing.assert_array_equal(bat.run(src[:16]), want)
     finally:
-        ibfs_amd.set_wait_timeout(old)
         eng.close()
+        bat.close()
+        buf.free()
     eng = ibfs_amd.Engine(g, lane_bits=512)
-    got = eng.traversal(src[:16])
-    np.testing.assert_array_equal(got, oracle.bfs_many(beg, adj, src[:16], threads=8))
+    np.testing.assert_array_equal(eng.traversal(src[:16]), want)
 
 
 def test_host_copies_every_size_class():
