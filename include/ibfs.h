@@ -158,9 +158,11 @@ int ibfs_device_count(int *n);
 /* Limit, in seconds, of every host wait of the library on device work
  * (process-wide; default 120).  A wait that exceeds it fails with
  * IBFS_ERR_STATE and a message naming the wait, instead of spinning on a
- * stream that never drains; the object it happened on should then be
- * destroyed.  The reference's host loop spins on a mapped flag with no limit
- * (traversal.cu:101-130). */
+ * stream that never drains.  An engine or batch whose call failed this way
+ * may be called again: that call first waits, within the limit, for the
+ * failed call's work to end (IBFS_ERR_STATE again if it has not).  Servers
+ * should be destroyed.  The reference's host loop spins on a mapped flag
+ * with no limit (traversal.cu:101-130). */
 int ibfs_set_wait_timeout(double seconds);
 int ibfs_get_wait_timeout(double *seconds);
 
