@@ -41,6 +41,14 @@ WORKLOADS = {
     "s16": dict(scale=16, sources=128),
 }
 
+# Plain-leg engine settings per workload: inspector workgroup size and the
+# degree classes' thresholds (the reference's gpu_ibfs takes sml_shed and
+# lrg_shed as constructor arguments, gpu_ibfs.cu:36-45).  For R-MAT s22 a sweep
+# of the three on 8 x 128 lanes took 15.12 ms per 1,024 sources with these
+# against 15.43-15.47 ms with the library defaults (512, 64, 512); other
+# workloads keep the defaults.
+PLAIN_SETTINGS = {"s22": dict(wg_size=1024, sml_shed=128, lrg_shed=2048)}
+
 
 def log(msg: str) -> None:
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
@@ -662,14 +670,15 @@ def main() -> int:
     plain = roof = kernels = levels = None
     verified = {}
     if not args.no_plain:
-        plain = plain_leg(g, src, K, args.steps, args.warmup, tune, emit_mode, out)
+        plain_tune = dict(PLAIN_SETTINGS.get(args.workload, {}), **tune)
+        plain = plain_leg(g, src, K, args.steps, args.warmup, plain_tune, emit_mode, out)
         log(f"plain: {plain['ms_per_step']:.2f} ms/step, {plain['gteps']:.1f} GTEPS "
             f"({plain['events_ms_per_step']:.2f} ms/step with kernel events)")
         if want is not None:  # untimed: the plain leg's own output
             if not verify_sample(out, sample_rows, want, V, "plain"):
                 return 4
             verified["plain"] = len(sample_rows)
-        roof, kernels, levels = roofline_leg(g, src, K, V, tune, plain["kern_ms"],
+        roof, kernels, levels = roofline_leg(g, src, K, V, plain_tune, plain["kern_ms"],
                                              plain["launches"], args.steps)
         roof["traffic"], roof["traffic_source"] = measured_traffic(args.workload, K, roof["kernel"])
         if roof["traffic"]:
@@ -775,6 +784,9 @@ def main() -> int:
                    "graph_seed": 1, "source_seed": 2, "permuted_ids": True,
                    "policy": (f"alpha_beta({plain['cfg'].alpha},{plain['cfg'].beta})"
                               if plain else None),
+                   "engine_settings": ({"wg_size": plain["cfg"].wg_size,
+                                        "sml_shed": plain["cfg"].sml_shed,
+                                        "lrg_shed": plain["cfg"].lrg_shed} if plain else None),
                    "depth_out": "device int32 [sources][V]"},
         "roofline": roof,
         "cpu_baseline": cpu_rec,

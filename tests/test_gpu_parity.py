@@ -65,6 +65,21 @@ def test_s16_128_sources(s16, lane_bits, pol):
     assert st.traversed_edges == want_edges
 
 
+@pytest.mark.parametrize("sml,lrg,wg", [(1, 1, 256), (32, 512, 512), (128, 2048, 1024),
+                                         (16384, 65536, 1024)])
+def test_s16_degree_class_thresholds(s16, sml, lrg, wg):
+    """The degree-class thresholds (thread / wave / workgroup classes of the
+    top-down frontier and the bottom-up candidates, inspector.cu:130-134) and
+    the inspector's workgroup size only move vertices between queues (1/1:
+    every vertex of degree above 1 in the workgroup class; 16384/65536: every
+    vertex in the thread class, the largest degree being 9,752),
+    and the depths equal the oracle's under each policy."""
+    for pol in ("ab", "td", "bu", "hy"):
+        say(f"[gpu] s16 thresholds sml={sml} lrg={lrg} wg={wg} policy={pol}")
+        eng = ibfs_amd.Engine(s16["g"], sml_shed=sml, lrg_shed=lrg, wg_size=wg, **POLICIES[pol])
+        np.testing.assert_array_equal(eng.run_groups(s16["src"]), s16["want"])
+
+
 def test_s16_fixed_switch_every_level(s16):
     src = s16["src"][:64]
     for sw in range(0, 8):
